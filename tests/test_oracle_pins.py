@@ -1,0 +1,219 @@
+"""Pin the oracle against mathematics that fixes the answer independently of any sign table.
+
+* one Pauli / Majorana gate on every operator basis element vs dense ``U^dag O U``;
+* random untruncated circuits vs dense matrices and vs statevector;
+* the reference's closed-form truncation table (tests/test_coeff_trunc.py:59-99, 142-227).
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import dense
+from monoprop_amd.circuit import Circuit, ExpGate, expand_monomials
+from monoprop_amd.majorana import MajoranaOperator
+from monoprop_amd.pauli import Pauli, PauliOperator
+
+
+def _evolve_oracle(oracle_mod, basis, n, init, gates, thetas, cutoff=None, **kw):
+    circ = Circuit(gates, n, thetas)
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    o = oracle_mod.Oracle(init, n, basis=basis, cutoff=cutoff if cutoff is not None else 2 * n,
+                          cutoff_type="support" if basis == "pauli" else "length", **kw)
+    o.propagate(monos, pm, gcs, circ.parameters)
+    return o
+
+
+def _dicts_close(a, b, tol=1e-12):
+    keys = set(a) | set(b)
+    for k in keys:
+        assert abs(a.get(k, 0) - b.get(k, 0)) <= tol, (k, a.get(k), b.get(k))
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_single_pauli_gate_all_strings(oracle_mod, n):
+    """Every (observable string, generator string) pair of n qubits against dense U^dag P U."""
+    theta = 0.37
+    strings = ["".join(s) for s in itertools.product("IXYZ", repeat=n)]
+    rng = np.random.default_rng(5)
+    gens = [s for s in strings if set(s) != {"I"}]
+    obs_all = [s for s in strings if set(s) != {"I"}]
+    if n == 3:
+        gens = list(rng.choice(gens, 12, replace=False))
+    for g in gens:
+        gate = ExpGate(PauliOperator({g: 1.0}, n))
+        G = dense.pauli_matrix({q: l for q, l in enumerate(g) if l != "I"}, n)
+        for o in obs_all:
+            letters = {q: l for q, l in enumerate(o) if l != "I"}
+            init = {dense.letters_to_slots(letters): 1.0}
+            orc = _evolve_oracle(oracle_mod, "pauli", n, init, [gate], [theta])
+            got = orc.evolved_operator(atol=0.0)
+            want = dense.pauli_decompose(dense.heisenberg(dense.pauli_matrix(letters, n), [G], [theta]), n)
+            # the store keeps every inserted term; drop exact zeros before comparing
+            got = {k: v for k, v in got.items() if abs(v) > 1e-13}
+            _dicts_close(got, want)
+
+
+def test_pauli_gen_coeff_scaling(oracle_mod):
+    """A generator coefficient g rotates by alpha = theta * g."""
+    n = 2
+    gate = ExpGate(PauliOperator({"XY": 0.6}, n))
+    init = {dense.letters_to_slots({0: "Z"}): 1.0}
+    orc = _evolve_oracle(oracle_mod, "pauli", n, init, [gate], [0.9])
+    G = 0.6 * dense.pauli_matrix({0: "X", 1: "Y"}, n)
+    want = dense.pauli_decompose(dense.heisenberg(dense.pauli_matrix({0: "Z"}, n), [G], [0.9]), n)
+    _dicts_close({k: v for k, v in orc.evolved_operator(0.0).items() if abs(v) > 1e-13}, want)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_pauli_circuit_untruncated(oracle_mod, seed):
+    rng = np.random.default_rng(seed)
+    n = 4
+    gates, thetas, gmats = [], [], []
+    for _ in range(14):
+        w = int(rng.integers(1, 4))
+        qs = sorted(rng.choice(n, w, replace=False).tolist())
+        letters = {int(q): str(rng.choice(list("XYZ"))) for q in qs}
+        gates.append(ExpGate(PauliOperator({Pauli("".join(letters[q] for q in qs), qs): 1.0}, n)))
+        gmats.append(dense.pauli_matrix(letters, n))
+        thetas.append(float(rng.uniform(-np.pi, np.pi)))
+    obs = {dense.letters_to_slots({1: "Z"}): 0.7, dense.letters_to_slots({0: "X", 3: "Y"}): -0.4}
+    omat = 0.7 * dense.pauli_matrix({1: "Z"}, n) - 0.4 * dense.pauli_matrix({0: "X", 3: "Y"}, n)
+    orc = _evolve_oracle(oracle_mod, "pauli", n, obs, gates, thetas)
+    want_m = dense.heisenberg(omat, gmats, thetas)
+    _dicts_close({k: v for k, v in orc.evolved_operator(0.0).items() if abs(v) > 1e-13},
+                 dense.pauli_decompose(want_m, n), tol=1e-11)
+    for occ in [(), (0, 2), (1, 2, 3)]:
+        orc2 = _evolve_oracle(oracle_mod, "pauli", n, obs, gates, thetas, initial_state=occ)
+        b = np.zeros(2**n)
+        b[sum(1 << (n - 1 - q) for q in occ)] = 1
+        assert abs(orc2.expectation() - (b @ want_m @ b).real) < 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_single_majorana_gate_all_monomials(oracle_mod, n):
+    """Weight-2 (imaginary) and weight-4 (real) Hermitian generators on every Majorana monomial."""
+    ms = dense.majorana_matrices(n)
+    theta = 0.41
+    gens = [((0, 1), 1j), ((1, 2), 1j), ((0, 3), -0.5j)]
+    if n >= 2:
+        gens.append(((0, 1, 2, 3), 1.0))
+    if n >= 3:
+        gens += [((1, 2, 4, 5), -1.0), ((2, 5), 1j), ((0, 1, 2), None)]
+    for gmono, gc in gens:
+        if gc is None:
+            continue
+        H = gc * dense.majorana_product(gmono, ms)
+        assert np.allclose(H, H.conj().T)
+        gate = ExpGate(MajoranaOperator({gmono: gc}, n))
+        for w in range(1, 2 * n + 1):
+            for mono in itertools.combinations(range(2 * n), w):
+                # a Hermitian observable term: coefficient i^{C(w,2)}
+                c0 = (1j) ** (w * (w - 1) // 2)
+                O = c0 * dense.majorana_product(mono, ms)
+                orc = _evolve_oracle(oracle_mod, "majorana", n, {mono: c0}, [gate], [theta])
+                got = {k: v for k, v in orc.evolved_operator(0.0).items() if abs(v) > 1e-13}
+                want = dense.majorana_decompose(dense.heisenberg(O, [H], [theta]), n)
+                _dicts_close(got, want)
+
+
+def test_majorana_odd_generator(oracle_mod):
+    """An odd-weight generator exercises the |G| odd parity correction of the anticommutation test."""
+    n = 2
+    ms = dense.majorana_matrices(n)
+    # i m0 m1 m2 is Hermitian? (i m0m1m2)^dag = -i m2 m1 m0 = -i (-1)^3 m0m1m2 = i m0m1m2
+    H = 1j * dense.majorana_product((0, 1, 2), ms)
+    assert np.allclose(H, H.conj().T)
+    gate_op = MajoranaOperator({(0, 1, 2): 1j}, n)
+    gate = ExpGate(gate_op)
+    theta = 0.3
+    for w in range(1, 5):
+        for mono in itertools.combinations(range(4), w):
+            c0 = (1j) ** (w * (w - 1) // 2)
+            O = c0 * dense.majorana_product(mono, ms)
+            orc = _evolve_oracle(oracle_mod, "majorana", n, {mono: c0}, [gate], [theta])
+            got = {k: v for k, v in orc.evolved_operator(0.0).items() if abs(v) > 1e-13}
+            want = dense.majorana_decompose(dense.heisenberg(O, [H], [theta]), n)
+            _dicts_close(got, want)
+
+
+def test_reference_truncation_table(oracle_mod):
+    """Closed-form truncation scenarios of the reference (tests/test_coeff_trunc.py:59-99)."""
+    n = 5
+    circ = Circuit.from_dense_arrays(initial_state=[], majoranas=[(1, 2, 3, 4)], parameters=[np.pi / 6],
+                                     gen_coeffs=[1.0], param_inds=[0], system_size=n)
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    cos_pi6, sin_pi6 = (0.5, -0.8660254037844386)
+    act = {(0, 1): 1j * cos_pi6, (0, 2, 3, 4): sin_pi6}
+    trunc = {(0, 1): 1j * cos_pi6}
+    for cutoff, upper, lower, want in [(4, None, None, act), (1, 0.49, None, act), (1, 1.0, None, trunc),
+                                       (4, None, 0.9, trunc), (10, None, 0.1, act)]:
+        o = oracle_mod.Oracle({(0, 1): 1j}, n, basis="majorana", cutoff=cutoff, cutoff_type="length",
+                              lower_atol=lower, upper_atol=upper)
+        o.propagate(monos, pm, gcs, circ.parameters)
+        got = o.evolved_operator()
+        assert set(got) == set(want)
+        for k in want:
+            assert np.isclose(got[k], want[k])
+
+
+@pytest.mark.parametrize("upper,lower", [(None, None), (None, 1e-6), (None, 1e-10), (1e-4, None), (1e-4, 1e-6),
+                                         (1e-4, 1e-10), (1e-10, None), (1e-10, 1e-10)])
+def test_reference_small_coeff_cases(oracle_mod, upper, lower):
+    """tests/test_coeff_trunc.py:169-227: both endpoints present, a tiny one and a large one."""
+    n, p = 2, 1e-5
+    init = {(0, 1): 1.0j, (0, 2): 1e-7j}
+    circ = Circuit.from_dense_arrays(initial_state=[], majoranas=[(1, 2)], parameters=[p], gen_coeffs=[1.0],
+                                     param_inds=[0], system_size=n)
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    want = {(0, 1): init[(0, 1)] * np.cos(2 * p) - init[(0, 2)] * np.sin(2 * p),
+            (0, 2): init[(0, 2)] * np.cos(2 * p) + init[(0, 1)] * np.sin(2 * p)}
+    o = oracle_mod.Oracle(init, n, basis="majorana", cutoff=4, cutoff_type="length", lower_atol=lower,
+                          upper_atol=upper)
+    o.propagate(monos, pm, gcs, circ.parameters)
+    got = o.evolved_operator()
+    assert o.size() == 2
+    assert set(got) == set(want)
+    for k in want:
+        assert np.isclose(got[k], want[k])
+
+
+def test_z_only_partner_exempt_from_weight_cutoff(oracle_mod):
+    """A fully paired (Z-only) partner is kept at any weight (AlgebraCommon.h:318-342)."""
+    n = 4
+    # X0 Z1 Z2 Z3 rotated by X0 -> partner Y0 Z1 Z2 Z3 ... use Y0 generator on Z0 Z1 Z2 Z3: partner X0Z1Z2Z3
+    # Instead: observable X0 Z1 Z2 Z3, generator Y0: partner = Z0 Z1 Z2 Z3 (Z-only, weight 4 > cutoff 1).
+    init = {dense.letters_to_slots({0: "X", 1: "Z", 2: "Z", 3: "Z"}): 1.0}
+    gate = ExpGate(PauliOperator({Pauli("Y", (0,)): 1.0}, n))
+    orc = _evolve_oracle(oracle_mod, "pauli", n, init, [gate], [0.3], cutoff=1)
+    got = orc.evolved_operator(0.0)
+    assert dense.letters_to_slots({q: "Z" for q in range(4)}) in got
+    # and a non-paired weight-4 partner is cut: observable Z0 Z1 Z2 Z3, generator X0 -> Y0 Z1 Z2 Z3
+    init2 = {dense.letters_to_slots({q: "Z" for q in range(4)}): 1.0}
+    gate2 = ExpGate(PauliOperator({Pauli("X", (0,)): 1.0}, n))
+    orc2 = _evolve_oracle(oracle_mod, "pauli", n, init2, [gate2], [0.3], cutoff=3)
+    assert orc2.size() == 1
+
+
+def test_statevector_c1_short(oracle_mod):
+    """Kicked-Ising chain, 12 qubits, 2 steps, untruncated: oracle <Z_6> equals the statevector."""
+    n, steps = 12, 2
+    tj, th = -np.pi / 4, 0.3
+    gates, thetas, sv_gates = [], [], []
+    for _ in range(steps):
+        for i in range(n - 1):
+            gates.append(ExpGate(PauliOperator({Pauli("ZZ", (i, i + 1)): 1.0}, n)))
+            thetas.append(tj)
+            sv_gates.append({i: "Z", i + 1: "Z"})
+        for i in range(n):
+            gates.append(ExpGate(PauliOperator({Pauli("X", (i,)): 1.0}, n)))
+            thetas.append(th)
+            sv_gates.append({i: "X"})
+    obs = {dense.letters_to_slots({6: "Z"}): 1.0}
+    orc = _evolve_oracle(oracle_mod, "pauli", n, obs, gates, thetas)
+    want = dense.statevector_expectation(n, [({6: "Z"}, 1.0)], sv_gates, thetas)
+    assert abs(orc.expectation() - want) < 1e-12
