@@ -1,0 +1,107 @@
+/*
+ * monoprop_amd -- C ABI of the MI355X operator-propagation engine.
+ *
+ * One opaque handle per propagator; plain pointers and sizes only.  Every function returns 0 on
+ * success and a negative code on failure; mpa_last_error() then names the failure (thread-local).
+ * Codes: -1 invalid argument (reference ValueError), -2 runtime failure (reference RuntimeError),
+ * -3 HIP/device failure.
+ *
+ * Encodings at the boundary are the reference's: operators and generators are CSR lists of slot
+ * tuples (Pauli: X_q -> 2q, Y_q -> 2q+1, Z_q -> {2q, 2q+1}; Majorana: the index itself), with
+ * complex coefficients for operators (re, im arrays) and real structural coefficients for
+ * generators.  `lower_atol` / `upper_atol` take NaN for "None".  Layer arrays are in authoring
+ * order; the engine applies them back to front (Heisenberg picture).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef MONOPROP_AMD_H
+#define MONOPROP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpa_engine mpa_engine;
+
+enum { MPA_BASIS_MAJORANA = 0, MPA_BASIS_PAULI = 1 };
+enum { MPA_CUTOFF_LENGTH = 0, MPA_CUTOFF_SUPPORT = 1 };
+
+/* Last error message of the calling thread ("" if none). */
+const char *mpa_last_error(void);
+
+/* Replaces MonomialPropagator<N>::MonomialPropagator (reference
+ * cpp/monoprop/detail/monomial_propagator/MonomialPropagator.inl:73-188, bound at
+ * src/monoprop/bindings/binder.h:60-100).  Terms are taken in lexicographic slot-tuple order; the
+ * empty tuple is the core (identity) term. */
+int mpa_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type,
+               double lower_atol, double upper_atol, const int64_t *term_offsets,
+               const int32_t *term_slots, const double *coeff_re, const double *coeff_im,
+               int64_t n_terms, const int32_t *initial_state, int64_t n_state, int device);
+
+/* Releases all device memory of the engine. */
+int mpa_destroy(mpa_engine *e);
+
+/* Replaces MonomialPropagator::propagate (MonomialPropagator.inl:692-733, binder.h:118-125):
+ * contract-immediately Heisenberg evolution, alpha_i = params[param_map[i]] * gen_coeffs[i].
+ * only_rotate_len_k < 0 means None. */
+int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots,
+                  int64_t n_layers, const int64_t *param_map, const double *gen_coeffs,
+                  const double *params, int64_t n_params, int32_t only_rotate_len_k);
+
+/* Replaces MonomialPropagator::build_graph (MonomialPropagator.inl:620-690, binder.h:102-110).
+ * params == NULL builds structurally; otherwise the build is coefficient-seeded at params. */
+int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots,
+                    int64_t n_layers, const int64_t *param_map, const double *gen_coeffs,
+                    const double *params, int64_t n_params);
+
+/* Replaces MonomialPropagator::expectation_value (MonomialPropagator.inl:1034-1041,
+ * binder.h:127-130): replays the stored graph at params (n_params must equal the graph's
+ * parameter count; 0 with no graph). */
+int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out);
+
+/* Replaces MonomialPropagator::expectation_value_and_gradient (MonomialPropagator.inl:1043-1050,
+ * MPFunctions.cpp:207-246): value and d<O>/d params[p] for every p. */
+int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int64_t n_params,
+                                       double *value, double *grad);
+
+/* Replaces MonomialPropagator::size (binder.h:233): number of stored terms. */
+int mpa_size(mpa_engine *e, int64_t *out);
+
+/* Graph metadata (binder.h:235-240): stored layers and the parameter count they reference. */
+int mpa_graph_layers(mpa_engine *e, int64_t *layers, int64_t *n_params);
+
+/* Replaces MonomialPropagator::evolved_operator_terms (MonomialPropagator.inl:1099-1127) plus the
+ * core-term rule of binder.h:200-220: terms with |c| >= atol, decoded to complex and rounded to
+ * 1e-12.  Two-call protocol: call with slots == NULL to get n_terms / n_slots, then again with
+ * buffers of those sizes (offsets has n_terms + 1 entries).  params as in mpa_expectation_value. */
+int mpa_export_terms(mpa_engine *e, const double *params, int64_t n_params, double atol,
+                     int64_t *n_terms, int64_t *n_slots, int64_t *offsets, int32_t *slots,
+                     double *re, double *im);
+
+/* Raw store in insertion order: words_per_row u64 words per row (Pauli: x-plane limbs then z-plane
+ * limbs, qubit q at bit q; Majorana: slot s at bit s) and the current coefficients.  No counterpart
+ * in the reference (its store is private); used by parity tests. */
+int mpa_words_per_row(mpa_engine *e, int32_t *out);
+int mpa_read_rows(mpa_engine *e, uint64_t *words, double *coeffs);
+
+/* Per-gate counters of the last propagate, in processing order: live terms before the gate,
+ * anticommuting terms, emitted partners, inserted terms.  `len` in/out. */
+int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t *em, int64_t *ins);
+
+/* Device-side kernel time of the last propagate (ms, HIP events) per kernel class:
+ * [0] column scan + compaction, [1] classify/match/apply, [2] insert + index update,
+ * [3] everything; and the algorithmic bytes of the same kernels. */
+int mpa_kernel_timing(mpa_engine *e, double *ms, double *bytes);
+
+/* Replaces the cutoff / cutoff_type / lower_atol / upper_atol setters (binder.h:158-189); applies to
+ * gates propagated afterwards. */
+int mpa_set_truncation(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower_atol,
+                       double upper_atol);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MONOPROP_AMD_H */

@@ -1,0 +1,394 @@
+"""ctypes binding of ``libmonoprop_amd.so`` and the compiled-core adapter class.
+
+``DeviceSimulator`` mirrors the reference's nanobind class ``MonomialPropagatorNNN``
+(src/monoprop/bindings/binder.h:53-274): same constructor keywords, methods, properties and error
+types (``ValueError`` for invalid arguments, ``RuntimeError`` for engine failures).  It is the only
+place the C ABI (include/monoprop_amd.h) is called from Python.
+
+The library must be built (``python -m monoprop_amd.build``); a missing library raises instead of
+falling back to any CPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from collections.abc import Mapping, Sequence
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libmonoprop_amd.so"
+_LIB = None
+
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_P = C.c_void_p
+
+# name -> (restype, argtypes); every function returns int status unless stated
+_SIGNATURES = {
+    "mpa_last_error": (C.c_char_p, []),
+    "mpa_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double, C.c_double,
+                             _i64p, _i32p, _f64p, _f64p, C.c_int64, _i32p, C.c_int64, C.c_int]),
+    "mpa_destroy": (C.c_int, [_P]),
+    "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
+    "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
+    "mpa_expectation_value": (C.c_int, [_P, _f64p, C.c_int64, _f64p]),
+    "mpa_expectation_value_and_gradient": (C.c_int, [_P, _f64p, C.c_int64, _f64p, _f64p]),
+    "mpa_size": (C.c_int, [_P, _i64p]),
+    "mpa_graph_layers": (C.c_int, [_P, _i64p, _i64p]),
+    "mpa_export_terms": (C.c_int, [_P, _f64p, C.c_int64, C.c_double, _i64p, _i64p, _i64p, _i32p, _f64p, _f64p]),
+    "mpa_words_per_row": (C.c_int, [_P, _i32p]),
+    "mpa_read_rows": (C.c_int, [_P, _u64p, _f64p]),
+    "mpa_gate_stats": (C.c_int, [_P, _i64p, _i64p, _i64p, _i64p, _i64p]),
+    "mpa_kernel_timing": (C.c_int, [_P, _f64p, _f64p]),
+    "mpa_set_truncation": (C.c_int, [_P, C.c_uint32, C.c_int, C.c_double, C.c_double]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load_library():
+    """Load the HIP engine; raise ``ImportError`` if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not _LIB_PATH.exists():
+            raise ImportError(
+                f"monoprop_amd engine library not found at {_LIB_PATH}; build it with `python -m monoprop_amd.build`."
+            )
+        lib = C.CDLL(str(_LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = load_library().mpa_last_error().decode()
+    if rc == -1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _csr(tuples: Sequence[Sequence[int]]):
+    offsets = np.zeros(len(tuples) + 1, dtype=np.int64)
+    flat: list[int] = []
+    for k, t in enumerate(tuples):
+        for i in t:
+            i = int(i)
+            if i < 0:
+                raise ValueError(f"Majorana/Pauli index {i} is out of range; must be non-negative.")
+            flat.append(i)
+        offsets[k + 1] = len(flat)
+    return offsets, np.asarray(flat if flat else [0], dtype=np.int32)
+
+
+def _f64(xs) -> np.ndarray:
+    a = np.asarray(list(xs), dtype=np.float64)
+    return a if a.size else np.zeros(1, dtype=np.float64)
+
+
+def _nan_if_none(x) -> float:
+    return math.nan if x is None else float(x)
+
+
+def default_device() -> int:
+    return int(os.environ.get("MONOPROP_AMD_DEVICE", "0"))
+
+
+class DeviceSimulator:
+    """HIP-engine counterpart of the reference's compiled ``MonomialPropagatorNNN``."""
+
+    storage_num_modes = 256
+
+    def __init__(
+        self,
+        initial_operator: Mapping[Sequence[int], complex],
+        cutoff: int,
+        initial_state: Sequence[int],
+        comm=None,
+        schrodinger_cutoff: int | None = None,
+        lower_atol: float | None = None,
+        upper_atol: float | None = None,
+        cutoff_type: str = "length",
+        basis_change=None,
+        logical_num_modes: int | None = None,
+        basis: str = "majorana",
+        partitions: int = 0,
+    ) -> None:
+        lib = load_library()
+        if comm is not None:
+            raise ValueError("MPI communicators are not supported: the engine holds the whole operator on one GPU.")
+        if schrodinger_cutoff is not None:
+            raise ValueError("The Schrodinger picture is not supported by the HIP engine (Heisenberg only).")
+        if basis_change is not None:
+            raise ValueError("basis_change is not supported by the HIP engine.")
+        if basis not in ("majorana", "pauli"):
+            raise ValueError(f"Unknown basis {basis!r}; expected 'majorana' or 'pauli'.")
+        if cutoff_type not in ("length", "support"):
+            raise ValueError(f"Unknown cutoff type {cutoff_type!r}; expected 'length' or 'support'.")
+        if isinstance(cutoff, bool) or int(cutoff) < 0:
+            raise ValueError(f"cutoff must be a non-negative integer; got {cutoff!r}.")
+        n = int(logical_num_modes) if logical_num_modes is not None else self.storage_num_modes
+        keys = [tuple(int(i) for i in k) for k in initial_operator]
+        vals = [complex(initial_operator[k]) for k in initial_operator]
+        offs, slots = _csr(keys)
+        re = _f64(v.real for v in vals)
+        im = _f64(v.imag for v in vals)
+        st = np.asarray([int(i) for i in initial_state] or [0], dtype=np.int32)
+        self._basis = basis
+        self._num_modes = n
+        self._cutoff = int(cutoff)
+        self._cutoff_type = cutoff_type
+        self._lower = lower_atol
+        self._upper = upper_atol
+        h = _P()
+        _check(lib.mpa_create(C.byref(h), 1 if basis == "pauli" else 0, n, int(cutoff),
+                              0 if cutoff_type == "length" else 1, _nan_if_none(lower_atol),
+                              _nan_if_none(upper_atol), _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
+                              _ptr(re, C.c_double), _ptr(im, C.c_double), len(keys), _ptr(st, C.c_int32),
+                              len(list(initial_state)), default_device()))
+        self._h = h
+        self._parameter_mapping: list[int] = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _LIB is not None:
+            _LIB.mpa_destroy(h)
+            self._h = None
+
+    # --- evolution -------------------------------------------------------------------------------
+    @staticmethod
+    def _layers(majoranas, parameter_mapping, gen_coeffs):
+        if len(parameter_mapping) != len(majoranas) or len(gen_coeffs) != len(majoranas):
+            raise ValueError(
+                f"parameter_mapping ({len(parameter_mapping)}) and gen_coeffs ({len(gen_coeffs)}) must have one "
+                f"entry per monomial ({len(majoranas)})."
+            )
+        offs, slots = _csr(majoranas)
+        pm = np.asarray([int(p) for p in parameter_mapping] or [0], dtype=np.int64)
+        gc = _f64(gen_coeffs)
+        return offs, slots, pm, gc
+
+    def propagate(self, majoranas, parameter_mapping, gen_coeffs, parameters, only_rotate_len_k=None) -> None:
+        if not len(majoranas):
+            return
+        if self.graph_layers() > 0:
+            raise RuntimeError(
+                f"Cannot propagate() on top of a non-empty graph of {self.graph_layers()} layer(s): propagate() "
+                "evolves and contracts in place and assumes no stored graph."
+            )
+        offs, slots, pm, gc = self._layers(majoranas, parameter_mapping, gen_coeffs)
+        params = list(parameters)
+        need = (max(int(p) for p in parameter_mapping) + 1) if len(parameter_mapping) else 0
+        if len(params) < need:
+            raise ValueError(f"Expected at least {need} parameter value(s), got {len(params)}.")
+        th = _f64(params)
+        k = -1 if only_rotate_len_k is None else int(only_rotate_len_k)
+        _check(load_library().mpa_propagate(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(majoranas),
+                                            _ptr(pm, C.c_int64), _ptr(gc, C.c_double), _ptr(th, C.c_double),
+                                            len(params), k))
+
+    def build_graph(self, majoranas, parameter_mapping, gen_coeffs, gate_indices=None, parameters=None,
+                    only_rotate_len_k=None) -> None:
+        if not len(majoranas):
+            return
+        if only_rotate_len_k is not None:
+            raise ValueError("only_rotate_len_k is supported by propagate() only.")
+        offs, slots, pm, gc = self._layers(majoranas, parameter_mapping, gen_coeffs)
+        if parameters is None:
+            th_ptr, n = None, 0
+        else:
+            th = _f64(parameters)
+            th_ptr, n = _ptr(th, C.c_double), len(list(parameters))
+        _check(load_library().mpa_build_graph(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(majoranas),
+                                              _ptr(pm, C.c_int64), _ptr(gc, C.c_double), th_ptr, n))
+        self._parameter_mapping = list(self._parameter_mapping) + [int(p) for p in reversed(list(parameter_mapping))]
+
+    # --- evaluation ------------------------------------------------------------------------------
+    def expectation_value(self, parameters) -> float:
+        th = _f64(parameters)
+        out = C.c_double()
+        _check(load_library().mpa_expectation_value(self._h, _ptr(th, C.c_double), len(list(parameters)),
+                                                    C.byref(out)))
+        return out.value
+
+    def expectation_value_and_gradient(self, parameters):
+        params = list(parameters)
+        th = _f64(params)
+        g = np.zeros(max(len(params), 1), dtype=np.float64)
+        out = C.c_double()
+        _check(load_library().mpa_expectation_value_and_gradient(self._h, _ptr(th, C.c_double), len(params),
+                                                                 C.byref(out), _ptr(g, C.c_double)))
+        return out.value, list(g[: len(params)])
+
+    def expectation_value_functional(self, pare_threshold=None):
+        if pare_threshold is not None:
+            raise ValueError("graph paring is not supported by the HIP engine.")
+        return lambda params: self.expectation_value(params)
+
+    def expectation_value_and_gradient_functional(self, pare_threshold=None):
+        if pare_threshold is not None:
+            raise ValueError("graph paring is not supported by the HIP engine.")
+        return lambda params: self.expectation_value_and_gradient(params)
+
+    def evolved_operator(self, parameters, atol: float) -> dict:
+        lib = load_library()
+        params = list(parameters)
+        th = _f64(params)
+        nt, ns = C.c_int64(), C.c_int64()
+        _check(lib.mpa_export_terms(self._h, _ptr(th, C.c_double), len(params), float(atol), C.byref(nt),
+                                    C.byref(ns), None, None, None, None))
+        offs = np.zeros(nt.value + 1, dtype=np.int64)
+        slots = np.zeros(max(ns.value, 1), dtype=np.int32)
+        re = np.zeros(max(nt.value, 1), dtype=np.float64)
+        im = np.zeros(max(nt.value, 1), dtype=np.float64)
+        _check(lib.mpa_export_terms(self._h, _ptr(th, C.c_double), len(params), float(atol), C.byref(nt),
+                                    C.byref(ns), _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
+                                    _ptr(re, C.c_double), _ptr(im, C.c_double)))
+        out = {}
+        for k in range(nt.value):
+            out[tuple(int(s) for s in slots[offs[k]:offs[k + 1]])] = complex(re[k], im[k])
+        return out
+
+    # --- introspection ---------------------------------------------------------------------------
+    def size(self) -> int:
+        v = C.c_int64()
+        _check(load_library().mpa_size(self._h, C.byref(v)))
+        return v.value
+
+    def graph_layers(self) -> int:
+        layers, n = C.c_int64(), C.c_int64()
+        _check(load_library().mpa_graph_layers(self._h, C.byref(layers), C.byref(n)))
+        return layers.value
+
+    def n_gates(self) -> int:
+        return 0 if self.graph_layers() == 0 else len(self._parameter_mapping)
+
+    def graph_size(self) -> tuple[int, int]:
+        return (0, 0)
+
+    def rows(self):
+        """Raw store: ``(words[N, nw] uint64, coeffs[N] float64)`` in insertion order."""
+        lib = load_library()
+        nw = C.c_int32()
+        _check(lib.mpa_words_per_row(self._h, C.byref(nw)))
+        n = self.size()
+        w = np.zeros((max(n, 1), nw.value), dtype=np.uint64)
+        c = np.zeros(max(n, 1), dtype=np.float64)
+        _check(lib.mpa_read_rows(self._h, _ptr(w, C.c_uint64), _ptr(c, C.c_double)))
+        return w[:n], c[:n]
+
+    def gate_stats(self):
+        """Per processed gate: (live terms before, anticommuting, emitted, inserted)."""
+        lib = load_library()
+        ln = C.c_int64(0)
+        _check(lib.mpa_gate_stats(self._h, C.byref(ln), None, None, None, None))
+        arrs = [np.zeros(max(ln.value, 1), dtype=np.int64) for _ in range(4)]
+        _check(lib.mpa_gate_stats(self._h, C.byref(ln), *[_ptr(a, C.c_int64) for a in arrs]))
+        return tuple(a[: ln.value] for a in arrs)
+
+    def kernel_timing(self):
+        ms = np.zeros(4, dtype=np.float64)
+        by = np.zeros(4, dtype=np.float64)
+        _check(load_library().mpa_kernel_timing(self._h, _ptr(ms, C.c_double), _ptr(by, C.c_double)))
+        return ms, by
+
+    # --- truncation properties (binder.h:158-189) ------------------------------------------------
+    def _push_truncation(self):
+        _check(load_library().mpa_set_truncation(self._h, self._cutoff, 0 if self._cutoff_type == "length" else 1,
+                                                 _nan_if_none(self._lower), _nan_if_none(self._upper)))
+
+    @property
+    def cutoff(self) -> int:
+        return self._cutoff
+
+    @cutoff.setter
+    def cutoff(self, v: int) -> None:
+        self._cutoff = int(v)
+        self._push_truncation()
+
+    @property
+    def lower_atol(self):
+        return self._lower
+
+    @lower_atol.setter
+    def lower_atol(self, v) -> None:
+        self._lower = v
+        self._push_truncation()
+
+    @property
+    def upper_atol(self):
+        return self._upper
+
+    @upper_atol.setter
+    def upper_atol(self, v) -> None:
+        self._upper = v
+        self._push_truncation()
+
+    @property
+    def cutoff_type(self) -> str:
+        return self._cutoff_type
+
+    @cutoff_type.setter
+    def cutoff_type(self, v: str) -> None:
+        if v not in ("length", "support"):
+            raise ValueError(f"Unknown cutoff type {v!r}; expected 'length' or 'support'.")
+        old = self._cutoff_type
+        self._cutoff_type = v
+        try:
+            self._push_truncation()
+        except Exception:
+            self._cutoff_type = old
+            raise
+
+    @property
+    def basis_change(self):
+        return None
+
+    @property
+    def schrodinger(self) -> bool:
+        return False
+
+    @property
+    def basis(self) -> str:
+        return self._basis
+
+    @property
+    def num_modes(self) -> int:
+        return self._num_modes
+
+    @property
+    def parameter_mapping(self) -> list[int]:
+        return list(self._parameter_mapping)
+
+    @parameter_mapping.setter
+    def parameter_mapping(self, mapping) -> None:
+        self._parameter_mapping = [int(m) for m in mapping]
+
+
+def dispatch(num_modes: int):
+    """Adapter factory for a mode count (reference tools/generate-dispatch.py:145-190)."""
+    if num_modes <= 0 or num_modes > DeviceSimulator.storage_num_modes:
+        raise ValueError(f"num_modes={num_modes} is out of range (1..{DeviceSimulator.storage_num_modes}).")
+
+    def make(**kwargs):
+        kwargs.setdefault("logical_num_modes", num_modes)
+        return DeviceSimulator(**kwargs)
+
+    return make

@@ -1,0 +1,1250 @@
+// monoprop_amd engine: Heisenberg operator propagation on one MI355X (gfx950).
+//
+// Term store in HBM (append-only, insertion order = the reference's row order):
+//   rows  : AoS, NW u64 words per term (layout in algebra.h)            -- gathered per anticommuting term
+//   coef  : f64 per term
+//   cols  : bit-sliced column index, one u64 per 64 terms per column     -- streamed by the anticommutation
+//           Pauli: x-plane qubit q -> column q, z-plane -> column n+q;    scan (reference inverted index,
+//           Majorana: slot s -> column s, popcount parity -> column 2n.   detail/operator/InvertedIndex.h)
+//
+// Per gate G (reference build_layer + apply_fused_contract, layer_build/Engine.h:522-637,
+// FusedApply.h:32-83), all on one stream:
+//   k_scan       XOR of G's fold columns -> anticommuting masks, split by G's pivot bit into leaders
+//                (pivot clear) and followers (pivot set) (Scan.h:125-185, 265-269)
+//   k_scan_sums  exclusive scan of per-block counts
+//   k_expand     masks -> ascending item list AL = [leaders..., followers...]
+//   (host reads nL, nF)
+//   k_leaders    classify leaders (emission gate, partner phase), insert their monomial into a per-gate
+//                open-addressing hash table keyed by the full monomial
+//   k_followers  classify followers, look up their partner M^G among the leaders: a term's partner
+//                exists in the store iff it is an anticommuting leader, so no global index is needed.
+//                The follower thread applies the whole 2x2 update of a pair (Engine.h:465-513 net rule:
+//                rotated iff either endpoint emits, the leader's emission taking precedence).
+//   k_leaders_unmatched  cos-only / insert arm of leaders without a partner
+//   k_count_miss, k_scan_sums, k_insert   stable compaction of partners to create (leader misses then
+//                follower misses, Engine.h:428-444) appended at N..N+I
+//   k_cols_update  wave64 ballots transpose the new rows into the column index
+//   k_advance    N += I and the per-gate counters
+// Floating point follows the reference's operation order; the file is compiled with
+// -ffp-contract=off so nothing is fused into FMA.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/monoprop_amd.h"
+#include "algebra.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kScanBlock = 1024;
+constexpr int kExpectBlocks = 1024;
+
+// counters[] on the device
+enum : int { C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_ERR = 5, C_COUNT = 8 };
+
+thread_local std::string g_error;
+
+int set_error(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            throw HipFailure(std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+    } while (0)
+
+struct HipFailure {
+    std::string what;
+    explicit HipFailure(std::string w) : what(std::move(w)) {}
+};
+struct ArgError {
+    std::string what;
+    explicit ArgError(std::string w) : what(std::move(w)) {}
+};
+struct RunError {
+    std::string what;
+    explicit RunError(std::string w) : what(std::move(w)) {}
+};
+
+// ------------------------------------------------------------------------------------------------
+// device helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Inclusive scan across one wavefront of 64 lanes.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= static_cast<uint32_t>(d)) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan across a block of kBlock threads; returns the exclusive prefix, total in *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, uint32_t *total) {
+    const uint32_t incl = wave_incl_scan(v);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 63) lds[wave] = incl;
+    __syncthreads();
+    uint32_t wave_base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint32_t s = lds[w];
+        if (static_cast<uint32_t>(w) < wave) wave_base += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_base + incl - v;
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds) {
+    uint32_t tot;
+    (void)block_excl_scan(v, lds, &tot);
+    return tot;
+}
+
+// ------------------------------------------------------------------------------------------------
+// per-gate constants
+
+struct GateDesc {
+    uint64_t g[mpa::kMaxWords];
+    double k;       // cos(2 alpha)
+    double s;       // sin(2 alpha)
+    double inv_k;   // 1 / cos(2 alpha), fused mode
+    double abs_s;   // |sin(2 alpha)| of the emission gate
+    int32_t fused;  // 1: cos folded into the scan, partner value recovered as (v k) (1/k)
+    int32_t fold_off, fold_cnt;
+    int32_t piv0, piv1;
+    int32_t identity;
+};
+
+struct Trunc {
+    uint32_t cutoff;
+    int32_t cutoff_type;
+    int32_t check_lower, check_upper;
+    double lower, upper;
+    int32_t only_rot;  // -1: none
+};
+
+// ------------------------------------------------------------------------------------------------
+// kernels
+
+// Anticommutation + pivot split over the column index: one thread per 64-term word.
+__global__ __launch_bounds__(kBlock) void k_scan(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                 const int32_t *__restrict__ fold, int nfold, int piv0,
+                                                 int piv1, const uint64_t *__restrict__ counters,
+                                                 uint64_t *__restrict__ lead, uint64_t *__restrict__ foll,
+                                                 uint32_t *__restrict__ blk_l, uint32_t *__restrict__ blk_f) {
+    __shared__ uint32_t lds[kWaves];
+    const uint64_t N = counters[C_N];
+    const uint64_t nwords = (N + 63) / 64;
+    const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    uint64_t lw = 0, fw = 0;
+    if (w < nwords) {
+        uint64_t a = 0, p = 0;
+        for (int j = 0; j < nfold; ++j) a ^= cols[static_cast<uint64_t>(fold[j]) * colstride + w];
+        if (piv0 >= 0) p = cols[static_cast<uint64_t>(piv0) * colstride + w];
+        if (piv1 >= 0) p ^= cols[static_cast<uint64_t>(piv1) * colstride + w];
+        if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
+        lw = a & ~p;
+        fw = a & p;
+        lead[w] = lw;
+        foll[w] = fw;
+    }
+    const uint32_t cl = block_sum(static_cast<uint32_t>(__popcll(lw)), lds);
+    const uint32_t cf = block_sum(static_cast<uint32_t>(__popcll(fw)), lds);
+    if (threadIdx.x == 0) {
+        blk_l[blockIdx.x] = cl;
+        blk_f[blockIdx.x] = cf;
+    }
+}
+
+// Exclusive scan of one or two per-block count arrays in a single block; totals to counters.
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t *__restrict__ a, uint32_t *__restrict__ b,
+                                                          uint32_t nb, uint64_t *__restrict__ counters,
+                                                          int ta, int tb, int zero_from, int zero_to) {
+    __shared__ uint32_t lds[kScanBlock / 64];
+    __shared__ uint32_t carry_a, carry_b;
+    if (threadIdx.x == 0) {
+        carry_a = 0;
+        carry_b = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < nb; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t va = (i < nb) ? a[i] : 0;
+        uint32_t vb = (b != nullptr && i < nb) ? b[i] : 0;
+        for (int which = 0; which < 2; ++which) {
+            uint32_t v = which == 0 ? va : vb;
+            if (which == 1 && b == nullptr) break;
+            const uint32_t incl = wave_incl_scan(v);
+            const uint32_t wave = threadIdx.x >> 6;
+            if (lane_id() == 63) lds[wave] = incl;
+            __syncthreads();
+            uint32_t wave_base = 0, tot = 0;
+            for (int w = 0; w < kScanBlock / 64; ++w) {
+                if (static_cast<uint32_t>(w) < wave) wave_base += lds[w];
+                tot += lds[w];
+            }
+            uint32_t &carry = which == 0 ? carry_a : carry_b;
+            const uint32_t excl = carry + wave_base + incl - v;
+            if (i < nb) (which == 0 ? a : b)[i] = excl;
+            __syncthreads();
+            if (threadIdx.x == 0) carry += tot;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (ta >= 0) counters[ta] = carry_a;
+        if (tb >= 0) counters[tb] = carry_b;
+        for (int z = zero_from; z < zero_to; ++z) counters[z] = 0;
+    }
+}
+
+// Masks -> ascending item list: leaders at [0, nL), followers at [nL, nL + nF).
+__global__ __launch_bounds__(kBlock) void k_expand(const uint64_t *__restrict__ lead,
+                                                   const uint64_t *__restrict__ foll,
+                                                   const uint32_t *__restrict__ blk_l,
+                                                   const uint32_t *__restrict__ blk_f,
+                                                   const uint64_t *__restrict__ counters,
+                                                   uint32_t *__restrict__ AL) {
+    __shared__ uint32_t lds[kWaves];
+    const uint64_t N = counters[C_N];
+    const uint64_t nwords = (N + 63) / 64;
+    const uint64_t nL = counters[C_NL];
+    const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    uint64_t lw = 0, fw = 0;
+    if (w < nwords) {
+        lw = lead[w];
+        fw = foll[w];
+    }
+    uint32_t tot;
+    const uint32_t ol = block_excl_scan(static_cast<uint32_t>(__popcll(lw)), lds, &tot);
+    const uint32_t of = block_excl_scan(static_cast<uint32_t>(__popcll(fw)), lds, &tot);
+    uint64_t pos = blk_l[blockIdx.x] + ol;
+    for (uint64_t m = lw; m; m &= m - 1) AL[pos++] = static_cast<uint32_t>(w * 64 + __ffsll(static_cast<unsigned long long>(m)) - 1);
+    pos = nL + blk_f[blockIdx.x] + of;
+    for (uint64_t m = fw; m; m &= m - 1) AL[pos++] = static_cast<uint32_t>(w * 64 + __ffsll(static_cast<unsigned long long>(m)) - 1);
+}
+
+template <int NW>
+__device__ __forceinline__ void load_row(const uint64_t *__restrict__ rows, uint32_t r, uint64_t *m) {
+    const uint64_t *p = rows + static_cast<uint64_t>(r) * NW;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) m[w] = p[w];
+}
+
+// Emission decision of one anticommuting term (Scan.h:146-157, 315-347; CutoffContext.h:36-41).
+// bit0 emit, bit1 phase is -1, bit2 term is in the cos set.
+template <int NW, int PAULI>
+__device__ __forceinline__ uint32_t classify(const uint64_t *m, const GateDesc &gd, const Trunc &tr, double v,
+                                             uint64_t *t) {
+    const int mono_pop = mpa::slot_pop<NW, PAULI>(m);
+    const bool in_cos = tr.only_rot < 0 || mono_pop <= tr.only_rot;
+    const double abs_c = fabs(v);
+    bool emit = in_cos && !(tr.check_lower && gd.abs_s * abs_c <= tr.lower);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) t[w] = m[w] ^ gd.g[w];
+    int phase = 1;
+    if (emit) {
+        const bool keep = mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type);
+        emit = keep || (tr.check_upper && gd.abs_s * abs_c >= tr.upper);
+        if (emit) phase = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
+    }
+    return (emit ? 1u : 0u) | (phase < 0 ? 2u : 0u) | (in_cos ? 4u : 0u);
+}
+
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_leaders(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
+                                                    const uint64_t *__restrict__ rows,
+                                                    const double *__restrict__ coef,
+                                                    const uint32_t *__restrict__ AL, uint32_t nL,
+                                                    uint64_t *__restrict__ table, uint64_t tmask, uint64_t epoch,
+                                                    uint8_t *__restrict__ rec, double *__restrict__ lv,
+                                                    uint8_t *__restrict__ match, uint64_t *__restrict__ counters) {
+    constexpr int SW = NW + 1;
+    const GateDesc &gd = gdesc[gi];
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    bool emit = false;
+    if (b < nL) {
+        const uint32_t r = AL[b];
+        uint64_t m[NW], t[NW];
+        load_row<NW>(rows, r, m);
+        const double v = coef[r];
+        const uint32_t cls = classify<NW, PAULI>(m, gd, tr, v, t);
+        emit = cls & 1u;
+        rec[b] = static_cast<uint8_t>(cls);
+        lv[b] = v;
+        match[b] = 0;
+        // insert the leader's monomial: claim a slot of this gate's epoch, then write the key
+        const uint64_t tag = (epoch << 32) | b;
+        uint64_t h = mpa::hash_key<NW>(m) & tmask;
+        bool done = false;
+        for (uint64_t probe = 0; probe <= tmask; ++probe) {
+            uint64_t *slot = table + h * SW;
+            const uint64_t cur = __hip_atomic_load(slot + NW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((cur >> 32) != epoch) {
+                const unsigned long long old =
+                    atomicCAS(reinterpret_cast<unsigned long long *>(slot + NW), cur, tag);
+                if (old == cur) {
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) slot[w] = m[w];
+                    done = true;
+                    break;
+                }
+            }
+            h = (h + 1) & tmask;
+        }
+        if (!done) atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 1ull);
+    }
+    const uint64_t ball = __ballot(emit);
+    if (lane_id() == 0 && ball) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NEMIT),
+                                          static_cast<unsigned long long>(__popcll(ball)));
+}
+
+// Coefficient update of a pair.  S is the emitting endpoint with phase ph; T the other.  The
+// expression order is the reference's: c[S] = k v_S + (s (-ph)) v_T~, c[T] = k v_T + (s ph) v_S,
+// where v_T~ is the post-cos stored value times 1/k in fused mode (Engine.h:209-211) and v_T otherwise.
+__device__ __forceinline__ void rotate_pair(const GateDesc &gd, double vS, double vT, bool cosS, bool cosT,
+                                            int ph, double *cS, double *cT) {
+    const double wS = cosS ? vS * gd.k : vS;
+    const double wT = cosT ? vT * gd.k : vT;
+    const double vTu = gd.fused ? wT * gd.inv_k : vT;
+    *cS = wS + gd.s * static_cast<double>(-ph) * vTu;
+    *cT = wT + gd.s * static_cast<double>(ph) * vS;
+}
+
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
+                                                      const uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                      const uint32_t *__restrict__ AL, uint32_t nL, uint32_t nF,
+                                                      const uint64_t *__restrict__ table, uint64_t tmask,
+                                                      uint64_t epoch, const uint8_t *__restrict__ rec,
+                                                      const double *__restrict__ lv, uint8_t *__restrict__ match,
+                                                      uint8_t *__restrict__ miss, double *__restrict__ ins,
+                                                      uint64_t *__restrict__ counters) {
+    constexpr int SW = NW + 1;
+    const GateDesc &gd = gdesc[gi];
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    bool emit = false;
+    if (f < nF) {
+        const uint32_t item = nL + f;
+        const uint32_t r = AL[item];
+        uint64_t m[NW], t[NW];
+        load_row<NW>(rows, r, m);
+        const double v = coef[r];
+        const uint32_t cls = classify<NW, PAULI>(m, gd, tr, v, t);
+        emit = cls & 1u;
+        const int ph = (cls & 2u) ? -1 : 1;
+        const bool in_cos = cls & 4u;
+        // partner lookup: t = m ^ g is the leader form of this pair
+        int64_t b = -1;
+        uint64_t h = mpa::hash_key<NW>(t) & tmask;
+        for (uint64_t probe = 0; probe <= tmask; ++probe) {
+            const uint64_t *slot = table + h * SW;
+            const uint64_t tag = slot[NW];
+            if ((tag >> 32) != epoch) break;
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) eq = eq && (slot[w] == t[w]);
+            if (eq) {
+                b = static_cast<int64_t>(tag & 0xffffffffull);
+                break;
+            }
+            h = (h + 1) & tmask;
+        }
+        uint8_t is_miss = 0;
+        double new_coef = 0.0;
+        if (b >= 0) {
+            match[b] = 1;
+            const uint32_t rl = AL[b];
+            const double vL = lv[b];
+            const uint32_t lc = rec[b];
+            const bool eL = lc & 1u, cosL = lc & 4u;
+            const int phL = (lc & 2u) ? -1 : 1;
+            double cL, cF;
+            if (eL) {
+                rotate_pair(gd, vL, v, cosL, in_cos, phL, &cL, &cF);
+            } else if (emit) {
+                rotate_pair(gd, v, vL, in_cos, cosL, ph, &cF, &cL);
+            } else {
+                cL = cosL ? vL * gd.k : vL;
+                cF = in_cos ? v * gd.k : v;
+            }
+            coef[rl] = cL;
+            coef[r] = cF;
+        } else {
+            const double w = in_cos ? v * gd.k : v;
+            if (emit) {
+                coef[r] = w + gd.s * static_cast<double>(-ph) * 0.0;
+                new_coef = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
+                is_miss = 1;
+            } else {
+                coef[r] = w;
+            }
+        }
+        miss[item] = is_miss;
+        ins[item] = new_coef;
+    }
+    const uint64_t ball = __ballot(emit);
+    if (lane_id() == 0 && ball) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NEMIT),
+                                          static_cast<unsigned long long>(__popcll(ball)));
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__restrict__ gdesc, int gi,
+                                                              double *__restrict__ coef,
+                                                              const uint32_t *__restrict__ AL, uint32_t nL,
+                                                              const uint8_t *__restrict__ rec,
+                                                              const double *__restrict__ lv,
+                                                              const uint8_t *__restrict__ match,
+                                                              uint8_t *__restrict__ miss, double *__restrict__ ins) {
+    const GateDesc &gd = gdesc[gi];
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nL) return;
+    if (match[b]) {
+        miss[b] = 0;
+        return;
+    }
+    const uint32_t lc = rec[b];
+    const bool e = lc & 1u, in_cos = lc & 4u;
+    const int ph = (lc & 2u) ? -1 : 1;
+    const double v = lv[b];
+    const double w = in_cos ? v * gd.k : v;
+    const uint32_t r = AL[b];
+    if (e) {
+        coef[r] = w + gd.s * static_cast<double>(-ph) * 0.0;
+        ins[b] = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
+        miss[b] = 1;
+    } else {
+        coef[r] = w;
+        miss[b] = 0;
+    }
+}
+
+// Per-block counts of miss flags over the combined item range [0, nA).
+__global__ __launch_bounds__(kBlock) void k_count_miss(const uint8_t *__restrict__ miss, uint32_t nA,
+                                                       uint32_t *__restrict__ blk) {
+    __shared__ uint32_t lds[kWaves];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t v = (i < nA) ? miss[i] : 0;
+    const uint32_t s = block_sum(v, lds);
+    if (threadIdx.x == 0) blk[blockIdx.x] = s;
+}
+
+// Append the partners to create at N + rank (rank over leaders then followers).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_insert(const GateDesc *__restrict__ gdesc, int gi,
+                                                   uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                   const uint32_t *__restrict__ AL, uint32_t nA,
+                                                   const uint8_t *__restrict__ miss, const double *__restrict__ ins,
+                                                   const uint32_t *__restrict__ blk,
+                                                   const uint64_t *__restrict__ counters) {
+    __shared__ uint32_t lds[kWaves];
+    const GateDesc &gd = gdesc[gi];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t v = (i < nA) ? miss[i] : 0;
+    uint32_t tot;
+    const uint32_t rank = block_excl_scan(v, lds, &tot);
+    if (!v) return;
+    const uint64_t dest = counters[C_N] + blk[blockIdx.x] + rank;
+    const uint32_t src = AL[i];
+    const uint64_t *ps = rows + static_cast<uint64_t>(src) * NW;
+    uint64_t *pd = rows + dest * NW;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) pd[w] = ps[w] ^ gd.g[w];
+    coef[dest] = ins[i];
+}
+
+// Transpose rows [start, start + count) into the column index: one wave per 64-term word, one
+// ballot per column.  start/count come from counters (C_N, C_NINS) when `from_counters`.
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_cols_update(const uint64_t *__restrict__ rows,
+                                                        uint64_t *__restrict__ cols, uint64_t colstride,
+                                                        int n_modes, const uint64_t *__restrict__ counters,
+                                                        uint64_t start_arg, uint64_t count_arg, int from_counters) {
+    const uint64_t start = from_counters ? counters[C_N] : start_arg;
+    const uint64_t count = from_counters ? counters[C_NINS] : count_arg;
+    if (count == 0) return;
+    const uint64_t w0 = start / 64;
+    const uint64_t w = w0 + static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    const uint64_t wend = (start + count + 63) / 64;
+    if (w >= wend) return;  // whole wave exits together
+    const uint64_t r = w * 64 + lane_id();
+    uint64_t m[NW];
+    const bool live = r >= start && r < start + count;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) m[k] = live ? rows[r * NW + k] : 0;
+    const bool lane0 = lane_id() == 0;
+    if (PAULI) {
+        constexpr int L = NW / 2;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            for (int bit = 0; bit < 64; ++bit) {
+                const int q = l * 64 + bit;
+                if (q >= n_modes) break;
+                const uint64_t bx = __ballot((m[l] >> bit) & 1u);
+                const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
+                if (lane0) {
+                    if (bx) cols[static_cast<uint64_t>(q) * colstride + w] |= bx;
+                    if (bz) cols[static_cast<uint64_t>(n_modes + q) * colstride + w] |= bz;
+                }
+            }
+        }
+    } else {
+        int pop = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            pop += __popcll(m[k]);
+            for (int bit = 0; bit < 64; ++bit) {
+                const int s = k * 64 + bit;
+                if (s >= 2 * n_modes) break;
+                const uint64_t b = __ballot((m[k] >> bit) & 1u);
+                if (lane0 && b) cols[static_cast<uint64_t>(s) * colstride + w] |= b;
+            }
+        }
+        const uint64_t bp = __ballot(pop & 1);
+        if (lane0 && bp) cols[static_cast<uint64_t>(2 * n_modes) * colstride + w] |= bp;
+    }
+}
+
+__global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__ stats, int gi, int identity) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t *s = stats + static_cast<int64_t>(gi) * 4;
+    s[0] = static_cast<int64_t>(counters[C_N]);
+    if (identity) {
+        s[1] = s[2] = s[3] = 0;
+        return;
+    }
+    s[1] = static_cast<int64_t>(counters[C_NL] + counters[C_NF]);
+    s[2] = static_cast<int64_t>(counters[C_NEMIT]);
+    s[3] = static_cast<int64_t>(counters[C_NINS]);
+    counters[C_N] += counters[C_NINS];
+    counters[C_NINS] = 0;
+    counters[C_NEMIT] = 0;
+}
+
+// <O> partials: sum over fully paired rows of phase * c, per block in a fixed order.
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_expect(const uint64_t *__restrict__ rows,
+                                                   const double *__restrict__ coef, uint64_t N,
+                                                   const uint64_t *__restrict__ occ_dev,
+                                                   double *__restrict__ partial) {
+    __shared__ double red[kBlock];
+    uint64_t occ[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
+    double acc = 0.0;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        uint64_t m[NW];
+        load_row<NW>(rows, static_cast<uint32_t>(i), m);
+        if (mpa::is_paired<NW, PAULI>(m)) acc += mpa::state_phase<NW, PAULI>(m, occ) * coef[i];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(s)) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------------------------------------------
+// host side
+
+struct DevArray {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevArray() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+    // Grow to at least `need` bytes (contents not preserved).
+    void reserve(size_t need) {
+        if (need <= bytes) return;
+        size_t nb = std::max(need, bytes + bytes / 2);
+        nb = (nb + 4095) & ~size_t{4095};
+        if (p) HIP_TRY(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        HIP_TRY(hipMalloc(&p, nb));
+        bytes = nb;
+    }
+};
+
+struct Slots {
+    std::vector<int64_t> off;
+    std::vector<int32_t> idx;
+};
+
+}  // namespace
+
+struct mpa_engine {
+    int basis = 0, n_modes = 0, L = 0, nw = 0, ncols = 0;
+    Trunc tr{};
+    double core = 0.0;
+    std::vector<uint64_t> occ;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    uint64_t N = 0;     // live terms (exact whenever the host has synchronised)
+    uint64_t cap = 0;   // row capacity (multiple of 64 * kBlock)
+    DevArray rows, coef, cols;
+    DevArray AL, lead, foll, blk_a, blk_b, rec, lv, match, miss, ins, table, counters, stats, gdesc, fold, occ_dev,
+        partial;
+    uint64_t table_slots = 0;
+    uint64_t epoch = 0;
+    int64_t *h_counters = nullptr;  // pinned
+
+    std::vector<double> init_coef;  // initial coefficients of rows [0, n_init)
+    uint64_t n_init = 0;
+    std::vector<int64_t> stat_n, stat_a, stat_e, stat_i;
+    double t_ms[4] = {0, 0, 0, 0};
+    double t_bytes[4] = {0, 0, 0, 0};
+
+    ~mpa_engine() {
+        if (h_counters) (void)hipHostFree(h_counters);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    uint64_t colstride() const { return cap / 64; }
+
+    // Grow the store to hold at least `need` rows, preserving rows, coefficients and columns.
+    void ensure_capacity(uint64_t need) {
+        if (need <= cap) return;
+        uint64_t nc = std::max<uint64_t>(need, cap + cap / 2);
+        const uint64_t q = 64ull * kBlock;
+        nc = std::max<uint64_t>((nc + q - 1) / q * q, q);
+        DevArray r2, c2, k2;
+        r2.reserve(nc * nw * 8);
+        c2.reserve(nc * 8);
+        k2.reserve(static_cast<size_t>(ncols) * (nc / 64) * 8);
+        HIP_TRY(hipMemsetAsync(k2.p, 0, static_cast<size_t>(ncols) * (nc / 64) * 8, stream));
+        if (N > 0) {
+            HIP_TRY(hipMemcpyAsync(r2.p, rows.p, N * nw * 8, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(c2.p, coef.p, N * 8, hipMemcpyDeviceToDevice, stream));
+            const uint64_t nwords = (N + 63) / 64;
+            HIP_TRY(hipMemcpy2DAsync(k2.p, (nc / 64) * 8, cols.p, colstride() * 8, nwords * 8, ncols,
+                                     hipMemcpyDeviceToDevice, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::swap(rows.p, r2.p);
+        std::swap(rows.bytes, r2.bytes);
+        std::swap(coef.p, c2.p);
+        std::swap(coef.bytes, c2.bytes);
+        std::swap(cols.p, k2.p);
+        std::swap(cols.bytes, k2.bytes);
+        cap = nc;
+        // per-term scratch sized by capacity; the item list of the current gate is kept
+        if (AL.bytes < cap * 4) {
+            DevArray a2;
+            a2.reserve(cap * 4);
+            if (AL.p) HIP_TRY(hipMemcpyAsync(a2.p, AL.p, AL.bytes, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            std::swap(AL.p, a2.p);
+            std::swap(AL.bytes, a2.bytes);
+        }
+        lead.reserve((cap / 64) * 8);
+        foll.reserve((cap / 64) * 8);
+        const uint64_t nblk = cap / 64 / kBlock + 2;
+        blk_a.reserve(std::max<uint64_t>(nblk, cap / kBlock + 2) * 4);
+        blk_b.reserve(nblk * 4);
+    }
+
+    void ensure_items(uint64_t nL, uint64_t nA) {
+        rec.reserve(std::max<uint64_t>(nL, 1));
+        lv.reserve(std::max<uint64_t>(nL, 1) * 8);
+        match.reserve(std::max<uint64_t>(nL, 1));
+        miss.reserve(std::max<uint64_t>(nA, 1));
+        ins.reserve(std::max<uint64_t>(nA, 1) * 8);
+        blk_a.reserve((nA / kBlock + 2) * 4);
+        uint64_t want = 64;
+        while (want < 2 * nL) want <<= 1;
+        if (want > table_slots) {
+            table.reserve(want * (nw + 1) * 8);
+            HIP_TRY(hipMemsetAsync(table.p, 0, table.bytes, stream));
+            table_slots = want;
+            epoch = 0;  // fresh memory: every tag reads as empty for any epoch >= 1
+        }
+    }
+
+    void sync_counters() {
+        HIP_TRY(hipMemcpyAsync(h_counters, counters.p, C_COUNT * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+
+    template <int NW, int PAULI>
+    void launch_cols_update(uint64_t start, uint64_t count, bool from_counters, uint64_t count_ub) {
+        const uint64_t words = (count_ub + 63) / 64 + 1;
+        const uint64_t blocks = (words + kWaves - 1) / kWaves;
+        if (blocks == 0) return;
+        hipLaunchKernelGGL((k_cols_update<NW, PAULI>), dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream,
+                           rows.as<uint64_t>(), cols.as<uint64_t>(), colstride(), n_modes,
+                           counters.as<uint64_t>(), start, count, from_counters ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+    }
+
+    template <int NW, int PAULI>
+    void propagate_impl(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds) {
+        const size_t ng = gds.size();
+        gdesc.reserve(std::max<size_t>(ng, 1) * sizeof(GateDesc));
+        fold.reserve(std::max<size_t>(folds.size(), 1) * 4);
+        stats.reserve(std::max<size_t>(ng, 1) * 4 * 8);
+        if (ng) HIP_TRY(hipMemcpyAsync(gdesc.p, gds.data(), ng * sizeof(GateDesc), hipMemcpyHostToDevice, stream));
+        if (!folds.empty())
+            HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
+        uint64_t n_ub = N;  // upper bound of the live count at the current gate
+        hipEvent_t ev[2];
+        HIP_TRY(hipEventCreate(&ev[0]));
+        HIP_TRY(hipEventCreate(&ev[1]));
+        HIP_TRY(hipEventRecord(ev[0], stream));
+        for (size_t gi = 0; gi < ng; ++gi) {
+            const GateDesc &gd = gds[gi];
+            if (gd.identity) {
+                hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(),
+                                   stats.as<int64_t>(), static_cast<int>(gi), 1);
+                HIP_TRY(hipGetLastError());
+                continue;
+            }
+            const uint64_t nwords_ub = (n_ub + 63) / 64;
+            const uint32_t sblocks = static_cast<uint32_t>(std::max<uint64_t>((nwords_ub + kBlock - 1) / kBlock, 1));
+            hipLaunchKernelGGL(k_scan, dim3(sblocks), dim3(kBlock), 0, stream, cols.as<uint64_t>(), colstride(),
+                               fold.as<int32_t>() + gd.fold_off, gd.fold_cnt, gd.piv0, gd.piv1,
+                               counters.as<uint64_t>(), lead.as<uint64_t>(), foll.as<uint64_t>(),
+                               blk_a.as<uint32_t>(), blk_b.as<uint32_t>());
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, stream, blk_a.as<uint32_t>(),
+                               blk_b.as<uint32_t>(), sblocks, counters.as<uint64_t>(), C_NL, C_NF, C_NINS,
+                               C_NEMIT + 1);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_expand, dim3(sblocks), dim3(kBlock), 0, stream, lead.as<uint64_t>(),
+                               foll.as<uint64_t>(), blk_a.as<uint32_t>(), blk_b.as<uint32_t>(),
+                               counters.as<uint64_t>(), AL.as<uint32_t>());
+            HIP_TRY(hipGetLastError());
+            sync_counters();
+            N = static_cast<uint64_t>(h_counters[C_N]);
+            if (h_counters[C_ERR]) throw RunError("device hash table overflow");
+            const uint64_t nL = static_cast<uint64_t>(h_counters[C_NL]);
+            const uint64_t nF = static_cast<uint64_t>(h_counters[C_NF]);
+            const uint64_t nA = nL + nF;
+            n_ub = N + nA;
+            if (n_ub >= (uint64_t{1} << 32)) throw RunError("term count exceeds the 32-bit row index");
+            ensure_capacity(n_ub);
+            ensure_items(nL, nA);
+            if (nA > 0) {
+                ++epoch;
+                const uint64_t tmask = table_slots - 1;
+                const uint32_t lb = static_cast<uint32_t>((nL + kBlock - 1) / kBlock);
+                const uint32_t fb = static_cast<uint32_t>((nF + kBlock - 1) / kBlock);
+                const uint32_t ab = static_cast<uint32_t>((nA + kBlock - 1) / kBlock);
+                if (lb)
+                    hipLaunchKernelGGL((k_leaders<NW, PAULI>), dim3(lb), dim3(kBlock), 0, stream,
+                                       gdesc.as<GateDesc>(), static_cast<int>(gi), tr, rows.as<uint64_t>(),
+                                       coef.as<double>(), AL.as<uint32_t>(), static_cast<uint32_t>(nL),
+                                       table.as<uint64_t>(), tmask, epoch, rec.as<uint8_t>(), lv.as<double>(),
+                                       match.as<uint8_t>(), counters.as<uint64_t>());
+                HIP_TRY(hipGetLastError());
+                if (fb)
+                    hipLaunchKernelGGL((k_followers<NW, PAULI>), dim3(fb), dim3(kBlock), 0, stream,
+                                       gdesc.as<GateDesc>(), static_cast<int>(gi), tr, rows.as<uint64_t>(),
+                                       coef.as<double>(), AL.as<uint32_t>(), static_cast<uint32_t>(nL),
+                                       static_cast<uint32_t>(nF), table.as<uint64_t>(), tmask, epoch,
+                                       rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(), miss.as<uint8_t>(),
+                                       ins.as<double>(), counters.as<uint64_t>());
+                HIP_TRY(hipGetLastError());
+                if (lb)
+                    hipLaunchKernelGGL(k_leaders_unmatched, dim3(lb), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                                       static_cast<int>(gi), coef.as<double>(), AL.as<uint32_t>(),
+                                       static_cast<uint32_t>(nL), rec.as<uint8_t>(), lv.as<double>(),
+                                       match.as<uint8_t>(), miss.as<uint8_t>(), ins.as<double>());
+                HIP_TRY(hipGetLastError());
+                hipLaunchKernelGGL(k_count_miss, dim3(ab), dim3(kBlock), 0, stream, miss.as<uint8_t>(),
+                                   static_cast<uint32_t>(nA), blk_a.as<uint32_t>());
+                HIP_TRY(hipGetLastError());
+                hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, stream, blk_a.as<uint32_t>(),
+                                   static_cast<uint32_t *>(nullptr), ab, counters.as<uint64_t>(), C_NINS, -1, 0, 0);
+                HIP_TRY(hipGetLastError());
+                hipLaunchKernelGGL((k_insert<NW>), dim3(ab), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                                   static_cast<int>(gi), rows.as<uint64_t>(), coef.as<double>(), AL.as<uint32_t>(),
+                                   static_cast<uint32_t>(nA), miss.as<uint8_t>(), ins.as<double>(),
+                                   blk_a.as<uint32_t>(), counters.as<uint64_t>());
+                HIP_TRY(hipGetLastError());
+                launch_cols_update<NW, PAULI>(0, 0, true, nA);
+            }
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(), stats.as<int64_t>(),
+                               static_cast<int>(gi), 0);
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipEventRecord(ev[1], stream));
+        sync_counters();
+        N = static_cast<uint64_t>(h_counters[C_N]);
+        if (h_counters[C_ERR]) throw RunError("device hash table overflow");
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        t_ms[3] = ms;
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        std::vector<int64_t> st(ng * 4);
+        if (ng) HIP_TRY(hipMemcpy(st.data(), stats.p, ng * 4 * 8, hipMemcpyDeviceToHost));
+        stat_n.resize(ng);
+        stat_a.resize(ng);
+        stat_e.resize(ng);
+        stat_i.resize(ng);
+        for (size_t g = 0; g < ng; ++g) {
+            stat_n[g] = st[4 * g];
+            stat_a[g] = st[4 * g + 1];
+            stat_e[g] = st[4 * g + 2];
+            stat_i[g] = st[4 * g + 3];
+        }
+    }
+
+    template <int NW, int PAULI>
+    double expect_impl() {
+        partial.reserve(kExpectBlocks * 8);
+        hipLaunchKernelGGL((k_expect<NW, PAULI>), dim3(kExpectBlocks), dim3(kBlock), 0, stream, rows.as<uint64_t>(),
+                           coef.as<double>(), N, occ_dev.as<uint64_t>(), partial.as<double>());
+        HIP_TRY(hipGetLastError());
+        std::vector<double> h(kExpectBlocks);
+        HIP_TRY(hipMemcpyAsync(h.data(), partial.p, kExpectBlocks * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        double r = 0.0;
+        for (double x : h) r += x;
+        return core + r;
+    }
+
+    // Runtime word count / basis -> template instantiation.
+    template <typename F>
+    void dispatch(F &&f) {
+        if (basis == MPA_BASIS_PAULI) {
+            switch (nw) {
+                case 2: f.template operator()<2, 1>(); return;
+                case 4: f.template operator()<4, 1>(); return;
+                case 6: f.template operator()<6, 1>(); return;
+                case 8: f.template operator()<8, 1>(); return;
+            }
+        } else {
+            switch (nw) {
+                case 1: f.template operator()<1, 0>(); return;
+                case 2: f.template operator()<2, 0>(); return;
+                case 3: f.template operator()<3, 0>(); return;
+                case 4: f.template operator()<4, 0>(); return;
+                case 5: f.template operator()<5, 0>(); return;
+                case 6: f.template operator()<6, 0>(); return;
+                case 7: f.template operator()<7, 0>(); return;
+                case 8: f.template operator()<8, 0>(); return;
+            }
+        }
+        throw ArgError("unsupported row width");
+    }
+
+    // Reference slot tuple -> row words (conversion_utils.py:66-88 Pauli slots).
+    void slots_to_words(const int32_t *s, int64_t n, uint64_t *out) const {
+        std::fill(out, out + nw, 0);
+        for (int64_t k = 0; k < n; ++k) {
+            const int32_t x = s[k];
+            if (x < 0 || x >= 2 * n_modes)
+                throw ArgError("Majorana/Pauli index " + std::to_string(x) + " is out of range; must be less than " +
+                               std::to_string(2 * n_modes) + ".");
+            if (basis == MPA_BASIS_PAULI) {
+                const int q = x / 2;
+                const uint64_t b = uint64_t{1} << (q & 63);
+                out[q >> 6] ^= b;                    // slot 2q (u) and slot 2q+1 (v) both flip x = u ^ v
+                if (x & 1) out[L + (q >> 6)] ^= b;  // slot 2q+1 (v) is z
+            } else {
+                out[x >> 6] |= uint64_t{1} << (x & 63);
+            }
+        }
+    }
+
+    void words_to_slots(const uint64_t *m, std::vector<int32_t> &out) const {
+        out.clear();
+        if (basis == MPA_BASIS_PAULI) {
+            for (int q = 0; q < n_modes; ++q) {
+                const uint64_t b = uint64_t{1} << (q & 63);
+                const bool x = m[q >> 6] & b, z = m[L + (q >> 6)] & b;
+                if (x ^ z) out.push_back(2 * q);
+                if (z) out.push_back(2 * q + 1);
+            }
+            return;
+        }
+        for (int s = 0; s < 2 * n_modes; ++s)
+            if (m[s >> 6] & (uint64_t{1} << (s & 63))) out.push_back(s);
+    }
+
+    // Fold and pivot columns of a generator (Scan.h:265-271; Algebra.h:76-78, 108-110).
+    void gate_columns(const uint64_t *g, std::vector<int32_t> &fold_out, int32_t &piv0, int32_t &piv1) const {
+        piv0 = piv1 = -1;
+        if (basis == MPA_BASIS_PAULI) {
+            int qmax = -1;
+            for (int q = 0; q < n_modes; ++q) {
+                const uint64_t b = uint64_t{1} << (q & 63);
+                const bool x = g[q >> 6] & b, z = g[L + (q >> 6)] & b;
+                if (z) fold_out.push_back(q);            // x_M . z_G
+                if (x) fold_out.push_back(n_modes + q);  // z_M . x_G
+                if (x || z) qmax = q;
+            }
+            if (qmax >= 0) {
+                const uint64_t b = uint64_t{1} << (qmax & 63);
+                if (g[L + (qmax >> 6)] & b) {
+                    piv0 = n_modes + qmax;  // Y or Z letter: the pivot is the z bit
+                } else {
+                    piv0 = qmax;  // X letter: the pivot is u = x ^ z
+                    piv1 = n_modes + qmax;
+                }
+            }
+            return;
+        }
+        int pop = 0, smax = -1;
+        for (int s = 0; s < 2 * n_modes; ++s)
+            if (g[s >> 6] & (uint64_t{1} << (s & 63))) {
+                fold_out.push_back(s);
+                ++pop;
+                smax = s;
+            }
+        if (pop & 1) fold_out.push_back(2 * n_modes);  // odd |G|: parity(|M|) correction
+        piv0 = smax;
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+
+namespace {
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_error.clear();
+        return 0;
+    } catch (const ArgError &e) {
+        return set_error(-1, e.what);
+    } catch (const RunError &e) {
+        return set_error(-2, e.what);
+    } catch (const HipFailure &e) {
+        return set_error(-3, e.what);
+    } catch (const std::bad_alloc &) {
+        return set_error(-2, "host allocation failed");
+    } catch (const std::exception &e) {
+        return set_error(-2, e.what());
+    }
+}
+
+bool is_none(double x) { return std::isnan(x); }
+
+// Template functors handed to mpa_engine::dispatch (the runtime basis / row width picks the instance).
+struct InitCols {
+    mpa_engine *e;
+    uint64_t n0;
+    template <int NW, int PAULI>
+    void operator()() {
+        e->launch_cols_update<NW, PAULI>(0, n0, false, n0);
+    }
+};
+
+struct RunPropagate {
+    mpa_engine *e;
+    const std::vector<GateDesc> &gds;
+    const std::vector<int32_t> &folds;
+    template <int NW, int PAULI>
+    void operator()() {
+        e->propagate_impl<NW, PAULI>(gds, folds);
+    }
+};
+
+struct RunExpect {
+    mpa_engine *e;
+    double *out;
+    template <int NW, int PAULI>
+    void operator()() {
+        *out = e->expect_impl<NW, PAULI>();
+    }
+};
+
+void set_trunc(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower, double upper) {
+    if (e->basis == MPA_BASIS_PAULI && cutoff_type != MPA_CUTOFF_SUPPORT)
+        throw ArgError("Pauli basis requires cutoff_type == Support (Length has no Pauli-weight meaning under the "
+                       "Pauli encoding).");
+    if (!is_none(lower) && !is_none(upper) && upper < lower)
+        throw RunError("upper_atol must be greater than or equal to lower_atol.");
+    e->tr.cutoff = cutoff;
+    e->tr.cutoff_type = cutoff_type;
+    e->tr.check_lower = is_none(lower) ? 0 : 1;
+    e->tr.check_upper = is_none(upper) ? 0 : 1;
+    e->tr.lower = is_none(lower) ? 0.0 : lower;
+    e->tr.upper = is_none(upper) ? 0.0 : upper;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpa_last_error(void) { return g_error.c_str(); }
+
+int mpa_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type, double lower_atol,
+               double upper_atol, const int64_t *term_offsets, const int32_t *term_slots, const double *coeff_re,
+               const double *coeff_im, int64_t n_terms, const int32_t *initial_state, int64_t n_state, int device) {
+    return guarded([&] {
+        if (!out) throw ArgError("out is null");
+        *out = nullptr;
+        if (basis != MPA_BASIS_PAULI && basis != MPA_BASIS_MAJORANA) throw ArgError("unknown basis");
+        if (num_modes <= 0 || num_modes > 256)
+            throw RunError("logical_num_modes (" + std::to_string(num_modes) + ") must be in the range [1, 256].");
+        auto e = std::make_unique<mpa_engine>();
+        e->basis = basis;
+        e->n_modes = num_modes;
+        e->L = (num_modes + 63) / 64;
+        e->nw = basis == MPA_BASIS_PAULI ? 2 * e->L : (2 * num_modes + 63) / 64;
+        e->ncols = basis == MPA_BASIS_PAULI ? 2 * num_modes : 2 * num_modes + 1;
+        set_trunc(e.get(), cutoff, cutoff_type, lower_atol, upper_atol);
+        e->tr.only_rot = -1;
+        e->device = device;
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&e->h_counters), C_COUNT * 8, hipHostMallocDefault));
+        e->occ.assign(mpa::kMaxWords, 0);
+        for (int64_t k = 0; k < n_state; ++k) {
+            const int j = initial_state[k];
+            if (j < 0 || j >= num_modes) throw ArgError("initial_state index out of range");
+            if (basis == MPA_BASIS_PAULI) e->occ[j >> 6] |= uint64_t{1} << (j & 63);
+            else e->occ[(2 * j) >> 6] |= uint64_t{1} << ((2 * j) & 63);
+        }
+        // initial terms in lexicographic slot-tuple order (std::map<VecZ, complex> iteration)
+        std::vector<int64_t> order(n_terms);
+        for (int64_t t = 0; t < n_terms; ++t) order[t] = t;
+        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+            return std::lexicographical_compare(term_slots + term_offsets[a], term_slots + term_offsets[a + 1],
+                                                term_slots + term_offsets[b], term_slots + term_offsets[b + 1]);
+        });
+        std::vector<uint64_t> hrows;
+        std::vector<double> hcoef;
+        std::vector<uint64_t> m(e->nw);
+        for (int64_t t : order) {
+            const int64_t n = term_offsets[t + 1] - term_offsets[t];
+            e->slots_to_words(term_slots + term_offsets[t], n, m.data());
+            const std::complex<double> c(coeff_re[t], coeff_im[t]);
+            double enc;
+            if (basis == MPA_BASIS_PAULI) {  // PauliAlgebra.h:150-155
+                if (std::abs(c.imag()) > 1e-10) throw RunError("Non-real Pauli coeffs detected");
+                enc = c.real();
+            } else {  // MajoranaAlgebra.h:162-171
+                static const std::complex<double> kPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+                const size_t un = static_cast<size_t>(n);
+                const std::complex<double> d = c / kPow[(un * (un - 1) / 2) % 4];
+                if (std::abs(d.imag()) > 1e-10) throw RunError("Non-Hermitian coeffs detected");
+                enc = d.real();
+            }
+            if (n == 0) {
+                e->core = enc;
+                continue;
+            }
+            hrows.insert(hrows.end(), m.begin(), m.end());
+            hcoef.push_back(enc);
+        }
+        const uint64_t n0 = hcoef.size();
+        // duplicate keys cannot occur: distinct sorted slot tuples give distinct rows
+        e->counters.reserve(C_COUNT * 8);
+        HIP_TRY(hipMemsetAsync(e->counters.p, 0, C_COUNT * 8, e->stream));
+        e->occ_dev.reserve(mpa::kMaxWords * 8);
+        HIP_TRY(hipMemcpyAsync(e->occ_dev.p, e->occ.data(), mpa::kMaxWords * 8, hipMemcpyHostToDevice, e->stream));
+        e->ensure_capacity(std::max<uint64_t>(n0, 1));
+        if (n0) {
+            HIP_TRY(hipMemcpyAsync(e->rows.p, hrows.data(), hrows.size() * 8, hipMemcpyHostToDevice, e->stream));
+            HIP_TRY(hipMemcpyAsync(e->coef.p, hcoef.data(), n0 * 8, hipMemcpyHostToDevice, e->stream));
+            InitCols init{e.get(), n0};
+            e->dispatch(init);
+            const int64_t c0[2] = {static_cast<int64_t>(n0), 0};
+            HIP_TRY(hipMemcpyAsync(e->counters.p, c0, 8, hipMemcpyHostToDevice, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+        }
+        e->N = n0;
+        e->n_init = n0;
+        e->init_coef = hcoef;
+        *out = e.release();
+    });
+}
+
+int mpa_destroy(mpa_engine *e) {
+    return guarded([&] { delete e; });
+}
+
+int mpa_set_truncation(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower_atol, double upper_atol) {
+    return guarded([&] { set_trunc(e, cutoff, cutoff_type, lower_atol, upper_atol); });
+}
+
+int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
+                  const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
+                  int32_t only_rotate_len_k) {
+    return guarded([&] {
+        if (only_rotate_len_k == 0 || only_rotate_len_k > 2 * e->n_modes)
+            throw ArgError("only_rotate_len_k is out of range; must be 0 < k <= 2*num_modes");
+        std::vector<GateDesc> gds;
+        std::vector<int32_t> folds;
+        gds.reserve(n_layers);
+        std::vector<uint64_t> g(mpa::kMaxWords);
+        // Heisenberg: authoring order back to front (MonomialPropagator.inl:720-733)
+        for (int64_t i = 0; i < n_layers; ++i) {
+            const int64_t idx = n_layers - 1 - i;
+            if (param_map[idx] < 0 || param_map[idx] >= n_params)
+                throw ArgError("parameter index out of range for the given parameters");
+            GateDesc gd{};
+            std::fill(g.begin(), g.end(), 0);
+            e->slots_to_words(gen_slots + gen_offsets[idx], gen_offsets[idx + 1] - gen_offsets[idx], g.data());
+            std::copy(g.begin(), g.begin() + e->nw, gd.g);
+            // alpha = (1.0 * theta) * g (MPFunctions.cpp:57-69); the angle functions are the host libm's
+            const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
+            gd.k = std::cos(2 * alpha);
+            gd.s = std::sin(2 * alpha);
+            gd.abs_s = std::abs(std::sin(2 * alpha));
+            const double cos_build = std::cos(2.0 * alpha);
+            gd.fused = (only_rotate_len_k < 0 && cos_build != 0.0) ? 1 : 0;
+            gd.inv_k = gd.fused ? 1.0 / cos_build : 1.0;
+            gd.fold_off = static_cast<int32_t>(folds.size());
+            e->gate_columns(gd.g, folds, gd.piv0, gd.piv1);
+            gd.fold_cnt = static_cast<int32_t>(folds.size()) - gd.fold_off;
+            gd.identity = gd.fold_cnt == 0 ? 1 : 0;
+            gds.push_back(gd);
+        }
+        e->tr.only_rot = only_rotate_len_k < 0 ? -1 : only_rotate_len_k;
+        RunPropagate run{e, gds, folds};
+        e->dispatch(run);
+        e->tr.only_rot = -1;
+    });
+}
+
+int mpa_build_graph(mpa_engine *, const int64_t *, const int32_t *, int64_t, const int64_t *, const double *,
+                    const double *, int64_t) {
+    return set_error(-2, "build_graph is not available in this build");
+}
+
+int mpa_expectation_value(mpa_engine *e, const double *, int64_t n_params, double *out) {
+    return guarded([&] {
+        if (n_params != 0)
+            throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
+        RunExpect ev{e, out};
+        e->dispatch(ev);
+    });
+}
+
+int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int64_t n_params, double *value,
+                                       double *) {
+    return mpa_expectation_value(e, params, n_params, value);
+}
+
+int mpa_size(mpa_engine *e, int64_t *out) {
+    return guarded([&] { *out = static_cast<int64_t>(e->N); });
+}
+
+int mpa_graph_layers(mpa_engine *, int64_t *layers, int64_t *n_params) {
+    *layers = 0;
+    *n_params = 0;
+    return 0;
+}
+
+int mpa_words_per_row(mpa_engine *e, int32_t *out) {
+    *out = e->nw;
+    return 0;
+}
+
+int mpa_read_rows(mpa_engine *e, uint64_t *words, double *coeffs) {
+    return guarded([&] {
+        if (e->N == 0) return;
+        HIP_TRY(hipMemcpyAsync(words, e->rows.p, e->N * e->nw * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(coeffs, e->coef.p, e->N * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mpa_export_terms(mpa_engine *e, const double *, int64_t n_params, double atol, int64_t *n_terms,
+                     int64_t *n_slots, int64_t *offsets, int32_t *slots, double *re, double *im) {
+    return guarded([&] {
+        if (n_params != 0)
+            throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
+        std::vector<uint64_t> w(e->N * e->nw);
+        std::vector<double> c(e->N);
+        if (e->N) {
+            HIP_TRY(hipMemcpyAsync(w.data(), e->rows.p, e->N * e->nw * 8, hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipMemcpyAsync(c.data(), e->coef.p, e->N * 8, hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+        }
+        std::vector<int32_t> tmp;
+        int64_t nt = 0, ns = 0;
+        static const std::complex<double> kPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+        for (uint64_t i = 0; i < e->N; ++i) {
+            if (std::abs(c[i]) < atol) continue;
+            e->words_to_slots(w.data() + i * e->nw, tmp);
+            if (slots) {
+                offsets[nt] = ns;
+                std::copy(tmp.begin(), tmp.end(), slots + ns);
+                std::complex<double> dec(c[i], 0.0);
+                if (e->basis == MPA_BASIS_MAJORANA) dec = dec * kPow[(tmp.size() * (tmp.size() - 1) / 2) % 4];
+                re[nt] = std::round(dec.real() * 1e12) / 1e12;
+                im[nt] = std::round(dec.imag() * 1e12) / 1e12;
+            }
+            ++nt;
+            ns += static_cast<int64_t>(tmp.size());
+        }
+        if (slots) {
+            offsets[nt] = ns;
+        }
+        // core term (binder.h:213-215)
+        if (std::abs(e->core) >= atol) {
+            if (slots) {
+                offsets[nt + 1] = ns;
+                re[nt] = e->core;
+                im[nt] = 0.0;
+            }
+            ++nt;
+        }
+        *n_terms = nt;
+        *n_slots = ns;
+    });
+}
+
+int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t *em, int64_t *ins) {
+    return guarded([&] {
+        const int64_t have = static_cast<int64_t>(e->stat_n.size());
+        if (!n) {
+            *len = have;
+            return;
+        }
+        const int64_t k = std::min(*len, have);
+        for (int64_t i = 0; i < k; ++i) {
+            n[i] = e->stat_n[i];
+            a[i] = e->stat_a[i];
+            em[i] = e->stat_e[i];
+            ins[i] = e->stat_i[i];
+        }
+        *len = k;
+    });
+}
+
+int mpa_kernel_timing(mpa_engine *e, double *ms, double *bytes) {
+    for (int i = 0; i < 4; ++i) {
+        ms[i] = e->t_ms[i];
+        bytes[i] = e->t_bytes[i];
+    }
+    return 0;
+}
+
+}  // extern "C"
