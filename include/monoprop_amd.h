@@ -90,10 +90,18 @@ int mpa_read_rows(mpa_engine *e, uint64_t *words, double *coeffs);
  * anticommuting terms, emitted partners, inserted terms.  `len` in/out. */
 int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t *em, int64_t *ins);
 
-/* Device-side kernel time of the last propagate (ms, HIP events) per kernel class:
- * [0] column scan + compaction, [1] classify/match/apply, [2] insert + index update,
- * [3] everything; and the algorithmic bytes of the same kernels. */
-int mpa_kernel_timing(mpa_engine *e, double *ms, double *bytes);
+/* Profiler (no counterpart in the reference): with profiling on, every kernel launch of propagate
+ * is bracketed by HIP events on the engine's stream.  mpa_kernel_profile reports, per kernel class
+ * `kind` (0 <= kind < 10; `name` is the kernel's name), the summed device time in ms, the summed
+ * algorithmic bytes (the traffic the algorithm must move: columns streamed by the scan, rows and
+ * coefficients gathered per anticommuting term, hash slots, appended rows) and the launch count,
+ * accumulated since profiling was switched on.  Returns -1 past the last kind. */
+int mpa_set_profiling(mpa_engine *e, int on);
+int mpa_kernel_profile(mpa_engine *e, int kind, const char **name, double *ms, double *bytes,
+                       int64_t *launches);
+
+/* Device time of the last propagate call from first to last launch (HIP events), in ms. */
+int mpa_last_propagate_ms(mpa_engine *e, double *ms);
 
 /* Replaces the cutoff / cutoff_type / lower_atol / upper_atol setters (binder.h:158-189); applies to
  * gates propagated afterwards. */

@@ -44,7 +44,9 @@ _SIGNATURES = {
     "mpa_words_per_row": (C.c_int, [_P, _i32p]),
     "mpa_read_rows": (C.c_int, [_P, _u64p, _f64p]),
     "mpa_gate_stats": (C.c_int, [_P, _i64p, _i64p, _i64p, _i64p, _i64p]),
-    "mpa_kernel_timing": (C.c_int, [_P, _f64p, _f64p]),
+    "mpa_set_profiling": (C.c_int, [_P, C.c_int]),
+    "mpa_kernel_profile": (C.c_int, [_P, C.c_int, C.POINTER(C.c_char_p), _f64p, _f64p, _i64p]),
+    "mpa_last_propagate_ms": (C.c_int, [_P, _f64p]),
     "mpa_set_truncation": (C.c_int, [_P, C.c_uint32, C.c_int, C.c_double, C.c_double]),
 }
 
@@ -303,11 +305,28 @@ class DeviceSimulator:
         _check(lib.mpa_gate_stats(self._h, C.byref(ln), *[_ptr(a, C.c_int64) for a in arrs]))
         return tuple(a[: ln.value] for a in arrs)
 
-    def kernel_timing(self):
-        ms = np.zeros(4, dtype=np.float64)
-        by = np.zeros(4, dtype=np.float64)
-        _check(load_library().mpa_kernel_timing(self._h, _ptr(ms, C.c_double), _ptr(by, C.c_double)))
-        return ms, by
+    def set_profiling(self, on: bool) -> None:
+        """Bracket every kernel launch with HIP events (resets the accumulators)."""
+        _check(load_library().mpa_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_profile(self) -> dict[str, dict[str, float]]:
+        """Per kernel: summed device ms, algorithmic bytes and launches since profiling was enabled."""
+        lib = load_library()
+        out = {}
+        kind = 0
+        while True:
+            name = C.c_char_p()
+            ms, by, n = C.c_double(), C.c_double(), C.c_int64()
+            if lib.mpa_kernel_profile(self._h, kind, C.byref(name), C.byref(ms), C.byref(by), C.byref(n)) != 0:
+                break
+            out[name.value.decode()] = {"ms": ms.value, "bytes": by.value, "launches": n.value}
+            kind += 1
+        return out
+
+    def last_propagate_ms(self) -> float:
+        v = C.c_double()
+        _check(load_library().mpa_last_propagate_ms(self._h, C.byref(v)))
+        return v.value
 
     # --- truncation properties (binder.h:158-189) ------------------------------------------------
     def _push_truncation(self):

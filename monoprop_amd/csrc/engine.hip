@@ -51,7 +51,15 @@ constexpr int kScanBlock = 1024;
 constexpr int kExpectBlocks = 1024;
 
 // counters[] on the device
-enum : int { C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_ERR = 5, C_COUNT = 8 };
+enum : int { C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6, C_COUNT = 8 };
+constexpr int kStatW = 6;  // per-gate stats: N, nA, emitted, inserted, leaders, pairs
+
+// Kernel classes timed by the profiler (mpa_kernel_profile).
+enum KernelKind : int { K_SCAN, K_SCANSUM, K_EXPAND, K_LEADERS, K_FOLLOWERS, K_UNMATCHED, K_COUNT, K_INSERT,
+                        K_COLS, K_ADVANCE, K_KINDS };
+const char *const kKindNames[K_KINDS] = {"k_scan", "k_scan_sums", "k_expand", "k_leaders", "k_followers",
+                                         "k_leaders_unmatched", "k_count_miss", "k_insert", "k_cols_update",
+                                         "k_advance"};
 
 thread_local std::string g_error;
 
@@ -343,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
     constexpr int SW = NW + 1;
     const GateDesc &gd = gdesc[gi];
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
-    bool emit = false;
+    bool emit = false, paired = false;
     if (f < nF) {
         const uint32_t item = nL + f;
         const uint32_t r = AL[item];
@@ -373,6 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
         uint8_t is_miss = 0;
         double new_coef = 0.0;
         if (b >= 0) {
+            paired = true;
             match[b] = 1;
             const uint32_t rl = AL[b];
             const double vL = lv[b];
@@ -406,6 +415,9 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
     const uint64_t ball = __ballot(emit);
     if (lane_id() == 0 && ball) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NEMIT),
                                           static_cast<unsigned long long>(__popcll(ball)));
+    const uint64_t pball = __ballot(paired);
+    if (lane_id() == 0 && pball) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR),
+                                           static_cast<unsigned long long>(__popcll(pball)));
 }
 
 __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__restrict__ gdesc, int gi,
@@ -526,18 +538,21 @@ __global__ __launch_bounds__(kBlock) void k_cols_update(const uint64_t *__restri
 
 __global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__ stats, int gi, int identity) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t *s = stats + static_cast<int64_t>(gi) * 4;
+    int64_t *s = stats + static_cast<int64_t>(gi) * kStatW;
     s[0] = static_cast<int64_t>(counters[C_N]);
     if (identity) {
-        s[1] = s[2] = s[3] = 0;
+        s[1] = s[2] = s[3] = s[4] = s[5] = 0;
         return;
     }
     s[1] = static_cast<int64_t>(counters[C_NL] + counters[C_NF]);
     s[2] = static_cast<int64_t>(counters[C_NEMIT]);
     s[3] = static_cast<int64_t>(counters[C_NINS]);
+    s[4] = static_cast<int64_t>(counters[C_NL]);
+    s[5] = static_cast<int64_t>(counters[C_NPAIR]);
     counters[C_N] += counters[C_NINS];
     counters[C_NINS] = 0;
     counters[C_NEMIT] = 0;
+    counters[C_NPAIR] = 0;
 }
 
 // <O> partials: sum over fully paired rows of phase * c, per block in a fixed order.
@@ -592,11 +607,6 @@ struct DevArray {
     }
 };
 
-struct Slots {
-    std::vector<int64_t> off;
-    std::vector<int32_t> idx;
-};
-
 }  // namespace
 
 struct mpa_engine {
@@ -618,13 +628,84 @@ struct mpa_engine {
 
     std::vector<double> init_coef;  // initial coefficients of rows [0, n_init)
     uint64_t n_init = 0;
-    std::vector<int64_t> stat_n, stat_a, stat_e, stat_i;
-    double t_ms[4] = {0, 0, 0, 0};
-    double t_bytes[4] = {0, 0, 0, 0};
+    std::vector<int64_t> stat_n, stat_a, stat_e, stat_i, stat_l, stat_p;
+
+    // Profiler: an event pair around every launch, resolved whenever the stream is idle.
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    double prof_ms[K_KINDS] = {};
+    double prof_bytes[K_KINDS] = {};
+    int64_t prof_launches[K_KINDS] = {};
+    double last_propagate_ms = 0.0;
 
     ~mpa_engine() {
+        for (auto &p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
         if (h_counters) (void)hipHostFree(h_counters);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t ev_get() {
+        if (!ev_pool.empty()) {
+            hipEvent_t ev = ev_pool.back();
+            ev_pool.pop_back();
+            return ev;
+        }
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreate(&ev));
+        return ev;
+    }
+    hipEvent_t prof_begin() {
+        if (!prof_on) return nullptr;
+        hipEvent_t a = ev_get();
+        HIP_TRY(hipEventRecord(a, stream));
+        return a;
+    }
+    void prof_end(int kind, hipEvent_t a) {
+        if (!prof_on) return;
+        hipEvent_t b = ev_get();
+        HIP_TRY(hipEventRecord(b, stream));
+        pending.push_back({kind, a, b});
+    }
+    // Call only when the stream is idle.
+    void prof_resolve() {
+        for (const Pending &p : pending) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+            prof_ms[p.kind] += ms;
+            prof_launches[p.kind] += 1;
+            ev_pool.push_back(p.a);
+            ev_pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    // Algorithmic bytes of one propagate's kernels from the per-gate counters (the traffic the
+    // algorithm must move, not what the caches see): NW-word rows, SW-word hash slots.
+    void add_algorithmic_bytes(const std::vector<int32_t> &fold_plus_piv) {
+        const double NW = nw, SW = nw + 1.0;
+        for (size_t g = 0; g < stat_n.size(); ++g) {
+            if (stat_a[g] == 0 && fold_plus_piv[g] == 0) continue;
+            const double words = std::ceil(static_cast<double>(stat_n[g]) / 64.0);
+            const double nA = stat_a[g], nL = stat_l[g], nF = nA - nL, pairs = stat_p[g], I = stat_i[g];
+            prof_bytes[K_SCAN] += words * 8.0 * (fold_plus_piv[g] + 2);
+            prof_bytes[K_SCANSUM] += 2.0 * 8.0 * std::ceil(words / kBlock);
+            prof_bytes[K_EXPAND] += words * 16.0 + nA * 4.0;
+            if (nA == 0) continue;
+            prof_bytes[K_LEADERS] += nL * (4 + 8 * NW + 8 + 8 * SW + 1 + 8 + 1);
+            prof_bytes[K_FOLLOWERS] += nF * (4 + 8 * NW + 8 + 8 * SW + 8 + 1 + 8) + pairs * (4 + 8 + 1 + 8 + 1);
+            prof_bytes[K_UNMATCHED] += nL * 1 + (nL - pairs) * (1 + 8 + 4 + 8 + 1 + 8);
+            prof_bytes[K_COUNT] += nA;
+            prof_bytes[K_INSERT] += nA + I * (4 + 16 * NW + 8);
+            prof_bytes[K_COLS] += I * (8 * NW + ncols / 8.0);
+        }
     }
 
     uint64_t colstride() const { return cap / 64; }
@@ -709,7 +790,7 @@ struct mpa_engine {
         const size_t ng = gds.size();
         gdesc.reserve(std::max<size_t>(ng, 1) * sizeof(GateDesc));
         fold.reserve(std::max<size_t>(folds.size(), 1) * 4);
-        stats.reserve(std::max<size_t>(ng, 1) * 4 * 8);
+        stats.reserve(std::max<size_t>(ng, 1) * kStatW * 8);
         if (ng) HIP_TRY(hipMemcpyAsync(gdesc.p, gds.data(), ng * sizeof(GateDesc), hipMemcpyHostToDevice, stream));
         if (!folds.empty())
             HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
@@ -718,30 +799,42 @@ struct mpa_engine {
         HIP_TRY(hipEventCreate(&ev[0]));
         HIP_TRY(hipEventCreate(&ev[1]));
         HIP_TRY(hipEventRecord(ev[0], stream));
+        std::vector<int32_t> fold_plus_piv(ng, 0);
         for (size_t gi = 0; gi < ng; ++gi) {
             const GateDesc &gd = gds[gi];
+            const int igi = static_cast<int>(gi);
             if (gd.identity) {
+                hipEvent_t pa = prof_begin();
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(),
-                                   stats.as<int64_t>(), static_cast<int>(gi), 1);
+                                   stats.as<int64_t>(), igi, 1);
                 HIP_TRY(hipGetLastError());
+                prof_end(K_ADVANCE, pa);
                 continue;
             }
+            fold_plus_piv[gi] = gd.fold_cnt + (gd.piv0 >= 0) + (gd.piv1 >= 0);
             const uint64_t nwords_ub = (n_ub + 63) / 64;
             const uint32_t sblocks = static_cast<uint32_t>(std::max<uint64_t>((nwords_ub + kBlock - 1) / kBlock, 1));
+            hipEvent_t pa = prof_begin();
             hipLaunchKernelGGL(k_scan, dim3(sblocks), dim3(kBlock), 0, stream, cols.as<uint64_t>(), colstride(),
                                fold.as<int32_t>() + gd.fold_off, gd.fold_cnt, gd.piv0, gd.piv1,
                                counters.as<uint64_t>(), lead.as<uint64_t>(), foll.as<uint64_t>(),
                                blk_a.as<uint32_t>(), blk_b.as<uint32_t>());
             HIP_TRY(hipGetLastError());
+            prof_end(K_SCAN, pa);
+            pa = prof_begin();
             hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, stream, blk_a.as<uint32_t>(),
                                blk_b.as<uint32_t>(), sblocks, counters.as<uint64_t>(), C_NL, C_NF, C_NINS,
-                               C_NEMIT + 1);
+                               C_NPAIR + 1);
             HIP_TRY(hipGetLastError());
+            prof_end(K_SCANSUM, pa);
+            pa = prof_begin();
             hipLaunchKernelGGL(k_expand, dim3(sblocks), dim3(kBlock), 0, stream, lead.as<uint64_t>(),
                                foll.as<uint64_t>(), blk_a.as<uint32_t>(), blk_b.as<uint32_t>(),
                                counters.as<uint64_t>(), AL.as<uint32_t>());
             HIP_TRY(hipGetLastError());
+            prof_end(K_EXPAND, pa);
             sync_counters();
+            prof_resolve();
             N = static_cast<uint64_t>(h_counters[C_N]);
             if (h_counters[C_ERR]) throw RunError("device hash table overflow");
             const uint64_t nL = static_cast<uint64_t>(h_counters[C_NL]);
@@ -757,65 +850,85 @@ struct mpa_engine {
                 const uint32_t lb = static_cast<uint32_t>((nL + kBlock - 1) / kBlock);
                 const uint32_t fb = static_cast<uint32_t>((nF + kBlock - 1) / kBlock);
                 const uint32_t ab = static_cast<uint32_t>((nA + kBlock - 1) / kBlock);
-                if (lb)
+                if (lb) {
+                    pa = prof_begin();
                     hipLaunchKernelGGL((k_leaders<NW, PAULI>), dim3(lb), dim3(kBlock), 0, stream,
-                                       gdesc.as<GateDesc>(), static_cast<int>(gi), tr, rows.as<uint64_t>(),
-                                       coef.as<double>(), AL.as<uint32_t>(), static_cast<uint32_t>(nL),
-                                       table.as<uint64_t>(), tmask, epoch, rec.as<uint8_t>(), lv.as<double>(),
-                                       match.as<uint8_t>(), counters.as<uint64_t>());
-                HIP_TRY(hipGetLastError());
-                if (fb)
+                                       gdesc.as<GateDesc>(), igi, tr, rows.as<uint64_t>(), coef.as<double>(),
+                                       AL.as<uint32_t>(), static_cast<uint32_t>(nL), table.as<uint64_t>(), tmask,
+                                       epoch, rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(),
+                                       counters.as<uint64_t>());
+                    HIP_TRY(hipGetLastError());
+                    prof_end(K_LEADERS, pa);
+                }
+                if (fb) {
+                    pa = prof_begin();
                     hipLaunchKernelGGL((k_followers<NW, PAULI>), dim3(fb), dim3(kBlock), 0, stream,
-                                       gdesc.as<GateDesc>(), static_cast<int>(gi), tr, rows.as<uint64_t>(),
-                                       coef.as<double>(), AL.as<uint32_t>(), static_cast<uint32_t>(nL),
-                                       static_cast<uint32_t>(nF), table.as<uint64_t>(), tmask, epoch,
-                                       rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(), miss.as<uint8_t>(),
-                                       ins.as<double>(), counters.as<uint64_t>());
-                HIP_TRY(hipGetLastError());
-                if (lb)
+                                       gdesc.as<GateDesc>(), igi, tr, rows.as<uint64_t>(), coef.as<double>(),
+                                       AL.as<uint32_t>(), static_cast<uint32_t>(nL), static_cast<uint32_t>(nF),
+                                       table.as<uint64_t>(), tmask, epoch, rec.as<uint8_t>(), lv.as<double>(),
+                                       match.as<uint8_t>(), miss.as<uint8_t>(), ins.as<double>(),
+                                       counters.as<uint64_t>());
+                    HIP_TRY(hipGetLastError());
+                    prof_end(K_FOLLOWERS, pa);
+                }
+                if (lb) {
+                    pa = prof_begin();
                     hipLaunchKernelGGL(k_leaders_unmatched, dim3(lb), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                       static_cast<int>(gi), coef.as<double>(), AL.as<uint32_t>(),
-                                       static_cast<uint32_t>(nL), rec.as<uint8_t>(), lv.as<double>(),
-                                       match.as<uint8_t>(), miss.as<uint8_t>(), ins.as<double>());
-                HIP_TRY(hipGetLastError());
+                                       igi, coef.as<double>(), AL.as<uint32_t>(), static_cast<uint32_t>(nL),
+                                       rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(), miss.as<uint8_t>(),
+                                       ins.as<double>());
+                    HIP_TRY(hipGetLastError());
+                    prof_end(K_UNMATCHED, pa);
+                }
+                pa = prof_begin();
                 hipLaunchKernelGGL(k_count_miss, dim3(ab), dim3(kBlock), 0, stream, miss.as<uint8_t>(),
                                    static_cast<uint32_t>(nA), blk_a.as<uint32_t>());
                 HIP_TRY(hipGetLastError());
+                prof_end(K_COUNT, pa);
+                pa = prof_begin();
                 hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, stream, blk_a.as<uint32_t>(),
                                    static_cast<uint32_t *>(nullptr), ab, counters.as<uint64_t>(), C_NINS, -1, 0, 0);
                 HIP_TRY(hipGetLastError());
-                hipLaunchKernelGGL((k_insert<NW>), dim3(ab), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                   static_cast<int>(gi), rows.as<uint64_t>(), coef.as<double>(), AL.as<uint32_t>(),
+                prof_end(K_SCANSUM, pa);
+                pa = prof_begin();
+                hipLaunchKernelGGL((k_insert<NW>), dim3(ab), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), igi,
+                                   rows.as<uint64_t>(), coef.as<double>(), AL.as<uint32_t>(),
                                    static_cast<uint32_t>(nA), miss.as<uint8_t>(), ins.as<double>(),
                                    blk_a.as<uint32_t>(), counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
+                prof_end(K_INSERT, pa);
+                pa = prof_begin();
                 launch_cols_update<NW, PAULI>(0, 0, true, nA);
+                prof_end(K_COLS, pa);
             }
+            pa = prof_begin();
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(), stats.as<int64_t>(),
-                               static_cast<int>(gi), 0);
+                               igi, 0);
             HIP_TRY(hipGetLastError());
+            prof_end(K_ADVANCE, pa);
         }
         HIP_TRY(hipEventRecord(ev[1], stream));
         sync_counters();
+        prof_resolve();
         N = static_cast<uint64_t>(h_counters[C_N]);
         if (h_counters[C_ERR]) throw RunError("device hash table overflow");
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        t_ms[3] = ms;
+        last_propagate_ms = ms;
         (void)hipEventDestroy(ev[0]);
         (void)hipEventDestroy(ev[1]);
-        std::vector<int64_t> st(ng * 4);
-        if (ng) HIP_TRY(hipMemcpy(st.data(), stats.p, ng * 4 * 8, hipMemcpyDeviceToHost));
-        stat_n.resize(ng);
-        stat_a.resize(ng);
-        stat_e.resize(ng);
-        stat_i.resize(ng);
+        std::vector<int64_t> st(ng * kStatW);
+        if (ng) HIP_TRY(hipMemcpy(st.data(), stats.p, ng * kStatW * 8, hipMemcpyDeviceToHost));
+        for (auto *v : {&stat_n, &stat_a, &stat_e, &stat_i, &stat_l, &stat_p}) v->resize(ng);
         for (size_t g = 0; g < ng; ++g) {
-            stat_n[g] = st[4 * g];
-            stat_a[g] = st[4 * g + 1];
-            stat_e[g] = st[4 * g + 2];
-            stat_i[g] = st[4 * g + 3];
+            stat_n[g] = st[kStatW * g];
+            stat_a[g] = st[kStatW * g + 1];
+            stat_e[g] = st[kStatW * g + 2];
+            stat_i[g] = st[kStatW * g + 3];
+            stat_l[g] = st[kStatW * g + 4];
+            stat_p[g] = st[kStatW * g + 5];
         }
+        if (prof_on) add_algorithmic_bytes(fold_plus_piv);
     }
 
     template <int NW, int PAULI>
@@ -1239,11 +1352,28 @@ int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t 
     });
 }
 
-int mpa_kernel_timing(mpa_engine *e, double *ms, double *bytes) {
-    for (int i = 0; i < 4; ++i) {
-        ms[i] = e->t_ms[i];
-        bytes[i] = e->t_bytes[i];
-    }
+int mpa_set_profiling(mpa_engine *e, int on) {
+    return guarded([&] {
+        e->prof_on = on != 0;
+        for (int k = 0; k < K_KINDS; ++k) {
+            e->prof_ms[k] = 0.0;
+            e->prof_bytes[k] = 0.0;
+            e->prof_launches[k] = 0;
+        }
+    });
+}
+
+int mpa_kernel_profile(mpa_engine *e, int kind, const char **name, double *ms, double *bytes, int64_t *launches) {
+    if (kind < 0 || kind >= K_KINDS) return set_error(-1, "kernel kind out of range");
+    *name = kKindNames[kind];
+    *ms = e->prof_ms[kind];
+    *bytes = e->prof_bytes[kind];
+    *launches = e->prof_launches[kind];
+    return 0;
+}
+
+int mpa_last_propagate_ms(mpa_engine *e, double *ms) {
+    *ms = e->last_propagate_ms;
     return 0;
 }
 

@@ -736,49 +736,63 @@ int orc_graph_ev(void *h, const double *params, int64_t n_params, double *out) {
     return 0;
 }
 
-// Reverse-mode gradient of the replay with stored pre-layer vectors (exact, no 1/cos un-evolution).
+// Reverse-mode gradient of the replay (exact, no 1/cos un-evolution): the forward replay keeps the
+// pre-layer values of the entries each layer touches (its cos set and rotation endpoints); the reverse
+// sweep applies the transposed layer to the adjoint vector lambda.
 // d/dalpha of a layer: cos-only rows -2s c_i; pair (S,T,phi): -2s c_S - 2k phi c_T, -2s c_T + 2k phi c_S.
 int orc_graph_grad(void *h, const double *params, int64_t n_params, double *value, double *grad) {
     auto *e = static_cast<Engine *>(h);
     const size_t nl = e->graph.size();
-    std::vector<std::vector<double>> pre(nl);
-    std::vector<std::vector<uint32_t>> coss(nl);
+    std::vector<std::vector<uint32_t>> cos_only(nl);
+    std::vector<std::vector<double>> snap_cos(nl), snap_rot(nl);
     std::vector<double> op = e->init_coeffs;
     op.resize(e->size(), 0.0);
+    std::vector<char> endpoint(e->size(), 0);
+    std::vector<uint32_t> cos;
     for (size_t l = 0; l < nl; ++l) {
         const Layer &ly = e->graph[l];
         if (ly.param_index >= n_params) return fail("parameters too short");
-        pre[l] = op;
-        e->layer_cos(ly, coss[l]);
-        e->evolve_step(op, ly, coss[l], 1.0 * params[ly.param_index] * ly.gen_coeff);
+        e->layer_cos(ly, cos);
+        for (const Rotation &r : ly.rots) endpoint[r.src] = endpoint[r.tgt] = 1;
+        for (uint32_t i : cos)
+            if (!endpoint[i]) {
+                cos_only[l].push_back(i);
+                snap_cos[l].push_back(op[i]);
+            }
+        for (const Rotation &r : ly.rots) {
+            snap_rot[l].push_back(op[r.src]);
+            snap_rot[l].push_back(op[r.tgt]);
+            endpoint[r.src] = endpoint[r.tgt] = 0;
+        }
+        e->evolve_step(op, ly, cos, 1.0 * params[ly.param_index] * ly.gen_coeff);
     }
     *value = e->expectation_of(op);
     std::vector<double> lam(e->size(), 0.0);
     for (size_t i = 0; i < e->size(); ++i)
         if (e->paired(e->row(i))) lam[i] = e->state_phase(e->row(i));
     for (int64_t p = 0; p < n_params; ++p) grad[p] = 0.0;
-    std::vector<char> endpoint(e->size(), 0);
     for (size_t l = nl; l-- > 0;) {
         const Layer &ly = e->graph[l];
         const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
         const double k = std::cos(2 * alpha), s = std::sin(2 * alpha);
-        const std::vector<double> &c = pre[l];
         double d = 0.0;
-        for (const Rotation &r : ly.rots) endpoint[r.src] = endpoint[r.tgt] = 1;
-        for (uint32_t i : coss[l])
-            if (!endpoint[i]) {
-                d += lam[i] * (-2.0 * s * c[i]);
-                lam[i] *= k;
-            }
-        for (const Rotation &r : ly.rots) {
-            const double ph = r.phase;
-            d += lam[r.src] * (-2.0 * s * c[r.src] - 2.0 * k * ph * c[r.tgt]);
-            d += lam[r.tgt] * (-2.0 * s * c[r.tgt] + 2.0 * k * ph * c[r.src]);
+        for (size_t j = 0; j < cos_only[l].size(); ++j) {
+            const uint32_t i = cos_only[l][j];
+            d += lam[i] * (-2.0 * s * snap_cos[l][j]);
+            lam[i] *= k;
+        }
+        for (size_t j = 0; j < ly.rots.size(); ++j) {
+            const Rotation &r = ly.rots[j];
+            const double ph = r.phase, cs = snap_rot[l][2 * j], ct = snap_rot[l][2 * j + 1];
+            d += lam[r.src] * (-2.0 * s * cs - 2.0 * k * ph * ct);
+            d += lam[r.tgt] * (-2.0 * s * ct + 2.0 * k * ph * cs);
             const double ls = lam[r.src], lt = lam[r.tgt];
             lam[r.src] = k * ls + s * ph * lt;
             lam[r.tgt] = k * lt - s * ph * ls;
         }
-        for (const Rotation &r : ly.rots) endpoint[r.src] = endpoint[r.tgt] = 0;
+        std::vector<uint32_t>().swap(cos_only[l]);
+        std::vector<double>().swap(snap_cos[l]);
+        std::vector<double>().swap(snap_rot[l]);
         grad[ly.param_index] += ly.gen_coeff * d;
     }
     return 0;
