@@ -66,11 +66,19 @@ int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params,
 int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int64_t n_params,
                                        double *value, double *grad);
 
+/* Replaces MonomialPropagator::contract_partially (MonomialPropagator.inl:1052-1097, binder.h:147-151):
+ * the evolved coefficients of every stored term at params (the graph replayed, or the contracted
+ * operator when no graph is stored), written to out[size] when out != NULL; inplace != 0 folds the
+ * graph into the operator and drops it. */
+int mpa_contract(mpa_engine *e, const double *params, int64_t n_params, int inplace, double *out);
+
 /* Replaces MonomialPropagator::size (binder.h:233): number of stored terms. */
 int mpa_size(mpa_engine *e, int64_t *out);
 
-/* Graph metadata (binder.h:235-240): stored layers and the parameter count they reference. */
+/* Graph metadata (binder.h:235-240): stored layers and the parameter count they reference; the
+ * number of cos-only records and rotation records the layers hold (reference graph_size, binder.h:236). */
 int mpa_graph_layers(mpa_engine *e, int64_t *layers, int64_t *n_params);
+int mpa_graph_records(mpa_engine *e, int64_t *cos_only, int64_t *rotations);
 
 /* Replaces MonomialPropagator::evolved_operator_terms (MonomialPropagator.inl:1099-1127) plus the
  * core-term rule of binder.h:200-220: terms with |c| >= atol, decoded to complex and rounded to

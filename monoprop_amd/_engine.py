@@ -38,7 +38,9 @@ _SIGNATURES = {
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
     "mpa_expectation_value": (C.c_int, [_P, _f64p, C.c_int64, _f64p]),
     "mpa_expectation_value_and_gradient": (C.c_int, [_P, _f64p, C.c_int64, _f64p, _f64p]),
+    "mpa_contract": (C.c_int, [_P, _f64p, C.c_int64, C.c_int, _f64p]),
     "mpa_size": (C.c_int, [_P, _i64p]),
+    "mpa_graph_records": (C.c_int, [_P, _i64p, _i64p]),
     "mpa_graph_layers": (C.c_int, [_P, _i64p, _i64p]),
     "mpa_export_terms": (C.c_int, [_P, _f64p, C.c_int64, C.c_double, _i64p, _i64p, _i64p, _i32p, _f64p, _f64p]),
     "mpa_words_per_row": (C.c_int, [_P, _i32p]),
@@ -167,6 +169,7 @@ class DeviceSimulator:
                               len(list(initial_state)), default_device()))
         self._h = h
         self._parameter_mapping: list[int] = []
+        self._n_gates = 0
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -220,7 +223,11 @@ class DeviceSimulator:
             th_ptr, n = _ptr(th, C.c_double), len(list(parameters))
         _check(load_library().mpa_build_graph(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(majoranas),
                                               _ptr(pm, C.c_int64), _ptr(gc, C.c_double), th_ptr, n))
-        self._parameter_mapping = list(self._parameter_mapping) + [int(p) for p in reversed(list(parameter_mapping))]
+        gates = list(gate_indices) if gate_indices is not None else list(range(len(majoranas)))
+        offset = self._n_gates
+        self._n_gates = offset + (max(gates) + 1 if gates else 0)
+        # parameter_mapping in optimizer order (reverse of the processing order of the layers)
+        self._parameter_mapping = [int(p) for p in parameter_mapping] + list(self._parameter_mapping)
 
     # --- evaluation ------------------------------------------------------------------------------
     def expectation_value(self, parameters) -> float:
@@ -280,10 +287,24 @@ class DeviceSimulator:
         return layers.value
 
     def n_gates(self) -> int:
-        return 0 if self.graph_layers() == 0 else len(self._parameter_mapping)
+        return self._n_gates if self.graph_layers() > 0 else 0
 
     def graph_size(self) -> tuple[int, int]:
-        return (0, 0)
+        """``(cos-only records, rotations)`` stored by the graph."""
+        c, r = C.c_int64(), C.c_int64()
+        _check(load_library().mpa_graph_records(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
+
+    def contract_partially(self, parameters, inplace: bool) -> list[float]:
+        params = list(parameters)
+        th = _f64(params)
+        out = np.zeros(max(self.size(), 1), dtype=np.float64)
+        _check(load_library().mpa_contract(self._h, _ptr(th, C.c_double), len(params), 1 if inplace else 0,
+                                           _ptr(out, C.c_double)))
+        if inplace:
+            self._parameter_mapping = []
+            self._n_gates = 0
+        return list(out[: self.size()])
 
     def rows(self):
         """Raw store: ``(words[N, nw] uint64, coeffs[N] float64)`` in insertion order."""
@@ -398,7 +419,7 @@ class DeviceSimulator:
 
     @parameter_mapping.setter
     def parameter_mapping(self, mapping) -> None:
-        self._parameter_mapping = [int(m) for m in mapping]
+        raise RuntimeError("re-wiring the parameter mapping of a stored graph is not supported by the HIP engine.")
 
 
 def dispatch(num_modes: int):

@@ -51,8 +51,14 @@ constexpr int kScanBlock = 1024;
 constexpr int kExpectBlocks = 1024;
 
 // counters[] on the device
-enum : int { C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6, C_COUNT = 8 };
-constexpr int kStatW = 6;  // per-gate stats: N, nA, emitted, inserted, leaders, pairs
+enum : int {
+    C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6,
+    C_GCOS = 8, C_GROT = 9,  // graph record counts (cumulative over a build)
+    C_COUNT = 16
+};
+// per-gate stats: N, nA, emitted, inserted, leaders, pairs, graph cos-record end, graph rotation end
+constexpr int kStatW = 8;
+constexpr uint32_t kPhaseBit = 0x80000000u;  // rotation record: target row | (phase < 0) << 31
 
 // Kernel classes timed by the profiler (mpa_kernel_profile).
 enum KernelKind : int { K_SCAN, K_SCANSUM, K_EXPAND, K_LEADERS, K_FOLLOWERS, K_UNMATCHED, K_COUNT, K_INSERT,
@@ -128,6 +134,19 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds) {
     return tot;
 }
 
+// Wave-aggregated append: every lane of the wave must call it; lanes with `pred` get consecutive slots
+// of the list whose length lives at *counter (one atomic per wave).
+__device__ __forceinline__ uint64_t wave_append(bool pred, uint64_t *counter) {
+    const uint64_t ball = __ballot(pred);
+    unsigned long long base = 0;
+    if (lane_id() == 0 && ball)
+        base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), static_cast<unsigned long long>(__popcll(ball)));
+    base = __shfl(base, 0, 64);
+    const uint32_t lane = lane_id();
+    const uint64_t below = lane == 0 ? 0 : (ball & ((uint64_t{1} << lane) - 1));
+    return base + static_cast<uint64_t>(__popcll(below));
+}
+
 // ------------------------------------------------------------------------------------------------
 // per-gate constants
 
@@ -141,6 +160,8 @@ struct GateDesc {
     int32_t fold_off, fold_cnt;
     int32_t piv0, piv1;
     int32_t identity;
+    int64_t param_index;  // graph layer bookkeeping (host only)
+    double gen_coeff;
 };
 
 struct Trunc {
@@ -339,6 +360,13 @@ __device__ __forceinline__ void rotate_pair(const GateDesc &gd, double vS, doubl
     *cT = wT + gd.s * static_cast<double>(ph) * vS;
 }
 
+// Graph recording (build_graph): cos-only rows and rotation pairs of the layer, appended in any order
+// (each row is touched by exactly one record, so replay order inside a layer is irrelevant).
+struct GraphRec {
+    uint32_t *cos;  // null when not recording
+    uint2 *rot;     // {source row, target row | phase bit}
+};
+
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
                                                       const uint64_t *__restrict__ rows, double *__restrict__ coef,
@@ -347,11 +375,15 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
                                                       uint64_t epoch, const uint8_t *__restrict__ rec,
                                                       const double *__restrict__ lv, uint8_t *__restrict__ match,
                                                       uint8_t *__restrict__ miss, double *__restrict__ ins,
-                                                      uint64_t *__restrict__ counters) {
+                                                      uint64_t *__restrict__ counters, GraphRec grec) {
     constexpr int SW = NW + 1;
     const GateDesc &gd = gdesc[gi];
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
     bool emit = false, paired = false;
+    // graph records of this thread: up to two cos-only rows, or one rotation
+    bool rc0 = false, rc1 = false, rr = false;
+    uint32_t c0 = 0, c1 = 0;
+    uint2 rot = {0u, 0u};
     if (f < nF) {
         const uint32_t item = nL + f;
         const uint32_t r = AL[item];
@@ -391,11 +423,18 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
             double cL, cF;
             if (eL) {
                 rotate_pair(gd, vL, v, cosL, in_cos, phL, &cL, &cF);
+                rr = true;
+                rot = make_uint2(rl, r | (phL < 0 ? kPhaseBit : 0u));
             } else if (emit) {
                 rotate_pair(gd, v, vL, in_cos, cosL, ph, &cF, &cL);
+                rr = true;
+                rot = make_uint2(r, rl | (ph < 0 ? kPhaseBit : 0u));
             } else {
                 cL = cosL ? vL * gd.k : vL;
                 cF = in_cos ? v * gd.k : v;
+                rc0 = rc1 = true;
+                c0 = rl;
+                c1 = r;
             }
             coef[rl] = cL;
             coef[r] = cF;
@@ -404,9 +443,11 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
             if (emit) {
                 coef[r] = w + gd.s * static_cast<double>(-ph) * 0.0;
                 new_coef = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
-                is_miss = 1;
+                is_miss = static_cast<uint8_t>(1u | (ph < 0 ? 2u : 0u));  // rotation recorded by k_insert
             } else {
                 coef[r] = w;
+                rc0 = true;
+                c0 = r;
             }
         }
         miss[item] = is_miss;
@@ -418,6 +459,14 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
     const uint64_t pball = __ballot(paired);
     if (lane_id() == 0 && pball) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR),
                                            static_cast<unsigned long long>(__popcll(pball)));
+    if (grec.cos != nullptr) {
+        const uint64_t p0 = wave_append(rc0, counters + C_GCOS);
+        if (rc0) grec.cos[p0] = c0;
+        const uint64_t p1 = wave_append(rc1, counters + C_GCOS);
+        if (rc1) grec.cos[p1] = c1;
+        const uint64_t pr = wave_append(rr, counters + C_GROT);
+        if (rr) grec.rot[pr] = rot;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__restrict__ gdesc, int gi,
@@ -426,27 +475,36 @@ __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__
                                                               const uint8_t *__restrict__ rec,
                                                               const double *__restrict__ lv,
                                                               const uint8_t *__restrict__ match,
-                                                              uint8_t *__restrict__ miss, double *__restrict__ ins) {
+                                                              uint8_t *__restrict__ miss, double *__restrict__ ins,
+                                                              uint64_t *__restrict__ counters, GraphRec grec) {
     const GateDesc &gd = gdesc[gi];
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= nL) return;
-    if (match[b]) {
-        miss[b] = 0;
-        return;
+    bool rc = false;
+    uint32_t row = 0;
+    if (b < nL) {
+        if (match[b]) {
+            miss[b] = 0;
+        } else {
+            const uint32_t lc = rec[b];
+            const bool e = lc & 1u, in_cos = lc & 4u;
+            const int ph = (lc & 2u) ? -1 : 1;
+            const double v = lv[b];
+            const double w = in_cos ? v * gd.k : v;
+            row = AL[b];
+            if (e) {
+                coef[row] = w + gd.s * static_cast<double>(-ph) * 0.0;
+                ins[b] = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
+                miss[b] = static_cast<uint8_t>(1u | (ph < 0 ? 2u : 0u));
+            } else {
+                coef[row] = w;
+                miss[b] = 0;
+                rc = true;
+            }
+        }
     }
-    const uint32_t lc = rec[b];
-    const bool e = lc & 1u, in_cos = lc & 4u;
-    const int ph = (lc & 2u) ? -1 : 1;
-    const double v = lv[b];
-    const double w = in_cos ? v * gd.k : v;
-    const uint32_t r = AL[b];
-    if (e) {
-        coef[r] = w + gd.s * static_cast<double>(-ph) * 0.0;
-        ins[b] = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
-        miss[b] = 1;
-    } else {
-        coef[r] = w;
-        miss[b] = 0;
+    if (grec.cos != nullptr) {
+        const uint64_t p = wave_append(rc, counters + C_GCOS);
+        if (rc) grec.cos[p] = row;
     }
 }
 
@@ -455,33 +513,43 @@ __global__ __launch_bounds__(kBlock) void k_count_miss(const uint8_t *__restrict
                                                        uint32_t *__restrict__ blk) {
     __shared__ uint32_t lds[kWaves];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t v = (i < nA) ? miss[i] : 0;
+    const uint32_t v = (i < nA) ? (miss[i] & 1u) : 0;
     const uint32_t s = block_sum(v, lds);
     if (threadIdx.x == 0) blk[blockIdx.x] = s;
 }
 
-// Append the partners to create at N + rank (rank over leaders then followers).
+// Append the partners to create at N + rank (rank over leaders then followers).  `zero_fill`, when
+// set, receives 0 at the new rows (the operator's own coefficients while a graph is being built).
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_insert(const GateDesc *__restrict__ gdesc, int gi,
                                                    uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                   const uint32_t *__restrict__ AL, uint32_t nA,
-                                                   const uint8_t *__restrict__ miss, const double *__restrict__ ins,
-                                                   const uint32_t *__restrict__ blk,
-                                                   const uint64_t *__restrict__ counters) {
+                                                   double *__restrict__ zero_fill, const uint32_t *__restrict__ AL,
+                                                   uint32_t nA, const uint8_t *__restrict__ miss,
+                                                   const double *__restrict__ ins, const uint32_t *__restrict__ blk,
+                                                   uint64_t *__restrict__ counters, GraphRec grec) {
     __shared__ uint32_t lds[kWaves];
     const GateDesc &gd = gdesc[gi];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t v = (i < nA) ? miss[i] : 0;
+    const uint32_t flag = (i < nA) ? miss[i] : 0u;
+    const uint32_t v = flag & 1u;
     uint32_t tot;
     const uint32_t rank = block_excl_scan(v, lds, &tot);
-    if (!v) return;
-    const uint64_t dest = counters[C_N] + blk[blockIdx.x] + rank;
-    const uint32_t src = AL[i];
-    const uint64_t *ps = rows + static_cast<uint64_t>(src) * NW;
-    uint64_t *pd = rows + dest * NW;
+    uint64_t dest = 0;
+    uint32_t src = 0;
+    if (v) {
+        dest = counters[C_N] + blk[blockIdx.x] + rank;
+        src = AL[i];
+        const uint64_t *ps = rows + static_cast<uint64_t>(src) * NW;
+        uint64_t *pd = rows + dest * NW;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) pd[w] = ps[w] ^ gd.g[w];
-    coef[dest] = ins[i];
+        for (int w = 0; w < NW; ++w) pd[w] = ps[w] ^ gd.g[w];
+        coef[dest] = ins[i];
+        if (zero_fill != nullptr) zero_fill[dest] = 0.0;
+    }
+    if (grec.cos != nullptr) {
+        const uint64_t p = wave_append(v != 0, counters + C_GROT);
+        if (v) grec.rot[p] = make_uint2(src, static_cast<uint32_t>(dest) | ((flag & 2u) ? kPhaseBit : 0u));
+    }
 }
 
 // Transpose rows [start, start + count) into the column index: one wave per 64-term word, one
@@ -540,6 +608,8 @@ __global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int64_t *s = stats + static_cast<int64_t>(gi) * kStatW;
     s[0] = static_cast<int64_t>(counters[C_N]);
+    s[6] = static_cast<int64_t>(counters[C_GCOS]);
+    s[7] = static_cast<int64_t>(counters[C_GROT]);
     if (identity) {
         s[1] = s[2] = s[3] = s[4] = s[5] = 0;
         return;
@@ -579,6 +649,102 @@ __global__ __launch_bounds__(kBlock) void k_expect(const uint64_t *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// One graph layer replayed at angle alpha (reference evolve_step, Evolution.cpp:471-512): cos-only
+// rows c *= k; rotations c_S = c_S k + (s (-ph)) c_T, c_T = c_T k + (s ph) c_S from the pre-layer values.
+// With `snap_c` / `snap_r` set, the pre-layer values are kept for the adjoint sweep.
+__global__ __launch_bounds__(kBlock) void k_replay(const uint32_t *__restrict__ gcos, const uint2 *__restrict__ grot,
+                                                   uint64_t cb, uint64_t ncos, uint64_t rb, uint64_t nrot, double k,
+                                                   double s, double *__restrict__ c, double *__restrict__ snap_c,
+                                                   double *__restrict__ snap_r) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < ncos) {
+        const uint32_t r = gcos[cb + i];
+        const double v = c[r];
+        if (snap_c) snap_c[cb + i] = v;
+        c[r] = v * k;
+    } else if (i < ncos + nrot) {
+        const uint64_t j = rb + (i - ncos);
+        const uint2 e = grot[j];
+        const uint32_t S = e.x, T = e.y & ~kPhaseBit;
+        const int ph = (e.y & kPhaseBit) ? -1 : 1;
+        const double vS = c[S], vT = c[T];
+        if (snap_r) {
+            snap_r[2 * j] = vS;
+            snap_r[2 * j + 1] = vT;
+        }
+        c[S] = vS * k + s * static_cast<double>(-ph) * vT;
+        c[T] = vT * k + s * static_cast<double>(ph) * vS;
+    }
+}
+
+// Adjoint of one layer: d<O>/d alpha from the pre-layer snapshots, then lambda <- R^T lambda.
+// cos-only: -2 s c_i; pair (S,T,ph): -2s c_S - 2k ph c_T and -2s c_T + 2k ph c_S.  Per-block sums of the
+// derivative go to partial[pb + block].
+__global__ __launch_bounds__(kBlock) void k_adjoint(const uint32_t *__restrict__ gcos, const uint2 *__restrict__ grot,
+                                                    uint64_t cb, uint64_t ncos, uint64_t rb, uint64_t nrot, double k,
+                                                    double s, const double *__restrict__ snap_c,
+                                                    const double *__restrict__ snap_r, double *__restrict__ lam,
+                                                    double *__restrict__ partial, uint64_t pb) {
+    __shared__ double red[kBlock];
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    double d = 0.0;
+    if (i < ncos) {
+        const uint32_t r = gcos[cb + i];
+        const double l = lam[r];
+        d = l * (-2.0 * s * snap_c[cb + i]);
+        lam[r] = l * k;
+    } else if (i < ncos + nrot) {
+        const uint64_t j = rb + (i - ncos);
+        const uint2 e = grot[j];
+        const uint32_t S = e.x, T = e.y & ~kPhaseBit;
+        const double ph = (e.y & kPhaseBit) ? -1.0 : 1.0;
+        const double cs = snap_r[2 * j], ct = snap_r[2 * j + 1];
+        const double ls = lam[S], lt = lam[T];
+        d = ls * (-2.0 * s * cs - 2.0 * k * ph * ct);
+        d += lt * (-2.0 * s * ct + 2.0 * k * ph * cs);
+        lam[S] = k * ls + s * ph * lt;
+        lam[T] = k * lt - s * ph * ls;
+    }
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(st)) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[pb + blockIdx.x] = red[0];
+}
+
+// lambda_i = <ref| M_i |ref>: the diagonal phase of fully paired rows, 0 elsewhere.
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_init_lambda(const uint64_t *__restrict__ rows, uint64_t N,
+                                                        const uint64_t *__restrict__ occ_dev,
+                                                        double *__restrict__ lam) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= N) return;
+    uint64_t occ[NW], m[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
+    load_row<NW>(rows, static_cast<uint32_t>(i), m);
+    lam[i] = mpa::is_paired<NW, PAULI>(m) ? mpa::state_phase<NW, PAULI>(m, occ) : 0.0;
+}
+
+// Per-layer sum of the adjoint partials, in block order: one block per layer.
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
+                                                            const uint64_t *__restrict__ offs,
+                                                            double *__restrict__ out) {
+    __shared__ double red[kBlock];
+    const uint64_t a = offs[blockIdx.x], b = offs[blockIdx.x + 1];
+    double acc = 0.0;
+    for (uint64_t i = a + threadIdx.x; i < b; i += kBlock) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(st)) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -622,6 +788,20 @@ struct mpa_engine {
     DevArray rows, coef, cols;
     DevArray AL, lead, foll, blk_a, blk_b, rec, lv, match, miss, ins, table, counters, stats, gdesc, fold, occ_dev,
         partial;
+
+    // Graph (build_graph): per-layer cos-only rows and rotation records, the coefficient vector the
+    // seeded build evolves, the operator coefficients the graph starts from, and replay scratch.
+    struct LayerInfo {
+        uint64_t cos_begin, cos_end, rot_begin, rot_end;
+        int64_t param_index;
+        double gen_coeff;
+    };
+    std::vector<LayerInfo> layers;
+    int64_t graph_nparams = 0;
+    DevArray g_cos, g_rot, seedc, base, rep, snap_c, snap_r, lam, gpartial, goffs, gdl;
+    uint64_t base_n = 0;
+    uint64_t gcos_total = 0, grot_total = 0;  // records stored
+    bool building = false;                    // seedc tracks capacity while a graph is built
     uint64_t table_slots = 0;
     uint64_t epoch = 0;
     int64_t *h_counters = nullptr;  // pinned
@@ -735,6 +915,14 @@ struct mpa_engine {
         std::swap(coef.bytes, c2.bytes);
         std::swap(cols.p, k2.p);
         std::swap(cols.bytes, k2.bytes);
+        if (building) {
+            DevArray s2;
+            s2.reserve(nc * 8);
+            if (N > 0) HIP_TRY(hipMemcpyAsync(s2.p, seedc.p, N * 8, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            std::swap(seedc.p, s2.p);
+            std::swap(seedc.bytes, s2.bytes);
+        }
         cap = nc;
         // per-term scratch sized by capacity; the item list of the current gate is kept
         if (AL.bytes < cap * 4) {
@@ -769,6 +957,17 @@ struct mpa_engine {
         }
     }
 
+    // Grow `a` to `need` bytes keeping its contents.
+    void grow_keep(DevArray &a, size_t need) {
+        if (need <= a.bytes) return;
+        DevArray b;
+        b.reserve(std::max(need, a.bytes + a.bytes / 2));
+        if (a.p) HIP_TRY(hipMemcpyAsync(b.p, a.p, a.bytes, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::swap(a.p, b.p);
+        std::swap(a.bytes, b.bytes);
+    }
+
     void sync_counters() {
         HIP_TRY(hipMemcpyAsync(h_counters, counters.p, C_COUNT * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -786,7 +985,8 @@ struct mpa_engine {
     }
 
     template <int NW, int PAULI>
-    void propagate_impl(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds) {
+    void propagate_impl(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds, bool graph,
+                        Trunc trl) {
         const size_t ng = gds.size();
         gdesc.reserve(std::max<size_t>(ng, 1) * sizeof(GateDesc));
         fold.reserve(std::max<size_t>(folds.size(), 1) * 4);
@@ -800,6 +1000,13 @@ struct mpa_engine {
         HIP_TRY(hipEventCreate(&ev[1]));
         HIP_TRY(hipEventRecord(ev[0], stream));
         std::vector<int32_t> fold_plus_piv(ng, 0);
+        // in a graph build the seed vector is evolved and the operator's own coefficients get zeros
+        uint64_t gcos_ub = gcos_total, grot_ub = grot_total;
+        if (graph) {
+            const int64_t gc[2] = {static_cast<int64_t>(gcos_total), static_cast<int64_t>(grot_total)};
+            HIP_TRY(hipMemcpyAsync(counters.as<uint64_t>() + C_GCOS, gc, 16, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
         for (size_t gi = 0; gi < ng; ++gi) {
             const GateDesc &gd = gds[gi];
             const int igi = static_cast<int>(gi);
@@ -844,6 +1051,15 @@ struct mpa_engine {
             if (n_ub >= (uint64_t{1} << 32)) throw RunError("term count exceeds the 32-bit row index");
             ensure_capacity(n_ub);
             ensure_items(nL, nA);
+            double *cv = graph ? seedc.as<double>() : coef.as<double>();
+            GraphRec grec{nullptr, nullptr};
+            if (graph && nA > 0) {
+                gcos_ub += nA;
+                grot_ub += nA;
+                grow_keep(g_cos, gcos_ub * 4);
+                grow_keep(g_rot, grot_ub * 8);
+                grec = GraphRec{g_cos.as<uint32_t>(), g_rot.as<uint2>()};
+            }
             if (nA > 0) {
                 ++epoch;
                 const uint64_t tmask = table_slots - 1;
@@ -853,7 +1069,7 @@ struct mpa_engine {
                 if (lb) {
                     pa = prof_begin();
                     hipLaunchKernelGGL((k_leaders<NW, PAULI>), dim3(lb), dim3(kBlock), 0, stream,
-                                       gdesc.as<GateDesc>(), igi, tr, rows.as<uint64_t>(), coef.as<double>(),
+                                       gdesc.as<GateDesc>(), igi, trl, rows.as<uint64_t>(), cv,
                                        AL.as<uint32_t>(), static_cast<uint32_t>(nL), table.as<uint64_t>(), tmask,
                                        epoch, rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(),
                                        counters.as<uint64_t>());
@@ -863,20 +1079,20 @@ struct mpa_engine {
                 if (fb) {
                     pa = prof_begin();
                     hipLaunchKernelGGL((k_followers<NW, PAULI>), dim3(fb), dim3(kBlock), 0, stream,
-                                       gdesc.as<GateDesc>(), igi, tr, rows.as<uint64_t>(), coef.as<double>(),
+                                       gdesc.as<GateDesc>(), igi, trl, rows.as<uint64_t>(), cv,
                                        AL.as<uint32_t>(), static_cast<uint32_t>(nL), static_cast<uint32_t>(nF),
                                        table.as<uint64_t>(), tmask, epoch, rec.as<uint8_t>(), lv.as<double>(),
                                        match.as<uint8_t>(), miss.as<uint8_t>(), ins.as<double>(),
-                                       counters.as<uint64_t>());
+                                       counters.as<uint64_t>(), grec);
                     HIP_TRY(hipGetLastError());
                     prof_end(K_FOLLOWERS, pa);
                 }
                 if (lb) {
                     pa = prof_begin();
                     hipLaunchKernelGGL(k_leaders_unmatched, dim3(lb), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                       igi, coef.as<double>(), AL.as<uint32_t>(), static_cast<uint32_t>(nL),
+                                       igi, cv, AL.as<uint32_t>(), static_cast<uint32_t>(nL),
                                        rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(), miss.as<uint8_t>(),
-                                       ins.as<double>());
+                                       ins.as<double>(), counters.as<uint64_t>(), grec);
                     HIP_TRY(hipGetLastError());
                     prof_end(K_UNMATCHED, pa);
                 }
@@ -892,9 +1108,9 @@ struct mpa_engine {
                 prof_end(K_SCANSUM, pa);
                 pa = prof_begin();
                 hipLaunchKernelGGL((k_insert<NW>), dim3(ab), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), igi,
-                                   rows.as<uint64_t>(), coef.as<double>(), AL.as<uint32_t>(),
-                                   static_cast<uint32_t>(nA), miss.as<uint8_t>(), ins.as<double>(),
-                                   blk_a.as<uint32_t>(), counters.as<uint64_t>());
+                                   rows.as<uint64_t>(), cv, graph ? coef.as<double>() : static_cast<double *>(nullptr),
+                                   AL.as<uint32_t>(), static_cast<uint32_t>(nA), miss.as<uint8_t>(), ins.as<double>(),
+                                   blk_a.as<uint32_t>(), counters.as<uint64_t>(), grec);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_INSERT, pa);
                 pa = prof_begin();
@@ -929,13 +1145,91 @@ struct mpa_engine {
             stat_p[g] = st[kStatW * g + 5];
         }
         if (prof_on) add_algorithmic_bytes(fold_plus_piv);
+        if (graph) {
+            uint64_t cb = gcos_total, rb = grot_total;
+            for (size_t g = 0; g < ng; ++g) {
+                const uint64_t ce = static_cast<uint64_t>(st[kStatW * g + 6]);
+                const uint64_t re = static_cast<uint64_t>(st[kStatW * g + 7]);
+                layers.push_back({cb, ce, rb, re, gds[g].param_index, gds[g].gen_coeff});
+                cb = ce;
+                rb = re;
+            }
+            gcos_total = cb;
+            grot_total = rb;
+        }
+    }
+
+    // Replay the stored graph at `params` into `rep` (evolve_operator, Evolution.cpp:529-540); with
+    // `snapshots` the pre-layer values of every record are kept for the adjoint sweep.
+    void replay(const double *params, bool snapshots) {
+        rep.reserve(std::max<uint64_t>(N, 1) * 8);
+        if (base_n) HIP_TRY(hipMemcpyAsync(rep.p, base.p, base_n * 8, hipMemcpyDeviceToDevice, stream));
+        if (N > base_n) HIP_TRY(hipMemsetAsync(rep.as<double>() + base_n, 0, (N - base_n) * 8, stream));
+        if (snapshots) {
+            snap_c.reserve(std::max<uint64_t>(gcos_total, 1) * 8);
+            snap_r.reserve(std::max<uint64_t>(grot_total, 1) * 16);
+        }
+        for (const LayerInfo &ly : layers) {
+            const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
+            if (nc + nr == 0) continue;
+            const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
+            const uint32_t blocks = static_cast<uint32_t>((nc + nr + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, stream, g_cos.as<uint32_t>(),
+                               g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr, std::cos(2 * alpha),
+                               std::sin(2 * alpha), rep.as<double>(), snapshots ? snap_c.as<double>() : nullptr,
+                               snapshots ? snap_r.as<double>() : nullptr);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+
+    // d<O>/d params: adjoint sweep over the layers in reverse (after replay(params, true)).
+    template <int NW, int PAULI>
+    void gradient(const double *params, int64_t n_params, double *grad) {
+        lam.reserve(std::max<uint64_t>(N, 1) * 8);
+        if (N) {
+            hipLaunchKernelGGL((k_init_lambda<NW, PAULI>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)),
+                               dim3(kBlock), 0, stream, rows.as<uint64_t>(), N, occ_dev.as<uint64_t>(),
+                               lam.as<double>());
+            HIP_TRY(hipGetLastError());
+        }
+        const size_t nl = layers.size();
+        std::vector<uint64_t> offs(nl + 1, 0);
+        for (size_t l = 0; l < nl; ++l) {
+            const uint64_t items = (layers[l].cos_end - layers[l].cos_begin) + (layers[l].rot_end - layers[l].rot_begin);
+            offs[l + 1] = offs[l] + (items + kBlock - 1) / kBlock;
+        }
+        gpartial.reserve(std::max<uint64_t>(offs[nl], 1) * 8);
+        goffs.reserve((nl + 1) * 8);
+        gdl.reserve(std::max<size_t>(nl, 1) * 8);
+        for (size_t l = nl; l-- > 0;) {
+            const LayerInfo &ly = layers[l];
+            const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
+            if (nc + nr == 0) continue;
+            const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
+            hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(offs[l + 1] - offs[l])), dim3(kBlock), 0, stream,
+                               g_cos.as<uint32_t>(), g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr,
+                               std::cos(2 * alpha), std::sin(2 * alpha), snap_c.as<double>(), snap_r.as<double>(),
+                               lam.as<double>(), gpartial.as<double>(), offs[l]);
+            HIP_TRY(hipGetLastError());
+        }
+        std::vector<double> dl(nl, 0.0);
+        if (nl) {
+            HIP_TRY(hipMemcpyAsync(goffs.p, offs.data(), (nl + 1) * 8, hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(k_reduce_partials, dim3(static_cast<uint32_t>(nl)), dim3(kBlock), 0, stream,
+                               gpartial.as<double>(), goffs.as<uint64_t>(), gdl.as<double>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(dl.data(), gdl.p, nl * 8, hipMemcpyDeviceToHost, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (int64_t p = 0; p < n_params; ++p) grad[p] = 0.0;
+        for (size_t l = nl; l-- > 0;) grad[layers[l].param_index] += layers[l].gen_coeff * dl[l];
     }
 
     template <int NW, int PAULI>
-    double expect_impl() {
+    double expect_impl(const double *c) {
         partial.reserve(kExpectBlocks * 8);
         hipLaunchKernelGGL((k_expect<NW, PAULI>), dim3(kExpectBlocks), dim3(kBlock), 0, stream, rows.as<uint64_t>(),
-                           coef.as<double>(), N, occ_dev.as<uint64_t>(), partial.as<double>());
+                           c, N, occ_dev.as<uint64_t>(), partial.as<double>());
         HIP_TRY(hipGetLastError());
         std::vector<double> h(kExpectBlocks);
         HIP_TRY(hipMemcpyAsync(h.data(), partial.p, kExpectBlocks * 8, hipMemcpyDeviceToHost, stream));
@@ -1079,20 +1373,91 @@ struct RunPropagate {
     mpa_engine *e;
     const std::vector<GateDesc> &gds;
     const std::vector<int32_t> &folds;
+    bool graph;
+    Trunc trl;
     template <int NW, int PAULI>
     void operator()() {
-        e->propagate_impl<NW, PAULI>(gds, folds);
+        e->propagate_impl<NW, PAULI>(gds, folds, graph, trl);
     }
 };
 
 struct RunExpect {
     mpa_engine *e;
+    const double *c;
     double *out;
     template <int NW, int PAULI>
     void operator()() {
-        *out = e->expect_impl<NW, PAULI>();
+        *out = e->expect_impl<NW, PAULI>(c);
     }
 };
+
+struct RunGradient {
+    mpa_engine *e;
+    const double *params;
+    int64_t n_params;
+    double *grad;
+    template <int NW, int PAULI>
+    void operator()() {
+        e->gradient<NW, PAULI>(params, n_params, grad);
+    }
+};
+
+// Layer arrays (authoring order) -> per-gate descriptors in processing order (Heisenberg: back to
+// front, MonomialPropagator.inl:720-733).  alpha = (1.0 * theta[map]) * g (MPFunctions.cpp:57-69);
+// without params (structural graph build) alpha = 0.  `fused`: the contract-immediately cos sweep.
+void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
+                const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
+                bool fused_allowed, std::vector<GateDesc> &gds, std::vector<int32_t> &folds) {
+    gds.clear();
+    folds.clear();
+    gds.reserve(n_layers);
+    std::vector<uint64_t> g(mpa::kMaxWords);
+    for (int64_t i = 0; i < n_layers; ++i) {
+        const int64_t idx = n_layers - 1 - i;
+        if (param_map[idx] < 0 || (params != nullptr && param_map[idx] >= n_params))
+            throw ArgError("parameter index out of range for the given parameters");
+        GateDesc gd{};
+        std::fill(g.begin(), g.end(), 0);
+        e->slots_to_words(gen_slots + gen_offsets[idx], gen_offsets[idx + 1] - gen_offsets[idx], g.data());
+        std::copy(g.begin(), g.begin() + e->nw, gd.g);
+        gd.param_index = param_map[idx];
+        gd.gen_coeff = gen_coeffs[idx];
+        if (params != nullptr) {
+            const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
+            gd.k = std::cos(2 * alpha);
+            gd.s = std::sin(2 * alpha);
+            gd.abs_s = std::abs(std::sin(2 * alpha));
+            const double cos_build = std::cos(2.0 * alpha);
+            gd.fused = (fused_allowed && cos_build != 0.0) ? 1 : 0;
+            gd.inv_k = gd.fused ? 1.0 / cos_build : 1.0;
+        } else {
+            gd.k = 1.0;
+            gd.s = 0.0;
+            gd.abs_s = 1.0;
+            gd.fused = 0;
+            gd.inv_k = 1.0;
+        }
+        gd.fold_off = static_cast<int32_t>(folds.size());
+        e->gate_columns(gd.g, folds, gd.piv0, gd.piv1);
+        gd.fold_cnt = static_cast<int32_t>(folds.size()) - gd.fold_off;
+        gd.identity = gd.fold_cnt == 0 ? 1 : 0;
+        gds.push_back(gd);
+    }
+}
+
+// The coefficient vector the current picture evaluates: the replay at params when a graph is stored
+// (n_params must cover it), else the contracted operator (n_params must be 0).
+const double *evaluated_coeffs(mpa_engine *e, const double *params, int64_t n_params, bool snapshots) {
+    if (e->layers.empty()) {
+        if (n_params != 0) throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
+        return e->coef.as<double>();
+    }
+    if (n_params != e->graph_nparams)
+        throw ArgError("Expected " + std::to_string(e->graph_nparams) + " parameter value(s), got " +
+                       std::to_string(n_params) + ".");
+    e->replay(params, snapshots);
+    return e->rep.as<double>();
+}
 
 void set_trunc(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower, double upper) {
     if (e->basis == MPA_BASIS_PAULI && cutoff_type != MPA_CUTOFF_SUPPORT)
@@ -1211,66 +1576,115 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
     return guarded([&] {
         if (only_rotate_len_k == 0 || only_rotate_len_k > 2 * e->n_modes)
             throw ArgError("only_rotate_len_k is out of range; must be 0 < k <= 2*num_modes");
+        if (!e->layers.empty())
+            throw RunError("Cannot propagate() on top of a non-empty graph of " + std::to_string(e->layers.size()) +
+                           " layer(s): propagate() evolves and contracts in place and assumes no stored graph.");
+        if (n_layers == 0) return;
         std::vector<GateDesc> gds;
         std::vector<int32_t> folds;
-        gds.reserve(n_layers);
-        std::vector<uint64_t> g(mpa::kMaxWords);
-        // Heisenberg: authoring order back to front (MonomialPropagator.inl:720-733)
-        for (int64_t i = 0; i < n_layers; ++i) {
-            const int64_t idx = n_layers - 1 - i;
-            if (param_map[idx] < 0 || param_map[idx] >= n_params)
-                throw ArgError("parameter index out of range for the given parameters");
-            GateDesc gd{};
-            std::fill(g.begin(), g.end(), 0);
-            e->slots_to_words(gen_slots + gen_offsets[idx], gen_offsets[idx + 1] - gen_offsets[idx], g.data());
-            std::copy(g.begin(), g.begin() + e->nw, gd.g);
-            // alpha = (1.0 * theta) * g (MPFunctions.cpp:57-69); the angle functions are the host libm's
-            const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
-            gd.k = std::cos(2 * alpha);
-            gd.s = std::sin(2 * alpha);
-            gd.abs_s = std::abs(std::sin(2 * alpha));
-            const double cos_build = std::cos(2.0 * alpha);
-            gd.fused = (only_rotate_len_k < 0 && cos_build != 0.0) ? 1 : 0;
-            gd.inv_k = gd.fused ? 1.0 / cos_build : 1.0;
-            gd.fold_off = static_cast<int32_t>(folds.size());
-            e->gate_columns(gd.g, folds, gd.piv0, gd.piv1);
-            gd.fold_cnt = static_cast<int32_t>(folds.size()) - gd.fold_off;
-            gd.identity = gd.fold_cnt == 0 ? 1 : 0;
-            gds.push_back(gd);
-        }
-        e->tr.only_rot = only_rotate_len_k < 0 ? -1 : only_rotate_len_k;
-        RunPropagate run{e, gds, folds};
+        make_gates(e, gen_offsets, gen_slots, n_layers, param_map, gen_coeffs, params, n_params,
+                   only_rotate_len_k < 0, gds, folds);
+        Trunc trl = e->tr;
+        trl.only_rot = only_rotate_len_k < 0 ? -1 : only_rotate_len_k;
+        RunPropagate run{e, gds, folds, false, trl};
         e->dispatch(run);
-        e->tr.only_rot = -1;
     });
 }
 
-int mpa_build_graph(mpa_engine *, const int64_t *, const int32_t *, int64_t, const int64_t *, const double *,
-                    const double *, int64_t) {
-    return set_error(-2, "build_graph is not available in this build");
+int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
+                    const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params) {
+    return guarded([&] {
+        if (n_layers == 0) return;
+        std::vector<GateDesc> gds;
+        std::vector<int32_t> folds;
+        make_gates(e, gen_offsets, gen_slots, n_layers, param_map, gen_coeffs, params, n_params, false, gds, folds);
+        // the seed the coefficient-informed build evolves (MonomialPropagator.inl:653-689)
+        e->seedc.reserve(std::max<uint64_t>(e->cap, 1) * 8);
+        if (e->layers.empty()) {
+            e->base.reserve(std::max<uint64_t>(e->N, 1) * 8);
+            if (e->N) HIP_TRY(hipMemcpyAsync(e->base.p, e->coef.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+            e->base_n = e->N;
+            if (e->N) HIP_TRY(hipMemcpyAsync(e->seedc.p, e->coef.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+        } else if (params != nullptr) {
+            if (n_params < e->graph_nparams)
+                throw ArgError("Coefficient-informed build_graph() needs at least " +
+                               std::to_string(e->graph_nparams) + " parameter value(s) to replay the existing graph.");
+            e->replay(params, false);
+            if (e->N) HIP_TRY(hipMemcpyAsync(e->seedc.p, e->rep.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+        } else if (e->N) {
+            HIP_TRY(hipMemcpyAsync(e->seedc.p, e->coef.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        Trunc trl = e->tr;
+        trl.only_rot = -1;
+        if (params == nullptr) {  // structural build: no coefficient checks (Scan.h:81-95)
+            trl.check_lower = 0;
+            trl.check_upper = 0;
+        }
+        e->building = true;
+        try {
+            RunPropagate run{e, gds, folds, true, trl};
+            e->dispatch(run);
+        } catch (...) {
+            e->building = false;
+            throw;
+        }
+        e->building = false;
+        for (int64_t i = 0; i < n_layers; ++i) e->graph_nparams = std::max(e->graph_nparams, param_map[i] + 1);
+    });
 }
 
-int mpa_expectation_value(mpa_engine *e, const double *, int64_t n_params, double *out) {
+int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out) {
     return guarded([&] {
-        if (n_params != 0)
-            throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
-        RunExpect ev{e, out};
+        const double *c = evaluated_coeffs(e, params, n_params, false);
+        RunExpect ev{e, c, out};
         e->dispatch(ev);
     });
 }
 
 int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int64_t n_params, double *value,
-                                       double *) {
-    return mpa_expectation_value(e, params, n_params, value);
+                                       double *grad) {
+    return guarded([&] {
+        const double *c = evaluated_coeffs(e, params, n_params, true);
+        RunExpect ev{e, c, value};
+        e->dispatch(ev);
+        if (!e->layers.empty()) {
+            RunGradient rg{e, params, n_params, grad};
+            e->dispatch(rg);
+        }
+    });
+}
+
+int mpa_contract(mpa_engine *e, const double *params, int64_t n_params, int inplace, double *out) {
+    return guarded([&] {
+        const double *c = evaluated_coeffs(e, params, n_params, false);
+        if (out && e->N) {
+            HIP_TRY(hipMemcpyAsync(out, c, e->N * 8, hipMemcpyDeviceToHost, e->stream));
+        }
+        if (inplace && !e->layers.empty()) {
+            if (e->N) HIP_TRY(hipMemcpyAsync(e->coef.p, c, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+            e->layers.clear();
+            e->graph_nparams = 0;
+            e->gcos_total = e->grot_total = 0;
+            e->base_n = 0;
+        }
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    });
 }
 
 int mpa_size(mpa_engine *e, int64_t *out) {
     return guarded([&] { *out = static_cast<int64_t>(e->N); });
 }
 
-int mpa_graph_layers(mpa_engine *, int64_t *layers, int64_t *n_params) {
-    *layers = 0;
-    *n_params = 0;
+int mpa_graph_layers(mpa_engine *e, int64_t *layers, int64_t *n_params) {
+    *layers = static_cast<int64_t>(e->layers.size());
+    *n_params = e->graph_nparams;
+    return 0;
+}
+
+int mpa_graph_records(mpa_engine *e, int64_t *cos_only, int64_t *rotations) {
+    *cos_only = static_cast<int64_t>(e->gcos_total);
+    *rotations = static_cast<int64_t>(e->grot_total);
     return 0;
 }
 
@@ -1288,16 +1702,15 @@ int mpa_read_rows(mpa_engine *e, uint64_t *words, double *coeffs) {
     });
 }
 
-int mpa_export_terms(mpa_engine *e, const double *, int64_t n_params, double atol, int64_t *n_terms,
+int mpa_export_terms(mpa_engine *e, const double *params, int64_t n_params, double atol, int64_t *n_terms,
                      int64_t *n_slots, int64_t *offsets, int32_t *slots, double *re, double *im) {
     return guarded([&] {
-        if (n_params != 0)
-            throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
+        const double *cdev = evaluated_coeffs(e, params, n_params, false);
         std::vector<uint64_t> w(e->N * e->nw);
         std::vector<double> c(e->N);
         if (e->N) {
             HIP_TRY(hipMemcpyAsync(w.data(), e->rows.p, e->N * e->nw * 8, hipMemcpyDeviceToHost, e->stream));
-            HIP_TRY(hipMemcpyAsync(c.data(), e->coef.p, e->N * 8, hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipMemcpyAsync(c.data(), cdev, e->N * 8, hipMemcpyDeviceToHost, e->stream));
             HIP_TRY(hipStreamSynchronize(e->stream));
         }
         std::vector<int32_t> tmp;
@@ -1317,9 +1730,7 @@ int mpa_export_terms(mpa_engine *e, const double *, int64_t n_params, double ato
             ++nt;
             ns += static_cast<int64_t>(tmp.size());
         }
-        if (slots) {
-            offsets[nt] = ns;
-        }
+        if (slots) offsets[nt] = ns;
         // core term (binder.h:213-215)
         if (std::abs(e->core) >= atol) {
             if (slots) {

@@ -191,6 +191,13 @@ class MonomialPropagator(ABC, Generic[T_op]):
     def expval_and_grad_functional(self, pare_threshold: float | None = None):
         return self.expectation_value_and_gradient_functional(pare_threshold)
 
+    def contract_partially(self, parameters: ParameterValues = None, *, inplace: bool = True) -> np.ndarray:
+        """Evolved coefficients at ``parameters`` (core term excluded); ``inplace`` folds the graph in."""
+        coeffs = np.asarray(self._simulator.contract_partially(self._bind(parameters), inplace))
+        if inplace:
+            self._n_params = max(self._simulator.parameter_mapping, default=-1) + 1
+        return coeffs
+
     @abstractmethod
     def evolved_operator(self, parameters: ParameterValues = None, *, atol: float = 1e-12) -> T_op:
         """The evolved operator with terms ``|c| >= atol``, coefficients rounded to 1e-12."""

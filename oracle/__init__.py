@@ -52,6 +52,8 @@ def _lib():
         lib.orc_expectation.argtypes = [P]
         lib.orc_graph_ev.restype = C.c_int
         lib.orc_graph_ev.argtypes = [P, f64p, C.c_int64, f64p]
+        lib.orc_graph_contract.restype = C.c_int
+        lib.orc_graph_contract.argtypes = [P, f64p, C.c_int64, f64p]
         lib.orc_graph_grad.restype = C.c_int
         lib.orc_graph_grad.argtypes = [P, f64p, C.c_int64, f64p, f64p]
         lib.orc_rows.argtypes = [P, C.POINTER(C.c_uint64), f64p]
@@ -167,6 +169,14 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(_lib().orc_last_error().decode())
         return out.value
+
+    def graph_contract(self, parameters) -> np.ndarray:
+        th = np.asarray(list(parameters) or [0.0], dtype=np.float64)
+        out = np.zeros(max(self.size(), 1), dtype=np.float64)
+        rc = _lib().orc_graph_contract(self._h, _ptr(th, C.c_double), len(list(parameters)), _ptr(out, C.c_double))
+        if rc != 0:
+            raise RuntimeError(_lib().orc_last_error().decode())
+        return out[: self.size()]
 
     def graph_grad(self, parameters):
         th = np.asarray(list(parameters), dtype=np.float64)

@@ -736,6 +736,21 @@ int orc_graph_ev(void *h, const double *params, int64_t n_params, double *out) {
     return 0;
 }
 
+// Replayed coefficient vector at params (contract_partially(params, inplace=False)).
+int orc_graph_contract(void *h, const double *params, int64_t n_params, double *out) {
+    auto *e = static_cast<Engine *>(h);
+    std::vector<double> op = e->init_coeffs;
+    op.resize(e->size(), 0.0);
+    std::vector<uint32_t> cos;
+    for (const Layer &ly : e->graph) {
+        if (ly.param_index >= n_params) return fail("parameters too short");
+        e->layer_cos(ly, cos);
+        e->evolve_step(op, ly, cos, 1.0 * params[ly.param_index] * ly.gen_coeff);
+    }
+    std::memcpy(out, op.data(), op.size() * sizeof(double));
+    return 0;
+}
+
 // Reverse-mode gradient of the replay (exact, no 1/cos un-evolution): the forward replay keeps the
 // pre-layer values of the entries each layer touches (its cos set and rotation endpoints); the reverse
 // sweep applies the transposed layer to the adjoint vector lambda.

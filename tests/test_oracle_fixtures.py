@@ -95,3 +95,22 @@ def test_seeded_graph_matches_propagate(oracle_mod, name, upper, lower, cutoff):
     b.build_graph(monos, pm, gcs, p.parameters)
     assert a.size() == b.size()
     assert abs(a.expectation() - b.graph_ev(p.parameters)) <= 1e-12
+
+
+def test_ring_gradient_parameter_shift(oracle_mod):
+    """64-qubit brickwork ring, 2 layers: the oracle adjoint equals the frozen-structure parameter shift
+    dE/dtheta = E(theta + pi/4) - E(theta - pi/4) (every gate has g = 1 and its own angle)."""
+    from monoprop_amd import workloads
+
+    circ = workloads.brickwork_ring(64, 2, seed=0)
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, 64)
+    o = oracle_mod.Oracle(workloads.ring_zz_mean(64).get_local_operator().terms, 64, basis="pauli", cutoff=8,
+                          lower_atol=1e-6)
+    o.build_graph(monos, pm, gcs, circ.parameters)
+    theta = np.asarray(circ.parameters)
+    val, grad = o.graph_grad(theta)
+    assert abs(val - o.graph_ev(theta)) <= 1e-14
+    for i in np.random.default_rng(1).choice(len(theta), 12, replace=False):
+        e = np.zeros_like(theta)
+        e[i] = np.pi / 4
+        assert abs((o.graph_ev(theta + e) - o.graph_ev(theta - e)) - grad[i]) <= 1e-12
