@@ -108,6 +108,12 @@ int mpa_set_profiling(mpa_engine *e, int on);
 int mpa_kernel_profile(mpa_engine *e, int kind, const char **name, double *ms, double *bytes,
                        int64_t *launches);
 
+/* Pipeline of mpa_propagate (no counterpart in the reference): fused != 0 (default) runs the
+ * single-sweep pipeline whenever every gate takes the reference's fused cos sweep (no orbital cap,
+ * cos 2a != 0); 0 forces the per-gate pipeline that also serves build_graph.  Both produce the same
+ * store and coefficients. */
+int mpa_set_pipeline(mpa_engine *e, int fused);
+
 /* Device time of the last propagate call from first to last launch (HIP events), in ms. */
 int mpa_last_propagate_ms(mpa_engine *e, double *ms);
 

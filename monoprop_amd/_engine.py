@@ -46,6 +46,7 @@ _SIGNATURES = {
     "mpa_words_per_row": (C.c_int, [_P, _i32p]),
     "mpa_read_rows": (C.c_int, [_P, _u64p, _f64p]),
     "mpa_gate_stats": (C.c_int, [_P, _i64p, _i64p, _i64p, _i64p, _i64p]),
+    "mpa_set_pipeline": (C.c_int, [_P, C.c_int]),
     "mpa_set_profiling": (C.c_int, [_P, C.c_int]),
     "mpa_kernel_profile": (C.c_int, [_P, C.c_int, C.POINTER(C.c_char_p), _f64p, _f64p, _i64p]),
     "mpa_last_propagate_ms": (C.c_int, [_P, _f64p]),
@@ -325,6 +326,10 @@ class DeviceSimulator:
         arrs = [np.zeros(max(ln.value, 1), dtype=np.int64) for _ in range(4)]
         _check(lib.mpa_gate_stats(self._h, C.byref(ln), *[_ptr(a, C.c_int64) for a in arrs]))
         return tuple(a[: ln.value] for a in arrs)
+
+    def set_pipeline(self, fused: bool) -> None:
+        """Select the fused single-sweep pipeline (default) or the per-gate pipeline for propagate()."""
+        _check(load_library().mpa_set_pipeline(self._h, 1 if fused else 0))
 
     def set_profiling(self, on: bool) -> None:
         """Bracket every kernel launch with HIP events (resets the accumulators)."""

@@ -53,7 +53,9 @@ constexpr int kExpectBlocks = 1024;
 // counters[] on the device
 enum : int {
     C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6,
+    C_STOP = 7,              // fused pipeline: 1 + index of the gate to resume at (0 = running)
     C_GCOS = 8, C_GROT = 9,  // graph record counts (cumulative over a build)
+    C_NA = 10,               // fused pipeline: anticommuting terms of the gate
     C_COUNT = 16
 };
 // per-gate stats: N, nA, emitted, inserted, leaders, pairs, graph cos-record end, graph rotation end
@@ -62,10 +64,11 @@ constexpr uint32_t kPhaseBit = 0x80000000u;  // rotation record: target row | (p
 
 // Kernel classes timed by the profiler (mpa_kernel_profile).
 enum KernelKind : int { K_SCAN, K_SCANSUM, K_EXPAND, K_LEADERS, K_FOLLOWERS, K_UNMATCHED, K_COUNT, K_INSERT,
-                        K_COLS, K_ADVANCE, K_KINDS };
+                        K_COLS, K_ADVANCE, K_SWEEP, K_RESOLVE_L, K_RESOLVE_F, K_APPEND, K_KINDS };
 const char *const kKindNames[K_KINDS] = {"k_scan", "k_scan_sums", "k_expand", "k_leaders", "k_followers",
                                          "k_leaders_unmatched", "k_count_miss", "k_insert", "k_cols_update",
-                                         "k_advance"};
+                                         "k_advance", "k_sweep", "k_resolve_leaders", "k_resolve_followers",
+                                         "k_append"};
 
 thread_local std::string g_error;
 
@@ -110,7 +113,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-// Exclusive scan across a block of kBlock threads; returns the exclusive prefix, total in *total.
+// Exclusive scan across a block of WAVES wavefronts; returns the exclusive prefix, total in *total.
+template <int WAVES = kWaves>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, uint32_t *total) {
     const uint32_t incl = wave_incl_scan(v);
     const uint32_t wave = threadIdx.x >> 6;
@@ -118,7 +122,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, u
     __syncthreads();
     uint32_t wave_base = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < WAVES; ++w) {
         const uint32_t s = lds[w];
         if (static_cast<uint32_t>(w) < wave) wave_base += s;
         tot += s;
@@ -211,6 +215,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t *__restrict__
                                                           int ta, int tb, int zero_from, int zero_to) {
     __shared__ uint32_t lds[kScanBlock / 64];
     __shared__ uint32_t carry_a, carry_b;
+    if (counters[C_STOP]) return;
     if (threadIdx.x == 0) {
         carry_a = 0;
         carry_b = 0;
@@ -255,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(const uint64_t *__restrict__ 
                                                    const uint64_t *__restrict__ counters,
                                                    uint32_t *__restrict__ AL) {
     __shared__ uint32_t lds[kWaves];
+    if (counters[C_STOP]) return;
     const uint64_t N = counters[C_N];
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t nL = counters[C_NL];
@@ -559,48 +565,51 @@ __global__ __launch_bounds__(kBlock) void k_cols_update(const uint64_t *__restri
                                                         uint64_t *__restrict__ cols, uint64_t colstride,
                                                         int n_modes, const uint64_t *__restrict__ counters,
                                                         uint64_t start_arg, uint64_t count_arg, int from_counters) {
+    if (from_counters && counters[C_STOP]) return;
     const uint64_t start = from_counters ? counters[C_N] : start_arg;
     const uint64_t count = from_counters ? counters[C_NINS] : count_arg;
     if (count == 0) return;
     const uint64_t w0 = start / 64;
-    const uint64_t w = w0 + static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
     const uint64_t wend = (start + count + 63) / 64;
-    if (w >= wend) return;  // whole wave exits together
-    const uint64_t r = w * 64 + lane_id();
-    uint64_t m[NW];
-    const bool live = r >= start && r < start + count;
-#pragma unroll
-    for (int k = 0; k < NW; ++k) m[k] = live ? rows[r * NW + k] : 0;
     const bool lane0 = lane_id() == 0;
-    if (PAULI) {
-        constexpr int L = NW / 2;
+    // grid-stride over words; a whole wave shares each word
+    for (uint64_t w = w0 + static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6); w < wend;
+         w += static_cast<uint64_t>(gridDim.x) * kWaves) {
+        const uint64_t r = w * 64 + lane_id();
+        uint64_t m[NW];
+        const bool live = r >= start && r < start + count;
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-            for (int bit = 0; bit < 64; ++bit) {
-                const int q = l * 64 + bit;
-                if (q >= n_modes) break;
-                const uint64_t bx = __ballot((m[l] >> bit) & 1u);
-                const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
-                if (lane0) {
-                    if (bx) cols[static_cast<uint64_t>(q) * colstride + w] |= bx;
-                    if (bz) cols[static_cast<uint64_t>(n_modes + q) * colstride + w] |= bz;
+        for (int k = 0; k < NW; ++k) m[k] = live ? rows[r * NW + k] : 0;
+        if (PAULI) {
+            constexpr int L = NW / 2;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                for (int bit = 0; bit < 64; ++bit) {
+                    const int q = l * 64 + bit;
+                    if (q >= n_modes) break;
+                    const uint64_t bx = __ballot((m[l] >> bit) & 1u);
+                    const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
+                    if (lane0) {
+                        if (bx) cols[static_cast<uint64_t>(q) * colstride + w] |= bx;
+                        if (bz) cols[static_cast<uint64_t>(n_modes + q) * colstride + w] |= bz;
+                    }
                 }
             }
-        }
-    } else {
-        int pop = 0;
+        } else {
+            int pop = 0;
 #pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            pop += __popcll(m[k]);
-            for (int bit = 0; bit < 64; ++bit) {
-                const int s = k * 64 + bit;
-                if (s >= 2 * n_modes) break;
-                const uint64_t b = __ballot((m[k] >> bit) & 1u);
-                if (lane0 && b) cols[static_cast<uint64_t>(s) * colstride + w] |= b;
+            for (int k = 0; k < NW; ++k) {
+                pop += __popcll(m[k]);
+                for (int bit = 0; bit < 64; ++bit) {
+                    const int sl = k * 64 + bit;
+                    if (sl >= 2 * n_modes) break;
+                    const uint64_t bb = __ballot((m[k] >> bit) & 1u);
+                    if (lane0 && bb) cols[static_cast<uint64_t>(sl) * colstride + w] |= bb;
+                }
             }
+            const uint64_t bp = __ballot(pop & 1);
+            if (lane0 && bp) cols[static_cast<uint64_t>(2 * n_modes) * colstride + w] |= bp;
         }
-        const uint64_t bp = __ballot(pop & 1);
-        if (lane0 && bp) cols[static_cast<uint64_t>(2 * n_modes) * colstride + w] |= bp;
     }
 }
 
@@ -624,6 +633,252 @@ __global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__
     counters[C_NEMIT] = 0;
     counters[C_NPAIR] = 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Fused contract-immediately pipeline (the default propagate: no orbital cap, cos != 0).
+//
+// A global open-addressing index over every stored row answers "is M^G stored?" for the emitting
+// terms only.  Slot = (fingerprint << 32) | (row + 1), 0 = empty; a fingerprint match is confirmed
+// against the stored row.
+
+template <int NW>
+__device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_t gmask, const uint64_t *key,
+                                            uint32_t row) {
+    const uint64_t h = mpa::hash_key<NW>(key);
+    const uint64_t val = (h & 0xffffffff00000000ull) | (static_cast<uint64_t>(row) + 1);
+    uint64_t pos = h & gmask;
+    for (uint64_t probe = 0; probe <= gmask; ++probe) {
+        const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(gidx + pos), 0ull, val);
+        if (old == 0ull) return true;
+        pos = (pos + 1) & gmask;
+    }
+    return false;
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t gidx_find(const uint64_t *__restrict__ gidx, uint64_t gmask,
+                                             const uint64_t *__restrict__ rows, const uint64_t *key) {
+    const uint64_t h = mpa::hash_key<NW>(key);
+    const uint64_t fp = h & 0xffffffff00000000ull;
+    uint64_t pos = h & gmask;
+    for (uint64_t probe = 0; probe <= gmask; ++probe) {
+        const uint64_t v = gidx[pos];
+        if (v == 0) return -1;
+        if ((v & 0xffffffff00000000ull) == fp) {
+            const uint64_t r = (v & 0xffffffffull) - 1;
+            const uint64_t *pr = rows + r * NW;
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) eq = eq && (pr[w] == key[w]);
+            if (eq) return static_cast<int64_t>(r);
+        }
+        pos = (pos + 1) & gmask;
+    }
+    return -1;
+}
+
+// Index rows [start, start + count).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_gidx_build(const uint64_t *__restrict__ rows, uint64_t start,
+                                                       uint64_t count, uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                       uint64_t *__restrict__ counters) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= count) return;
+    uint64_t m[NW];
+    load_row<NW>(rows, static_cast<uint32_t>(start + i), m);
+    if (!gidx_insert<NW>(gidx, gmask, m, static_cast<uint32_t>(start + i)))
+        atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
+}
+
+// Anticommutation + fused cos sweep + emission gate in one pass (Scan.h:384-401).  One wave per 64
+// column words; for every word holding anticommuting terms the wave processes its 64 terms one per
+// lane: c <- c k (the reference's in-place sweep), and terms past the lower_atol gate are tested
+// against the structural cutoff of their partner.  Emitters keep their pre-gate value in vsrc[] and
+// are marked in the leader / follower emit masks (pivot clear / set).
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
+                                                  const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                  const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
+                                                  const uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                  double *__restrict__ vsrc, uint64_t *__restrict__ elead,
+                                                  uint64_t *__restrict__ efoll, uint32_t *__restrict__ blk_l,
+                                                  uint32_t *__restrict__ blk_f, uint64_t *__restrict__ counters) {
+    __shared__ uint32_t lds[kWaves];
+    if (counters[C_STOP]) return;
+    const GateDesc &gd = gdesc[gi];
+    const uint64_t N = counters[C_N];
+    const uint64_t nwords = (N + 63) / 64;
+    const uint32_t lane = lane_id();
+    const uint64_t wbase = static_cast<uint64_t>(blockIdx.x) * kBlock + (threadIdx.x & ~63u);
+    const uint64_t w = wbase + lane;
+    uint64_t a = 0, p = 0;
+    if (w < nwords) {
+        for (int j = 0; j < nfold; ++j) a ^= cols[static_cast<uint64_t>(fold[j]) * colstride + w];
+        if (piv0 >= 0) p = cols[static_cast<uint64_t>(piv0) * colstride + w];
+        if (piv1 >= 0) p ^= cols[static_cast<uint64_t>(piv1) * colstride + w];
+        if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
+    }
+    uint64_t my_el = 0, my_ef = 0;
+    uint64_t nz = __ballot(a != 0);
+    while (nz) {
+        const int j = __ffsll(static_cast<unsigned long long>(nz)) - 1;
+        nz &= nz - 1;
+        const uint64_t aj = __shfl(a, j, 64);
+        const uint64_t pj = __shfl(p, j, 64);
+        const uint64_t r = (wbase + static_cast<uint64_t>(j)) * 64 + lane;
+        bool emit = false;
+        if ((aj >> lane) & 1u) {
+            const double v = coef[r];
+            coef[r] = v * gd.k;
+            const double abs_c = fabs(v);
+            if (!(tr.check_lower && gd.abs_s * abs_c <= tr.lower)) {
+                uint64_t m[NW], t[NW];
+                load_row<NW>(rows, static_cast<uint32_t>(r), m);
+#pragma unroll
+                for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+                emit = mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type) ||
+                       (tr.check_upper && gd.abs_s * abs_c >= tr.upper);
+                if (emit) vsrc[r] = v;
+            }
+        }
+        const uint64_t em = __ballot(emit);
+        if (lane == static_cast<uint32_t>(j)) {
+            my_el = em & ~pj;
+            my_ef = em & pj;
+        }
+    }
+    if (w < nwords) {
+        elead[w] = my_el;
+        efoll[w] = my_ef;
+    }
+    const uint32_t cl = block_sum(static_cast<uint32_t>(__popcll(my_el)), lds);
+    const uint32_t cf = block_sum(static_cast<uint32_t>(__popcll(my_ef)), lds);
+    const uint32_t ca = block_sum(static_cast<uint32_t>(__popcll(a)), lds);
+    if (threadIdx.x == 0) {
+        blk_l[blockIdx.x] = cl;
+        blk_f[blockIdx.x] = cf;
+        if (ca) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NA), static_cast<unsigned long long>(ca));
+        if (cl + cf)
+            atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NEMIT),
+                      static_cast<unsigned long long>(cl + cf));
+    }
+}
+
+// Partner resolution of the emitters (Engine.h:364-387, 465-513): leaders first, marking the partners
+// they rotate; then the followers no leader matched.  A hit applies the rotation against the swept
+// partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss stages the
+// new term's coefficient k*0 + (s ph) v.
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi,
+                                                    const uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                    const double *__restrict__ vsrc, const uint32_t *__restrict__ AL,
+                                                    const uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                    uint32_t *__restrict__ mepoch, uint32_t epoch,
+                                                    uint8_t *__restrict__ miss, double *__restrict__ ins,
+                                                    uint64_t *__restrict__ counters, int leader_pass) {
+    if (counters[C_STOP]) return;
+    const GateDesc &gd = gdesc[gi];
+    const uint64_t nEL = counters[C_NL], nEF = counters[C_NF];
+    const uint64_t lo = leader_pass ? 0 : nEL, hi = leader_pass ? nEL : nEL + nEF;
+    uint32_t hits = 0;
+    for (uint64_t i = lo + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < hi;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint32_t S = AL[i];
+        if (!leader_pass && mepoch[S] == epoch) {
+            miss[i] = 0;
+            continue;
+        }
+        uint64_t m[NW], t[NW];
+        load_row<NW>(rows, S, m);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+        const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
+        const double vS = vsrc[S];
+        const int64_t T = gidx_find<NW>(gidx, gmask, rows, t);
+        if (T >= 0) {
+            if (leader_pass) mepoch[T] = epoch;
+            const double wS = coef[S], wT = coef[T];
+            const double vTu = wT * gd.inv_k;
+            coef[S] = wS + gd.s * static_cast<double>(-ph) * vTu;
+            coef[T] = wT + gd.s * static_cast<double>(ph) * vS;
+            miss[i] = 0;
+            ++hits;
+        } else {
+            coef[S] = coef[S] + gd.s * static_cast<double>(-ph) * 0.0;
+            ins[i] = gd.k * 0.0 + gd.s * static_cast<double>(ph) * vS;
+            miss[i] = static_cast<uint8_t>(1u | (ph < 0 ? 2u : 0u));
+        }
+    }
+    if (hits) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR), static_cast<unsigned long long>(hits));
+}
+
+// Ordered append of the partners to create (leader misses then follower misses, Engine.h:428-444), by
+// one block of 1024 threads walking the emitter list; new rows enter the global index.
+template <int NW>
+__global__ __launch_bounds__(kScanBlock) void k_append(const GateDesc *__restrict__ gdesc, int gi,
+                                                       uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                       const uint32_t *__restrict__ AL,
+                                                       const uint8_t *__restrict__ miss, const double *__restrict__ ins,
+                                                       uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                       uint64_t *__restrict__ counters) {
+    __shared__ uint32_t lds[kScanBlock / 64];
+    if (counters[C_STOP]) return;
+    const GateDesc &gd = gdesc[gi];
+    const uint64_t N = counters[C_N];
+    const uint64_t n_items = counters[C_NL] + counters[C_NF];
+    uint64_t running = 0;
+    for (uint64_t base = 0; base < n_items; base += kScanBlock) {
+        const uint64_t i = base + threadIdx.x;
+        const uint32_t v = (i < n_items) ? (miss[i] & 1u) : 0u;
+        uint32_t tot;
+        const uint32_t rank = block_excl_scan<kScanBlock / 64>(v, lds, &tot);
+        if (v) {
+            const uint64_t dest = N + running + rank;
+            const uint64_t *ps = rows + static_cast<uint64_t>(AL[i]) * NW;
+            uint64_t *pd = rows + dest * NW;
+            uint64_t t[NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+                t[k] = ps[k] ^ gd.g[k];
+                pd[k] = t[k];
+            }
+            coef[dest] = ins[i];
+            if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest)))
+                atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
+        }
+        running += tot;
+    }
+    if (threadIdx.x == 0) counters[C_NINS] = running;
+}
+
+// End of a fused gate: per-gate counters, N += I, and the stop decision for the next gate -- it runs
+// only while its worst case (every live term anticommuting and creating a partner) fits the store and
+// keeps the index at most half full.
+__global__ void k_advance_fused(uint64_t *__restrict__ counters, int64_t *__restrict__ stats, int gi, int identity,
+                                uint64_t cap_rows, uint64_t gidx_slots) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (counters[C_STOP]) return;
+    int64_t *s = stats + static_cast<int64_t>(gi) * kStatW;
+    s[0] = static_cast<int64_t>(counters[C_N]);
+    if (identity) {
+        s[1] = s[2] = s[3] = s[4] = s[5] = s[6] = s[7] = 0;
+    } else {
+        s[1] = static_cast<int64_t>(counters[C_NA]);
+        s[2] = static_cast<int64_t>(counters[C_NEMIT]);
+        s[3] = static_cast<int64_t>(counters[C_NINS]);
+        s[4] = static_cast<int64_t>(counters[C_NL]);
+        s[5] = static_cast<int64_t>(counters[C_NPAIR]);
+        s[6] = s[7] = 0;
+        counters[C_N] += counters[C_NINS];
+    }
+    counters[C_NINS] = 0;
+    counters[C_NEMIT] = 0;
+    counters[C_NPAIR] = 0;
+    counters[C_NA] = 0;
+    const uint64_t n = counters[C_N];
+    if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+}
+
 
 // <O> partials: sum over fully paired rows of phase * c, per block in a fixed order.
 template <int NW, int PAULI>
@@ -802,6 +1057,15 @@ struct mpa_engine {
     uint64_t base_n = 0;
     uint64_t gcos_total = 0, grot_total = 0;  // records stored
     bool building = false;                    // seedc tracks capacity while a graph is built
+
+    // Fused pipeline state: global row index, pre-gate values of emitters, per-row match epochs.
+    DevArray gidx, vsrc, mepoch;
+    uint64_t gidx_slots = 0;
+    uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
+    uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
+    uint32_t match_epoch = 0;
+    static constexpr size_t kChunk = 256;  // gates enqueued between host synchronisations
+    bool fused_enabled = true;             // mpa_set_pipeline(e, 0) forces the per-gate pipeline
     uint64_t table_slots = 0;
     uint64_t epoch = 0;
     int64_t *h_counters = nullptr;  // pinned
@@ -1225,6 +1489,189 @@ struct mpa_engine {
         for (size_t l = nl; l-- > 0;) grad[layers[l].param_index] += layers[l].gen_coeff * dl[l];
     }
 
+    // Per-row arrays of the fused pipeline sized to the store, and a global index with >= 2 slots per
+    // row of capacity covering every stored row.
+    template <int NW>
+    void ensure_fused_arrays() {
+        if (fused_cap != cap) {
+            vsrc.reserve(cap * 8);
+            miss.reserve(cap);
+            ins.reserve(cap * 8);
+            void *old = mepoch.p;
+            mepoch.reserve(cap * 4);
+            if (mepoch.p != old) HIP_TRY(hipMemsetAsync(mepoch.p, 0, mepoch.bytes, stream));
+            fused_cap = cap;
+        }
+        uint64_t want = 1024;
+        while (want < 2 * cap) want <<= 1;
+        if (want > gidx_slots) {
+            gidx.reserve(want * 8);
+            gidx_slots = want;
+            gidx_rows = static_cast<uint64_t>(-1);  // force a rebuild
+        }
+        if (gidx_rows != N) {
+            HIP_TRY(hipMemsetAsync(gidx.p, 0, gidx_slots * 8, stream));
+            if (N) {
+                hipLaunchKernelGGL((k_gidx_build<NW>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)),
+                                   dim3(kBlock), 0, stream, rows.as<uint64_t>(), 0, N, gidx.as<uint64_t>(),
+                                   gidx_slots - 1, counters.as<uint64_t>());
+                HIP_TRY(hipGetLastError());
+            }
+            gidx_rows = N;
+        }
+    }
+
+    void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
+        const double NW = nw;
+        for (size_t g = 0; g < stat_n.size(); ++g) {
+            if (fold_plus_piv[g] == 0) continue;
+            const double words = std::ceil(static_cast<double>(stat_n[g]) / 64.0);
+            const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], EL = stat_l[g], hits = stat_p[g];
+            const double EF = E - EL;
+            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 2) + nA * 16.0 + E * (8 * NW + 8);
+            prof_bytes[K_SCANSUM] += 2.0 * 8.0 * std::ceil(words / kBlock);
+            prof_bytes[K_EXPAND] += words * 16.0 + E * 4.0;
+            const double per = 4 + 16 * NW + 8 + 8 + 16;
+            const double hit_l = E > 0 ? hits * EL / E : 0.0;
+            prof_bytes[K_RESOLVE_L] += EL * per + hit_l * (16 + 4) + (EL - hit_l) * 9;
+            prof_bytes[K_RESOLVE_F] += EF * (per + 4) + (hits - hit_l) * 16 + (EF - (hits - hit_l)) * 9;
+            prof_bytes[K_APPEND] += E + I * (4 + 16 * NW + 8 + 8 + 8);
+            prof_bytes[K_COLS] += I * (8 * NW + ncols / 8.0);
+        }
+    }
+
+    // Contract-immediately propagation without host synchronisation between gates: gates are
+    // enqueued in chunks; the device stops before a gate whose worst case could outgrow the store,
+    // the host grows it and resumes there.
+    template <int NW, int PAULI>
+    void propagate_fused(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds) {
+        const size_t ng = gds.size();
+        gdesc.reserve(std::max<size_t>(ng, 1) * sizeof(GateDesc));
+        fold.reserve(std::max<size_t>(folds.size(), 1) * 4);
+        stats.reserve(std::max<size_t>(ng, 1) * kStatW * 8);
+        if (ng) HIP_TRY(hipMemcpyAsync(gdesc.p, gds.data(), ng * sizeof(GateDesc), hipMemcpyHostToDevice, stream));
+        if (!folds.empty())
+            HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
+        std::vector<int32_t> fold_plus_piv(ng, 0);
+        for (size_t g = 0; g < ng; ++g)
+            if (!gds[g].identity) fold_plus_piv[g] = gds[g].fold_cnt + (gds[g].piv0 >= 0) + (gds[g].piv1 >= 0);
+        hipEvent_t ev[2];
+        HIP_TRY(hipEventCreate(&ev[0]));
+        HIP_TRY(hipEventCreate(&ev[1]));
+        HIP_TRY(hipEventRecord(ev[0], stream));
+        size_t gi = 0;
+        while (gi < ng) {
+            if (2 * N > cap) ensure_capacity(std::max<uint64_t>(4 * N, 64ull * kBlock));
+            ensure_fused_arrays<NW>();
+            const size_t end = std::min(ng, gi + kChunk);
+            const uint64_t words_ub = (cap / 2 + 63) / 64;
+            const uint32_t sblocks = static_cast<uint32_t>(std::max<uint64_t>((words_ub + kBlock - 1) / kBlock, 1));
+            const uint32_t rblocks =
+                static_cast<uint32_t>(std::min<uint64_t>(2048, std::max<uint64_t>(1, cap / 2 / kBlock / 8)));
+            const uint32_t cblocks =
+                static_cast<uint32_t>(std::min<uint64_t>(512, std::max<uint64_t>(1, cap / 2 / 64 / kWaves / 64)));
+            const uint64_t gmask = gidx_slots - 1;
+            for (size_t g = gi; g < end; ++g) {
+                const GateDesc &gd = gds[g];
+                const int ig = static_cast<int>(g);
+                hipEvent_t pa;
+                if (gd.identity) {
+                    pa = prof_begin();
+                    hipLaunchKernelGGL(k_advance_fused, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(),
+                                       stats.as<int64_t>(), ig, 1, cap, gidx_slots);
+                    HIP_TRY(hipGetLastError());
+                    prof_end(K_ADVANCE, pa);
+                    continue;
+                }
+                ++match_epoch;
+                if (match_epoch == 0) {  // wrapped: clear the marks once
+                    HIP_TRY(hipMemsetAsync(mepoch.p, 0, mepoch.bytes, stream));
+                    match_epoch = 1;
+                }
+                pa = prof_begin();
+                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sblocks), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                                   ig, tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off,
+                                   gd.fold_cnt, gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(),
+                                   vsrc.as<double>(), lead.as<uint64_t>(), foll.as<uint64_t>(), blk_a.as<uint32_t>(),
+                                   blk_b.as<uint32_t>(), counters.as<uint64_t>());
+                HIP_TRY(hipGetLastError());
+                prof_end(K_SWEEP, pa);
+                pa = prof_begin();
+                hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, stream, blk_a.as<uint32_t>(),
+                                   blk_b.as<uint32_t>(), sblocks, counters.as<uint64_t>(), C_NL, C_NF, 0, 0);
+                HIP_TRY(hipGetLastError());
+                prof_end(K_SCANSUM, pa);
+                pa = prof_begin();
+                hipLaunchKernelGGL(k_expand, dim3(sblocks), dim3(kBlock), 0, stream, lead.as<uint64_t>(),
+                                   foll.as<uint64_t>(), blk_a.as<uint32_t>(), blk_b.as<uint32_t>(),
+                                   counters.as<uint64_t>(), AL.as<uint32_t>());
+                HIP_TRY(hipGetLastError());
+                prof_end(K_EXPAND, pa);
+                for (int leader_pass = 1; leader_pass >= 0; --leader_pass) {
+                    pa = prof_begin();
+                    hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rblocks), dim3(kBlock), 0, stream,
+                                       gdesc.as<GateDesc>(), ig, rows.as<uint64_t>(), coef.as<double>(),
+                                       vsrc.as<double>(), AL.as<uint32_t>(), gidx.as<uint64_t>(), gmask,
+                                       mepoch.as<uint32_t>(), match_epoch, miss.as<uint8_t>(), ins.as<double>(),
+                                       counters.as<uint64_t>(), leader_pass);
+                    HIP_TRY(hipGetLastError());
+                    prof_end(leader_pass ? K_RESOLVE_L : K_RESOLVE_F, pa);
+                }
+                pa = prof_begin();
+                hipLaunchKernelGGL((k_append<NW>), dim3(1), dim3(kScanBlock), 0, stream, gdesc.as<GateDesc>(), ig,
+                                   rows.as<uint64_t>(), coef.as<double>(), AL.as<uint32_t>(), miss.as<uint8_t>(),
+                                   ins.as<double>(), gidx.as<uint64_t>(), gmask, counters.as<uint64_t>());
+                HIP_TRY(hipGetLastError());
+                prof_end(K_APPEND, pa);
+                pa = prof_begin();
+                hipLaunchKernelGGL((k_cols_update<NW, PAULI>), dim3(cblocks), dim3(kBlock), 0, stream,
+                                   rows.as<uint64_t>(), cols.as<uint64_t>(), colstride(), n_modes,
+                                   counters.as<uint64_t>(), uint64_t{0}, uint64_t{0}, 1);
+                HIP_TRY(hipGetLastError());
+                prof_end(K_COLS, pa);
+                pa = prof_begin();
+                hipLaunchKernelGGL(k_advance_fused, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(),
+                                   stats.as<int64_t>(), ig, 0, cap, gidx_slots);
+                HIP_TRY(hipGetLastError());
+                prof_end(K_ADVANCE, pa);
+            }
+            sync_counters();
+            prof_resolve();
+            if (h_counters[C_ERR]) throw RunError("device index overflow");
+            N = static_cast<uint64_t>(h_counters[C_N]);
+            gidx_rows = N;
+            const uint64_t stop = static_cast<uint64_t>(h_counters[C_STOP]);
+            if (stop) {
+                gi = static_cast<size_t>(stop - 1);
+                const uint64_t zero = 0;
+                HIP_TRY(hipMemcpyAsync(counters.as<uint64_t>() + C_STOP, &zero, 8, hipMemcpyHostToDevice, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                if (N >= (uint64_t{1} << 31)) throw RunError("term count exceeds the 32-bit row index");
+            } else {
+                gi = end;
+            }
+        }
+        HIP_TRY(hipEventRecord(ev[1], stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        last_propagate_ms = ms;
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        std::vector<int64_t> st(ng * kStatW);
+        if (ng) HIP_TRY(hipMemcpy(st.data(), stats.p, ng * kStatW * 8, hipMemcpyDeviceToHost));
+        for (auto *v : {&stat_n, &stat_a, &stat_e, &stat_i, &stat_l, &stat_p}) v->resize(ng);
+        for (size_t g = 0; g < ng; ++g) {
+            stat_n[g] = st[kStatW * g];
+            stat_a[g] = st[kStatW * g + 1];
+            stat_e[g] = st[kStatW * g + 2];
+            stat_i[g] = st[kStatW * g + 3];
+            stat_l[g] = st[kStatW * g + 4];
+            stat_p[g] = st[kStatW * g + 5];
+        }
+        if (prof_on) add_algorithmic_bytes_fused(fold_plus_piv);
+    }
+
     template <int NW, int PAULI>
     double expect_impl(const double *c) {
         partial.reserve(kExpectBlocks * 8);
@@ -1375,9 +1822,15 @@ struct RunPropagate {
     const std::vector<int32_t> &folds;
     bool graph;
     Trunc trl;
+    bool fused;  // every gate takes the fused cos sweep: run the fused pipeline
     template <int NW, int PAULI>
     void operator()() {
-        e->propagate_impl<NW, PAULI>(gds, folds, graph, trl);
+        if (fused) {
+            e->propagate_fused<NW, PAULI>(gds, folds);
+        } else {
+            e->gidx_rows = static_cast<uint64_t>(-1);  // the unfused pipeline does not maintain the index
+            e->propagate_impl<NW, PAULI>(gds, folds, graph, trl);
+        }
     }
 };
 
@@ -1586,7 +2039,9 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
                    only_rotate_len_k < 0, gds, folds);
         Trunc trl = e->tr;
         trl.only_rot = only_rotate_len_k < 0 ? -1 : only_rotate_len_k;
-        RunPropagate run{e, gds, folds, false, trl};
+        bool fused = only_rotate_len_k < 0 && e->fused_enabled;
+        for (const GateDesc &gd : gds) fused = fused && (gd.identity || gd.fused);
+        RunPropagate run{e, gds, folds, false, trl, fused};
         e->dispatch(run);
     });
 }
@@ -1623,7 +2078,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
         }
         e->building = true;
         try {
-            RunPropagate run{e, gds, folds, true, trl};
+            RunPropagate run{e, gds, folds, true, trl, false};
             e->dispatch(run);
         } catch (...) {
             e->building = false;
@@ -1761,6 +2216,11 @@ int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t 
         }
         *len = k;
     });
+}
+
+int mpa_set_pipeline(mpa_engine *e, int fused) {
+    e->fused_enabled = fused != 0;
+    return 0;
 }
 
 int mpa_set_profiling(mpa_engine *e, int on) {

@@ -145,3 +145,33 @@ def test_evolved_operator_export(oracle_mod):
     eo = o.evolved_operator(1e-12)
     ed = d.evolved_operator([], 1e-12)
     assert eo == ed
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_both_pipelines_lattice(oracle_mod, fused):
+    """The fused single-sweep pipeline and the per-gate pipeline give the oracle's store; the 840-gate
+    circuit crosses several enqueue chunks and store regrowths."""
+    print(f"case: lattice 10x10 steps=3 fused={fused}", flush=True)
+    circ = workloads.kicked_ising_lattice(10, 3, theta_h=0.4)
+    n = 100
+    init = PauliOperator({Pauli("Z", (s,)): 0.01 for s in range(n)}, n).get_local_operator().terms
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    o = oracle_mod.Oracle(init, n, basis="pauli", cutoff=10, cutoff_type="support", lower_atol=1e-5)
+    o.propagate(monos, pm, gcs, circ.parameters)
+    d = DeviceSimulator(init, 10, [], lower_atol=1e-5, cutoff_type="support", logical_num_modes=n, basis="pauli")
+    d.set_pipeline(fused)
+    d.propagate(monos, pm, gcs, circ.parameters)
+    _assert_same(o, d)
+
+
+def test_repeated_propagate_calls(oracle_mod):
+    """Two propagate calls in a row (the second continues from the first's store and index)."""
+    circ = workloads.kicked_ising_chain(16, 2)
+    init = workloads.z_observable(16, [8]).get_local_operator().terms
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, 16)
+    o = oracle_mod.Oracle(init, 16, basis="pauli", cutoff=6, cutoff_type="support", lower_atol=1e-8)
+    d = DeviceSimulator(init, 6, [], lower_atol=1e-8, cutoff_type="support", logical_num_modes=16, basis="pauli")
+    for _ in range(2):
+        o.propagate(monos, pm, gcs, circ.parameters)
+        d.propagate(monos, pm, gcs, circ.parameters)
+    _assert_same(o, d)
