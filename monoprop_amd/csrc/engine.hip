@@ -56,6 +56,8 @@ enum : int {
     C_STOP = 7,              // fused pipeline: 1 + index of the gate to resume at (0 = running)
     C_GCOS = 8, C_GROT = 9,  // graph record counts (cumulative over a build)
     C_NA = 10,               // fused pipeline: anticommuting terms of the gate
+    C_DONE_SWEEP = 11,       // fused pipeline: finished-block tickets (last block does the tail work)
+    C_DONE_RESOLVE = 12,
     C_COUNT = 16
 };
 // per-gate stats: N, nA, emitted, inserted, leaders, pairs, graph cos-record end, graph rotation end
@@ -64,11 +66,10 @@ constexpr uint32_t kPhaseBit = 0x80000000u;  // rotation record: target row | (p
 
 // Kernel classes timed by the profiler (mpa_kernel_profile).
 enum KernelKind : int { K_SCAN, K_SCANSUM, K_EXPAND, K_LEADERS, K_FOLLOWERS, K_UNMATCHED, K_COUNT, K_INSERT,
-                        K_COLS, K_ADVANCE, K_SWEEP, K_RESOLVE_L, K_RESOLVE_F, K_APPEND, K_KINDS };
+                        K_COLS, K_ADVANCE, K_SWEEP, K_RESOLVE, K_KINDS };
 const char *const kKindNames[K_KINDS] = {"k_scan", "k_scan_sums", "k_expand", "k_leaders", "k_followers",
                                          "k_leaders_unmatched", "k_count_miss", "k_insert", "k_cols_update",
-                                         "k_advance", "k_sweep", "k_resolve_leaders", "k_resolve_followers",
-                                         "k_append"};
+                                         "k_advance", "k_sweep", "k_resolve"};
 
 thread_local std::string g_error;
 
@@ -690,11 +691,41 @@ __global__ __launch_bounds__(kBlock) void k_gidx_build(const uint64_t *__restric
         atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
 }
 
-// Anticommutation + fused cos sweep + emission gate in one pass (Scan.h:384-401).  One wave per 64
-// column words; for every word holding anticommuting terms the wave processes its 64 terms one per
-// lane: c <- c k (the reference's in-place sweep), and terms past the lower_atol gate are tested
-// against the structural cutoff of their partner.  Emitters keep their pre-gate value in vsrc[] and
-// are marked in the leader / follower emit masks (pivot clear / set).
+// Loads of data other blocks of the same kernel wrote: agent-scope atomics read the coherence point
+// (the L2 caches of the XCDs are not coherent with each other).
+__device__ __forceinline__ uint32_t load_agent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t load_agent(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in exactly one block, the last to finish; every block's earlier global writes are then
+// visible to it through load_agent.  The ticket is reset by the caller's tail work.
+__device__ __forceinline__ bool last_block_done(uint64_t *ticket, uint32_t *lds_flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t =
+            __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(ticket), 1ull, __ATOMIC_ACQ_REL,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        *lds_flag = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    const bool last = *lds_flag != 0;
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last;
+}
+
+// Anticommutation + fused cos sweep + emission gate in one pass (Scan.h:384-401).  One thread per
+// 64-term column word: it walks the anticommuting terms of its word, applies c <- c k (the
+// reference's in-place sweep) and tests the terms past the lower_atol gate against the structural
+// cutoff of their partner.  Emitters keep their pre-gate value in vsrc[] and are marked in the leader
+// / follower emit masks (pivot clear / set).  The last block turns the per-block emitter counts into
+// exclusive offsets and totals.
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
                                                   const uint64_t *__restrict__ cols, uint64_t colstride,
@@ -704,13 +735,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
                                                   uint64_t *__restrict__ efoll, uint32_t *__restrict__ blk_l,
                                                   uint32_t *__restrict__ blk_f, uint64_t *__restrict__ counters) {
     __shared__ uint32_t lds[kWaves];
+    __shared__ uint32_t last_flag;
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
     const uint64_t N = counters[C_N];
     const uint64_t nwords = (N + 63) / 64;
-    const uint32_t lane = lane_id();
-    const uint64_t wbase = static_cast<uint64_t>(blockIdx.x) * kBlock + (threadIdx.x & ~63u);
-    const uint64_t w = wbase + lane;
+    const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     uint64_t a = 0, p = 0;
     if (w < nwords) {
         for (int j = 0; j < nfold; ++j) a ^= cols[static_cast<uint64_t>(fold[j]) * colstride + w];
@@ -718,41 +748,30 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
         if (piv1 >= 0) p ^= cols[static_cast<uint64_t>(piv1) * colstride + w];
         if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
     }
-    uint64_t my_el = 0, my_ef = 0;
-    uint64_t nz = __ballot(a != 0);
-    while (nz) {
-        const int j = __ffsll(static_cast<unsigned long long>(nz)) - 1;
-        nz &= nz - 1;
-        const uint64_t aj = __shfl(a, j, 64);
-        const uint64_t pj = __shfl(p, j, 64);
-        const uint64_t r = (wbase + static_cast<uint64_t>(j)) * 64 + lane;
-        bool emit = false;
-        if ((aj >> lane) & 1u) {
-            const double v = coef[r];
-            coef[r] = v * gd.k;
-            const double abs_c = fabs(v);
-            if (!(tr.check_lower && gd.abs_s * abs_c <= tr.lower)) {
-                uint64_t m[NW], t[NW];
-                load_row<NW>(rows, static_cast<uint32_t>(r), m);
+    uint64_t em = 0;
+    for (uint64_t bits = a; bits; bits &= bits - 1) {
+        const uint64_t r = w * 64 + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(bits)) - 1);
+        const double v = coef[r];
+        coef[r] = v * gd.k;
+        const double abs_c = fabs(v);
+        if (tr.check_lower && gd.abs_s * abs_c <= tr.lower) continue;
+        uint64_t m[NW], t[NW];
+        load_row<NW>(rows, static_cast<uint32_t>(r), m);
 #pragma unroll
-                for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-                emit = mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type) ||
-                       (tr.check_upper && gd.abs_s * abs_c >= tr.upper);
-                if (emit) vsrc[r] = v;
-            }
-        }
-        const uint64_t em = __ballot(emit);
-        if (lane == static_cast<uint32_t>(j)) {
-            my_el = em & ~pj;
-            my_ef = em & pj;
+        for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+        if (mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type) ||
+            (tr.check_upper && gd.abs_s * abs_c >= tr.upper)) {
+            vsrc[r] = v;
+            em |= bits & (~bits + 1);
         }
     }
+    const uint64_t el = em & ~p, ef = em & p;
     if (w < nwords) {
-        elead[w] = my_el;
-        efoll[w] = my_ef;
+        elead[w] = el;
+        efoll[w] = ef;
     }
-    const uint32_t cl = block_sum(static_cast<uint32_t>(__popcll(my_el)), lds);
-    const uint32_t cf = block_sum(static_cast<uint32_t>(__popcll(my_ef)), lds);
+    const uint32_t cl = block_sum(static_cast<uint32_t>(__popcll(el)), lds);
+    const uint32_t cf = block_sum(static_cast<uint32_t>(__popcll(ef)), lds);
     const uint32_t ca = block_sum(static_cast<uint32_t>(__popcll(a)), lds);
     if (threadIdx.x == 0) {
         blk_l[blockIdx.x] = cl;
@@ -762,79 +781,128 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
             atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NEMIT),
                       static_cast<unsigned long long>(cl + cf));
     }
+    if (!last_block_done(counters + C_DONE_SWEEP, &last_flag)) return;
+    // tail: exclusive scan of the block counts (leaders and followers), totals to the counters
+    uint32_t carry_l = 0, carry_f = 0;
+    for (uint32_t base = 0; base < gridDim.x; base += kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t vl = i < gridDim.x ? load_agent(blk_l + i) : 0u;
+        const uint32_t vf = i < gridDim.x ? load_agent(blk_f + i) : 0u;
+        uint32_t tl, tf;
+        const uint32_t xl = block_excl_scan(vl, lds, &tl);
+        const uint32_t xf = block_excl_scan(vf, lds, &tf);
+        if (i < gridDim.x) {
+            blk_l[i] = carry_l + xl;
+            blk_f[i] = carry_f + xf;
+        }
+        carry_l += tl;
+        carry_f += tf;
+    }
+    if (threadIdx.x == 0) {
+        counters[C_NL] = carry_l;
+        counters[C_NF] = carry_f;
+        counters[C_DONE_SWEEP] = 0;
+    }
 }
 
-// Partner resolution of the emitters (Engine.h:364-387, 465-513): leaders first, marking the partners
-// they rotate; then the followers no leader matched.  A hit applies the rotation against the swept
-// partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss stages the
-// new term's coefficient k*0 + (s ph) v.
+// One emitter's partner resolution (Engine.h:364-387, 465-513).  A follower whose partner leader
+// emits is rotated by that leader (the reference's matched-follower skip).  A hit applies the rotation
+// against the swept partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a
+// miss stages the new term's coefficient k*0 + (s ph) v.  Returns true on a hit.
+template <int NW, int PAULI>
+__device__ __forceinline__ bool resolve_one(const GateDesc &gd, const uint64_t *__restrict__ rows,
+                                            double *__restrict__ coef, const double *__restrict__ vsrc,
+                                            const uint64_t *__restrict__ elead, const uint64_t *__restrict__ gidx,
+                                            uint64_t gmask, uint32_t S, bool leader, uint32_t *miss_out,
+                                            double *ins_out) {
+    uint64_t m[NW], t[NW];
+    load_row<NW>(rows, S, m);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+    const int64_t T = gidx_find<NW>(gidx, gmask, rows, t);
+    if (T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u)) {
+        *miss_out = 0;  // the emitting leader T rotates this pair
+        return false;
+    }
+    const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
+    const double vS = vsrc[S];
+    if (T >= 0) {
+        const double wS = coef[S], wT = coef[T];
+        const double vTu = wT * gd.inv_k;
+        coef[S] = wS + gd.s * static_cast<double>(-ph) * vTu;
+        coef[T] = wT + gd.s * static_cast<double>(ph) * vS;
+        *miss_out = 0;
+        return true;
+    }
+    coef[S] = coef[S] + gd.s * static_cast<double>(-ph) * 0.0;
+    *ins_out = gd.k * 0.0 + gd.s * static_cast<double>(ph) * vS;
+    *miss_out = 1u;
+    return false;
+}
+
+// Resolution of every emitter, then (last block) the gate's tail: ordered append of the partners to
+// create (leader misses then follower misses, ascending, Engine.h:428-444), their index entries and
+// column bits, the per-gate counters, N += I and the stop decision for the next gate.  Block b covers
+// the same 256 column words as block b of k_sweep and writes its emitters at the offsets k_sweep
+// computed: leaders at AL[0, nEL), followers at AL[nEL, nEL + nEF).
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi,
-                                                    const uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                    const double *__restrict__ vsrc, const uint32_t *__restrict__ AL,
-                                                    const uint64_t *__restrict__ gidx, uint64_t gmask,
-                                                    uint32_t *__restrict__ mepoch, uint32_t epoch,
-                                                    uint8_t *__restrict__ miss, double *__restrict__ ins,
-                                                    uint64_t *__restrict__ counters, int leader_pass) {
-    if (counters[C_STOP]) return;
-    const GateDesc &gd = gdesc[gi];
-    const uint64_t nEL = counters[C_NL], nEF = counters[C_NF];
-    const uint64_t lo = leader_pass ? 0 : nEL, hi = leader_pass ? nEL : nEL + nEF;
-    uint32_t hits = 0;
-    for (uint64_t i = lo + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < hi;
-         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
-        const uint32_t S = AL[i];
-        if (!leader_pass && mepoch[S] == epoch) {
-            miss[i] = 0;
-            continue;
-        }
-        uint64_t m[NW], t[NW];
-        load_row<NW>(rows, S, m);
-#pragma unroll
-        for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-        const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
-        const double vS = vsrc[S];
-        const int64_t T = gidx_find<NW>(gidx, gmask, rows, t);
-        if (T >= 0) {
-            if (leader_pass) mepoch[T] = epoch;
-            const double wS = coef[S], wT = coef[T];
-            const double vTu = wT * gd.inv_k;
-            coef[S] = wS + gd.s * static_cast<double>(-ph) * vTu;
-            coef[T] = wT + gd.s * static_cast<double>(ph) * vS;
-            miss[i] = 0;
-            ++hits;
-        } else {
-            coef[S] = coef[S] + gd.s * static_cast<double>(-ph) * 0.0;
-            ins[i] = gd.k * 0.0 + gd.s * static_cast<double>(ph) * vS;
-            miss[i] = static_cast<uint8_t>(1u | (ph < 0 ? 2u : 0u));
-        }
-    }
-    if (hits) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR), static_cast<unsigned long long>(hits));
-}
-
-// Ordered append of the partners to create (leader misses then follower misses, Engine.h:428-444), by
-// one block of 1024 threads walking the emitter list; new rows enter the global index.
-template <int NW>
-__global__ __launch_bounds__(kScanBlock) void k_append(const GateDesc *__restrict__ gdesc, int gi,
-                                                       uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                       const uint32_t *__restrict__ AL,
-                                                       const uint8_t *__restrict__ miss, const double *__restrict__ ins,
-                                                       uint64_t *__restrict__ gidx, uint64_t gmask,
-                                                       uint64_t *__restrict__ counters) {
-    __shared__ uint32_t lds[kScanBlock / 64];
+                                                    uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                    const double *__restrict__ vsrc,
+                                                    const uint64_t *__restrict__ elead,
+                                                    const uint64_t *__restrict__ efoll,
+                                                    const uint32_t *__restrict__ blk_l,
+                                                    const uint32_t *__restrict__ blk_f, uint32_t *__restrict__ AL,
+                                                    uint32_t *__restrict__ miss, double *__restrict__ ins,
+                                                    uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                    uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
+                                                    uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
+                                                    uint64_t cap_rows, uint64_t gidx_slots) {
+    __shared__ uint32_t lds[kWaves];
+    __shared__ uint32_t last_flag;
+    __shared__ uint32_t sh_tot[2];
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
     const uint64_t N = counters[C_N];
-    const uint64_t n_items = counters[C_NL] + counters[C_NF];
+    const uint64_t nwords = (N + 63) / 64;
+    const uint64_t nEL = counters[C_NL], nEF = counters[C_NF];
+    const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t el = w < nwords ? elead[w] : 0, ef = w < nwords ? efoll[w] : 0;
+    uint32_t tl, tf;
+    const uint32_t ol = block_excl_scan(static_cast<uint32_t>(__popcll(el)), lds, &tl);
+    const uint32_t of = block_excl_scan(static_cast<uint32_t>(__popcll(ef)), lds, &tf);
+    const uint64_t base_l = blk_l[blockIdx.x], base_f = nEL + blk_f[blockIdx.x];
+    uint64_t pos = base_l + ol;
+    for (uint64_t bits = el; bits; bits &= bits - 1)
+        AL[pos++] = static_cast<uint32_t>(w * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
+    pos = base_f + of;
+    for (uint64_t bits = ef; bits; bits &= bits - 1)
+        AL[pos++] = static_cast<uint32_t>(w * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
+    __syncthreads();
+    uint32_t hits = 0;
+    for (uint32_t i = threadIdx.x; i < tl + tf; i += kBlock) {
+        const bool leader = i < tl;
+        const uint64_t item = leader ? base_l + i : base_f + (i - tl);
+        if (resolve_one<NW, PAULI>(gd, rows, coef, vsrc, elead, gidx, gmask, AL[item], leader, miss + item,
+                                   ins + item))
+            ++hits;
+    }
+    const uint32_t bh = block_sum(hits, lds);
+    if (threadIdx.x == 0 && bh)
+        atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR), static_cast<unsigned long long>(bh));
+    if (!last_block_done(counters + C_DONE_RESOLVE, &last_flag)) return;
+
+    // ---- tail (one block) ----
+    const uint64_t n_items = nEL + nEF;
     uint64_t running = 0;
-    for (uint64_t base = 0; base < n_items; base += kScanBlock) {
+    for (uint64_t base = 0; base < n_items; base += kBlock) {
         const uint64_t i = base + threadIdx.x;
-        const uint32_t v = (i < n_items) ? (miss[i] & 1u) : 0u;
+        const uint32_t v = (i < n_items) ? load_agent(miss + i) : 0u;
         uint32_t tot;
-        const uint32_t rank = block_excl_scan<kScanBlock / 64>(v, lds, &tot);
+        const uint32_t rank = block_excl_scan(v, lds, &tot);
         if (v) {
             const uint64_t dest = N + running + rank;
-            const uint64_t *ps = rows + static_cast<uint64_t>(AL[i]) * NW;
+            const uint64_t *ps = rows + static_cast<uint64_t>(load_agent(AL + i)) * NW;
             uint64_t *pd = rows + dest * NW;
             uint64_t t[NW];
 #pragma unroll
@@ -842,13 +910,76 @@ __global__ __launch_bounds__(kScanBlock) void k_append(const GateDesc *__restric
                 t[k] = ps[k] ^ gd.g[k];
                 pd[k] = t[k];
             }
-            coef[dest] = ins[i];
+            coef[dest] = load_agent(ins + i);
             if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest)))
                 atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
         }
         running += tot;
     }
-    if (threadIdx.x == 0) counters[C_NINS] = running;
+    __threadfence_block();
+    __syncthreads();
+    // column bits of the new rows [N, N + running): one wave per 64-term word
+    const uint64_t start = N, count = running;
+    if (count) {
+        const uint64_t w0 = start / 64, wend = (start + count + 63) / 64;
+        const bool lane0 = lane_id() == 0;
+        for (uint64_t cw = w0 + (threadIdx.x >> 6); cw < wend; cw += kWaves) {
+            const uint64_t r = cw * 64 + lane_id();
+            uint64_t m[NW];
+            const bool live = r >= start && r < start + count;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) m[k] = live ? rows[r * NW + k] : 0;
+            if (PAULI) {
+                constexpr int L = NW / 2;
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    for (int bit = 0; bit < 64; ++bit) {
+                        const int q = l * 64 + bit;
+                        if (q >= n_modes) break;
+                        const uint64_t bx = __ballot((m[l] >> bit) & 1u);
+                        const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
+                        if (lane0) {
+                            if (bx) cols[static_cast<uint64_t>(q) * colstride + cw] |= bx;
+                            if (bz) cols[static_cast<uint64_t>(n_modes + q) * colstride + cw] |= bz;
+                        }
+                    }
+                }
+            } else {
+                int pop = 0;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+                    pop += __popcll(m[k]);
+                    for (int bit = 0; bit < 64; ++bit) {
+                        const int sl = k * 64 + bit;
+                        if (sl >= 2 * n_modes) break;
+                        const uint64_t bb = __ballot((m[k] >> bit) & 1u);
+                        if (lane0 && bb) cols[static_cast<uint64_t>(sl) * colstride + cw] |= bb;
+                    }
+                }
+                const uint64_t bp = __ballot(pop & 1);
+                if (lane0 && bp) cols[static_cast<uint64_t>(2 * n_modes) * colstride + cw] |= bp;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
+        st[0] = static_cast<int64_t>(N);
+        st[1] = static_cast<int64_t>(load_agent(counters + C_NA));
+        st[2] = static_cast<int64_t>(load_agent(counters + C_NEMIT));
+        st[3] = static_cast<int64_t>(running);
+        st[4] = static_cast<int64_t>(nEL);
+        st[5] = static_cast<int64_t>(load_agent(counters + C_NPAIR));
+        st[6] = st[7] = 0;
+        const uint64_t n = N + running;
+        counters[C_N] = n;
+        counters[C_NINS] = 0;
+        counters[C_NEMIT] = 0;
+        counters[C_NPAIR] = 0;
+        counters[C_NA] = 0;
+        counters[C_DONE_RESOLVE] = 0;
+        if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+    }
+    (void)sh_tot;
 }
 
 // End of a fused gate: per-gate counters, N += I, and the stop decision for the next gate -- it runs
@@ -1058,12 +1189,11 @@ struct mpa_engine {
     uint64_t gcos_total = 0, grot_total = 0;  // records stored
     bool building = false;                    // seedc tracks capacity while a graph is built
 
-    // Fused pipeline state: global row index, pre-gate values of emitters, per-row match epochs.
-    DevArray gidx, vsrc, mepoch;
+    // Fused pipeline state: global row index and the pre-gate values of emitters.
+    DevArray gidx, vsrc;
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
-    uint32_t match_epoch = 0;
     static constexpr size_t kChunk = 256;  // gates enqueued between host synchronisations
     bool fused_enabled = true;             // mpa_set_pipeline(e, 0) forces the per-gate pipeline
     uint64_t table_slots = 0;
@@ -1495,11 +1625,8 @@ struct mpa_engine {
     void ensure_fused_arrays() {
         if (fused_cap != cap) {
             vsrc.reserve(cap * 8);
-            miss.reserve(cap);
+            miss.reserve(cap * 4);
             ins.reserve(cap * 8);
-            void *old = mepoch.p;
-            mepoch.reserve(cap * 4);
-            if (mepoch.p != old) HIP_TRY(hipMemsetAsync(mepoch.p, 0, mepoch.bytes, stream));
             fused_cap = cap;
         }
         uint64_t want = 1024;
@@ -1531,12 +1658,12 @@ struct mpa_engine {
             prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 2) + nA * 16.0 + E * (8 * NW + 8);
             prof_bytes[K_SCANSUM] += 2.0 * 8.0 * std::ceil(words / kBlock);
             prof_bytes[K_EXPAND] += words * 16.0 + E * 4.0;
-            const double per = 4 + 16 * NW + 8 + 8 + 16;
-            const double hit_l = E > 0 ? hits * EL / E : 0.0;
-            prof_bytes[K_RESOLVE_L] += EL * per + hit_l * (16 + 4) + (EL - hit_l) * 9;
-            prof_bytes[K_RESOLVE_F] += EF * (per + 4) + (hits - hit_l) * 16 + (EF - (hits - hit_l)) * 9;
-            prof_bytes[K_APPEND] += E + I * (4 + 16 * NW + 8 + 8 + 8);
-            prof_bytes[K_COLS] += I * (8 * NW + ncols / 8.0);
+            // resolve: emit masks, emitter list, row + vsrc + index slot + candidate row per emitter, the
+            // pair's coefficients, miss flag / staged value; tail: appended rows, index slots, column bits
+            (void)EL;
+            (void)EF;
+            prof_bytes[K_RESOLVE] += words * 16.0 + E * (4 + 8 * NW + 8 + 8 + 8 * NW + 16 + 4) + hits * 16 +
+                                     (E - hits) * 8 + E * 8 + I * (4 + 16 * NW + 8 + 8 + 8 + 8 * NW + ncols / 8.0);
         }
     }
 
@@ -1566,10 +1693,6 @@ struct mpa_engine {
             const size_t end = std::min(ng, gi + kChunk);
             const uint64_t words_ub = (cap / 2 + 63) / 64;
             const uint32_t sblocks = static_cast<uint32_t>(std::max<uint64_t>((words_ub + kBlock - 1) / kBlock, 1));
-            const uint32_t rblocks =
-                static_cast<uint32_t>(std::min<uint64_t>(2048, std::max<uint64_t>(1, cap / 2 / kBlock / 8)));
-            const uint32_t cblocks =
-                static_cast<uint32_t>(std::min<uint64_t>(512, std::max<uint64_t>(1, cap / 2 / 64 / kWaves / 64)));
             const uint64_t gmask = gidx_slots - 1;
             for (size_t g = gi; g < end; ++g) {
                 const GateDesc &gd = gds[g];
@@ -1583,11 +1706,6 @@ struct mpa_engine {
                     prof_end(K_ADVANCE, pa);
                     continue;
                 }
-                ++match_epoch;
-                if (match_epoch == 0) {  // wrapped: clear the marks once
-                    HIP_TRY(hipMemsetAsync(mepoch.p, 0, mepoch.bytes, stream));
-                    match_epoch = 1;
-                }
                 pa = prof_begin();
                 hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sblocks), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
                                    ig, tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off,
@@ -1597,43 +1715,14 @@ struct mpa_engine {
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, stream, blk_a.as<uint32_t>(),
-                                   blk_b.as<uint32_t>(), sblocks, counters.as<uint64_t>(), C_NL, C_NF, 0, 0);
+                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(sblocks), dim3(kBlock), 0, stream,
+                                   gdesc.as<GateDesc>(), ig, rows.as<uint64_t>(), coef.as<double>(), vsrc.as<double>(),
+                                   lead.as<uint64_t>(), foll.as<uint64_t>(), blk_a.as<uint32_t>(),
+                                   blk_b.as<uint32_t>(), AL.as<uint32_t>(), miss.as<uint32_t>(), ins.as<double>(),
+                                   gidx.as<uint64_t>(), gmask, cols.as<uint64_t>(), colstride(), n_modes,
+                                   counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots);
                 HIP_TRY(hipGetLastError());
-                prof_end(K_SCANSUM, pa);
-                pa = prof_begin();
-                hipLaunchKernelGGL(k_expand, dim3(sblocks), dim3(kBlock), 0, stream, lead.as<uint64_t>(),
-                                   foll.as<uint64_t>(), blk_a.as<uint32_t>(), blk_b.as<uint32_t>(),
-                                   counters.as<uint64_t>(), AL.as<uint32_t>());
-                HIP_TRY(hipGetLastError());
-                prof_end(K_EXPAND, pa);
-                for (int leader_pass = 1; leader_pass >= 0; --leader_pass) {
-                    pa = prof_begin();
-                    hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rblocks), dim3(kBlock), 0, stream,
-                                       gdesc.as<GateDesc>(), ig, rows.as<uint64_t>(), coef.as<double>(),
-                                       vsrc.as<double>(), AL.as<uint32_t>(), gidx.as<uint64_t>(), gmask,
-                                       mepoch.as<uint32_t>(), match_epoch, miss.as<uint8_t>(), ins.as<double>(),
-                                       counters.as<uint64_t>(), leader_pass);
-                    HIP_TRY(hipGetLastError());
-                    prof_end(leader_pass ? K_RESOLVE_L : K_RESOLVE_F, pa);
-                }
-                pa = prof_begin();
-                hipLaunchKernelGGL((k_append<NW>), dim3(1), dim3(kScanBlock), 0, stream, gdesc.as<GateDesc>(), ig,
-                                   rows.as<uint64_t>(), coef.as<double>(), AL.as<uint32_t>(), miss.as<uint8_t>(),
-                                   ins.as<double>(), gidx.as<uint64_t>(), gmask, counters.as<uint64_t>());
-                HIP_TRY(hipGetLastError());
-                prof_end(K_APPEND, pa);
-                pa = prof_begin();
-                hipLaunchKernelGGL((k_cols_update<NW, PAULI>), dim3(cblocks), dim3(kBlock), 0, stream,
-                                   rows.as<uint64_t>(), cols.as<uint64_t>(), colstride(), n_modes,
-                                   counters.as<uint64_t>(), uint64_t{0}, uint64_t{0}, 1);
-                HIP_TRY(hipGetLastError());
-                prof_end(K_COLS, pa);
-                pa = prof_begin();
-                hipLaunchKernelGGL(k_advance_fused, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(),
-                                   stats.as<int64_t>(), ig, 0, cap, gidx_slots);
-                HIP_TRY(hipGetLastError());
-                prof_end(K_ADVANCE, pa);
+                prof_end(K_RESOLVE, pa);
             }
             sync_counters();
             prof_resolve();
