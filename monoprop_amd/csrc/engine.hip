@@ -55,10 +55,12 @@ enum : int {
     C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6,
     C_STOP = 7,              // fused pipeline: 1 + index of the gate to resume at (0 = running)
     C_GCOS = 8, C_GROT = 9,  // graph record counts (cumulative over a build)
-    C_NA = 10,               // fused pipeline: anticommuting terms of the gate
-    C_DONE_SWEEP = 11,       // fused pipeline: finished-block tickets (last block does the tail work)
-    C_DONE_RESOLVE = 12,
-    C_COUNT = 16
+    // fused pipeline: work tickets of the sweep and resolve kernels (each kernel resets the other's),
+    // rebuilt column words of the sweep, and per gate parity {N, N_cols}: the live count and the
+    // number of leading rows whose column bits are current
+    C_TK_SWEEP = 10, C_TK_RES = 11, C_RB_DONE = 12,
+    C_NB = 16,  // C_NB + 2 * parity: N;  C_NB + 2 * parity + 1: N_cols
+    C_COUNT = 24
 };
 // per-gate stats: N, nA, emitted, inserted, leaders, pairs, graph cos-record end, graph rotation end
 constexpr int kStatW = 8;
@@ -693,321 +695,451 @@ __global__ __launch_bounds__(kBlock) void k_gidx_build(const uint64_t *__restric
 
 // Loads of data other blocks of the same kernel wrote: agent-scope atomics read the coherence point
 // (the L2 caches of the XCDs are not coherent with each other).
-__device__ __forceinline__ uint32_t load_agent(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ uint64_t load_agent(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ double load_agent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
 }
 
-// Returns true in exactly one block, the last to finish; every block's earlier global writes are then
-// visible to it through load_agent.  The ticket is reset by the caller's tail work.
-__device__ __forceinline__ bool last_block_done(uint64_t *ticket, uint32_t *lds_flag) {
-    __threadfence();
+// Block-wide work ticket (every thread gets the same value).
+__device__ __forceinline__ uint32_t take_ticket(uint64_t *ctr, uint32_t *slot) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long t =
-            __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(ticket), 1ull, __ATOMIC_ACQ_REL,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        *lds_flag = (t == gridDim.x - 1) ? 1u : 0u;
+    if (threadIdx.x == 0)
+        *slot = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(ctr), 1ull));
+    __syncthreads();
+    return *slot;
+}
+
+// Ordered prefix sums across the tiles of one launch (single-pass scan with decoupled look-back).
+// Tiles are numbered by work ticket, so every tile a tile waits on is already owned by a running
+// block.  Descriptor word: epoch (30 bits) | status (2 bits) | value (32 bits); a descriptor of another
+// epoch (another gate) reads as not yet published, so the arrays are never cleared between gates.
+constexpr uint64_t kDescAgg = 1, kDescIncl = 2;
+constexpr uint32_t kSpinLimit = 1u << 22;  // bounded waits: a broken invariant flags an error, never hangs
+constexpr uint64_t kErrIndex = 2, kErrLookBack = 4, kErrRebuild = 8;
+
+__device__ __forceinline__ uint64_t desc_pack(uint32_t ep, uint64_t status, uint32_t value) {
+    return (static_cast<uint64_t>(ep) << 34) | (status << 32) | value;
+}
+
+__device__ __forceinline__ void desc_publish(uint64_t *d, uint32_t ep, uint64_t status, uint32_t value) {
+    __hip_atomic_store(d, desc_pack(ep, status, value), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of tile t over tiles [0, t); called by all 64 lanes of one wave.
+__device__ uint32_t look_back(const uint64_t *desc, uint32_t t, uint32_t ep, uint64_t *counters) {
+    const uint32_t lane = lane_id();
+    uint32_t sum = 0, spins = 0;
+    int64_t pos = static_cast<int64_t>(t) - 1;
+    while (pos >= 0) {
+        const int64_t q = pos - static_cast<int64_t>(lane);
+        uint64_t d = desc_pack(ep, kDescIncl, 0);  // before tile 0: an inclusive prefix of 0
+        if (q >= 0) d = load_agent(desc + q);
+        const uint32_t st = (d >> 34) == ep ? static_cast<uint32_t>((d >> 32) & 3u) : 0u;
+        const uint32_t val = static_cast<uint32_t>(d);
+        const uint64_t incl = __ballot(st == kDescIncl);
+        const uint64_t stop = incl | __ballot(st == 0);
+        if (stop == 0) {  // 64 aggregates
+            sum += wave_sum(val);
+            pos -= 64;
+            continue;
+        }
+        const uint32_t first = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(stop)) - 1);
+        if ((incl >> first) & 1u) {
+            sum += wave_sum(lane <= first ? val : 0u);
+            break;
+        }
+        sum += wave_sum(lane < first ? val : 0u);  // aggregates before the first unpublished tile
+        pos -= first;
+        if (++spins > kSpinLimit) {
+            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrLookBack);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
-    __syncthreads();
-    const bool last = *lds_flag != 0;
-    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return last;
+    return sum;
 }
 
-// Anticommutation + fused cos sweep + emission gate in one pass (Scan.h:384-401).  One thread per
-// 64-term column word: it walks the anticommuting terms of its word, applies c <- c k (the
-// reference's in-place sweep) and tests the terms past the lower_atol gate against the structural
-// cutoff of their partner.  Emitters keep their pre-gate value in vsrc[] and are marked in the leader
-// / follower emit masks (pivot clear / set).  The last block turns the per-block emitter counts into
-// exclusive offsets and totals.
+// Column bits of one 64-term word rebuilt from the rows (one wave; lane = term).  The whole word is
+// written, so terms appended since the last rebuild get their bits without read-modify-write.
 template <int NW, int PAULI>
-__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
-                                                  const uint64_t *__restrict__ cols, uint64_t colstride,
+__device__ __forceinline__ void rebuild_word(const uint64_t *__restrict__ rows, uint64_t *__restrict__ cols,
+                                             uint64_t colstride, int n_modes, uint64_t N, uint64_t w) {
+    const uint32_t lane = lane_id();
+    const uint64_t r = w * 64 + lane;
+    uint64_t m[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) m[k] = r < N ? rows[r * NW + k] : 0;
+    if (PAULI) {
+        constexpr int L = NW / 2;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            uint64_t mx = 0, mz = 0;
+            for (int bit = 0; bit < 64 && l * 64 + bit < n_modes; ++bit) {
+                const uint64_t bx = __ballot((m[l] >> bit) & 1u);
+                const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
+                if (lane == static_cast<uint32_t>(bit)) {
+                    mx = bx;
+                    mz = bz;
+                }
+            }
+            const int q = l * 64 + static_cast<int>(lane);
+            if (q < n_modes) {
+                cols[static_cast<uint64_t>(q) * colstride + w] = mx;
+                cols[static_cast<uint64_t>(n_modes + q) * colstride + w] = mz;
+            }
+        }
+    } else {
+        int pop = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            pop += __popcll(m[k]);
+            uint64_t mk = 0;
+            for (int bit = 0; bit < 64 && k * 64 + bit < 2 * n_modes; ++bit) {
+                const uint64_t bb = __ballot((m[k] >> bit) & 1u);
+                if (lane == static_cast<uint32_t>(bit)) mk = bb;
+            }
+            const int sl = k * 64 + static_cast<int>(lane);
+            if (sl < 2 * n_modes) cols[static_cast<uint64_t>(sl) * colstride + w] = mk;
+        }
+        const uint64_t bp = __ballot(pop & 1);
+        if (lane == 0) cols[static_cast<uint64_t>(2 * n_modes) * colstride + w] = bp;
+    }
+}
+
+constexpr int kTileWords = 64;                          // sweep tile: 64 column words = 4096 terms
+constexpr int kRowsPerThread = kTileWords * 64 / kBlock;  // 16
+constexpr int kResChunk = kBlock;                       // resolve: one emitter per thread
+constexpr uint32_t kMaxGrid = 1024;                     // persistent grids (tickets hand out the work)
+
+// Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate, and the ordered
+// emitter lists.  Work tickets [0, R) rebuild the column words of the terms appended by the previous
+// gate (one word per wave); tickets [R, R + tiles) are 4096-term tiles.  Per tile:
+//   A  XOR of G's fold columns -> anticommuting mask, pivot mask (one lane per column word)
+//   B  c <- c k on every anticommuting term (the reference's in-place sweep; 16 coalesced loads in
+//      flight per thread), then the emission test of the terms past the lower_atol gate against the
+//      structural cutoff of their partner
+//   C  tile counts of leader / follower emitters (pivot clear / set), their global offsets by look-back
+//   D  emitters with their pre-gate values: leaders ascending at ITEM[0, nEL), followers ascending at
+//      ITEM[cap - 1 - k]; leader emit masks per word for the followers' skip test
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep, Trunc tr,
+                                                  uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
                                                   const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                  double *__restrict__ vsrc, uint64_t *__restrict__ elead,
-                                                  uint64_t *__restrict__ efoll, uint32_t *__restrict__ blk_l,
-                                                  uint32_t *__restrict__ blk_f, uint64_t *__restrict__ counters) {
-    __shared__ uint32_t lds[kWaves];
-    __shared__ uint32_t last_flag;
+                                                  uint32_t *__restrict__ item, double *__restrict__ ival,
+                                                  uint64_t cap_items, uint64_t *__restrict__ elead,
+                                                  uint64_t *__restrict__ desc_l, uint64_t *__restrict__ desc_f,
+                                                  uint64_t *__restrict__ counters, int64_t *__restrict__ stats) {
+    __shared__ uint64_t sa[kTileWords], sp[kTileWords], sem[kTileWords];
+    __shared__ uint32_t pre_l[kTileWords], pre_f[kTileWords];
+    __shared__ uint32_t s_ticket, s_base[2];
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
-    const uint64_t N = counters[C_N];
+    const int par = gi & 1;
+    const uint64_t N = counters[C_NB + 2 * par], Ncols = counters[C_NB + 2 * par + 1];
     const uint64_t nwords = (N + 63) / 64;
-    const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    uint64_t a = 0, p = 0;
-    if (w < nwords) {
-        for (int j = 0; j < nfold; ++j) a ^= cols[static_cast<uint64_t>(fold[j]) * colstride + w];
-        if (piv0 >= 0) p = cols[static_cast<uint64_t>(piv0) * colstride + w];
-        if (piv1 >= 0) p ^= cols[static_cast<uint64_t>(piv1) * colstride + w];
-        if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
+    const uint64_t ntiles = (nwords + kTileWords - 1) / kTileWords;
+    const uint64_t rb_lo = Ncols < N ? Ncols / 64 : nwords;
+    const uint64_t nrb = nwords - rb_lo;
+    const uint64_t R = (nrb + kWaves - 1) / kWaves;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[C_TK_RES] = 0;
+        if (ntiles == 0) counters[C_NL] = counters[C_NF] = 0;
     }
-    uint64_t em = 0;
-    for (uint64_t bits = a; bits; bits &= bits - 1) {
-        const uint64_t r = w * 64 + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(bits)) - 1);
-        const double v = coef[r];
-        coef[r] = v * gd.k;
-        const double abs_c = fabs(v);
-        if (tr.check_lower && gd.abs_s * abs_c <= tr.lower) continue;
-        uint64_t m[NW], t[NW];
-        load_row<NW>(rows, static_cast<uint32_t>(r), m);
+    for (;;) {
+        const uint64_t tk = take_ticket(counters + C_TK_SWEEP, &s_ticket);
+        if (tk >= R + ntiles) break;
+        if (tk < R) {
+            const uint64_t w = rb_lo + tk * kWaves + wave;
+            if (w < nwords) rebuild_word<NW, PAULI>(rows, cols, colstride, n_modes, N, w);
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_RB_DONE), 1ull);
+            continue;
+        }
+        const uint32_t tile = static_cast<uint32_t>(tk - R);
+        const uint64_t w0 = static_cast<uint64_t>(tile) * kTileWords;
+        const bool after_rb = nrb > 0 && w0 + kTileWords > rb_lo;
+        // ---- A
+        if (wave == 0) {
+            if (after_rb) {
+                uint32_t spins = 0;
+                while (load_agent(counters + C_RB_DONE) < R) {
+                    if (++spins > kSpinLimit) {
+                        if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrRebuild);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            const uint64_t w = w0 + lane;
+            uint64_t a = 0, p = 0;
+            if (w < nwords) {
+                if (after_rb) {
+                    for (int j = 0; j < nfold; ++j) a ^= load_agent(cols + static_cast<uint64_t>(fold[j]) * colstride + w);
+                    if (piv0 >= 0) p = load_agent(cols + static_cast<uint64_t>(piv0) * colstride + w);
+                    if (piv1 >= 0) p ^= load_agent(cols + static_cast<uint64_t>(piv1) * colstride + w);
+                } else {
+                    for (int j = 0; j < nfold; ++j) a ^= cols[static_cast<uint64_t>(fold[j]) * colstride + w];
+                    if (piv0 >= 0) p = cols[static_cast<uint64_t>(piv0) * colstride + w];
+                    if (piv1 >= 0) p ^= cols[static_cast<uint64_t>(piv1) * colstride + w];
+                }
+                if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
+            }
+            sa[lane] = a;
+            sp[lane] = p;
+        }
+        __syncthreads();
+        // ---- B: thread (wave, lane) owns term lane of words it * kWaves + wave
+        double v[kRowsPerThread];
+        uint32_t act = 0, cand = 0, emit = 0;
 #pragma unroll
-        for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-        if (mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type) ||
-            (tr.check_upper && gd.abs_s * abs_c >= tr.upper)) {
-            vsrc[r] = v;
-            em |= bits & (~bits + 1);
+        for (int it = 0; it < kRowsPerThread; ++it) {
+            const int j = it * kWaves + static_cast<int>(wave);
+            const uint64_t r = (w0 + j) * 64 + lane;
+            v[it] = 0.0;
+            if ((sa[j] >> lane) & 1u) {
+                v[it] = coef[r];
+                act |= 1u << it;
+            }
         }
-    }
-    const uint64_t el = em & ~p, ef = em & p;
-    if (w < nwords) {
-        elead[w] = el;
-        efoll[w] = ef;
-    }
-    const uint32_t cl = block_sum(static_cast<uint32_t>(__popcll(el)), lds);
-    const uint32_t cf = block_sum(static_cast<uint32_t>(__popcll(ef)), lds);
-    const uint32_t ca = block_sum(static_cast<uint32_t>(__popcll(a)), lds);
-    if (threadIdx.x == 0) {
-        blk_l[blockIdx.x] = cl;
-        blk_f[blockIdx.x] = cf;
-        if (ca) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NA), static_cast<unsigned long long>(ca));
-        if (cl + cf)
-            atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NEMIT),
-                      static_cast<unsigned long long>(cl + cf));
-    }
-    if (!last_block_done(counters + C_DONE_SWEEP, &last_flag)) return;
-    // tail: exclusive scan of the block counts (leaders and followers), totals to the counters
-    uint32_t carry_l = 0, carry_f = 0;
-    for (uint32_t base = 0; base < gridDim.x; base += kBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t vl = i < gridDim.x ? load_agent(blk_l + i) : 0u;
-        const uint32_t vf = i < gridDim.x ? load_agent(blk_f + i) : 0u;
-        uint32_t tl, tf;
-        const uint32_t xl = block_excl_scan(vl, lds, &tl);
-        const uint32_t xf = block_excl_scan(vf, lds, &tf);
-        if (i < gridDim.x) {
-            blk_l[i] = carry_l + xl;
-            blk_f[i] = carry_f + xf;
+#pragma unroll
+        for (int it = 0; it < kRowsPerThread; ++it) {
+            if (!((act >> it) & 1u)) continue;
+            const uint64_t r = (w0 + it * kWaves + wave) * 64 + lane;
+            coef[r] = v[it] * gd.k;
+            if (!(tr.check_lower && gd.abs_s * fabs(v[it]) <= tr.lower)) cand |= 1u << it;
         }
-        carry_l += tl;
-        carry_f += tf;
-    }
-    if (threadIdx.x == 0) {
-        counters[C_NL] = carry_l;
-        counters[C_NF] = carry_f;
-        counters[C_DONE_SWEEP] = 0;
+#pragma unroll
+        for (int it = 0; it < kRowsPerThread; ++it) {
+            const int j = it * kWaves + static_cast<int>(wave);
+            const bool c = (cand >> it) & 1u;
+            bool e = false;
+            if (__ballot(c)) {
+                if (c) {
+                    const uint64_t r = (w0 + j) * 64 + lane;
+                    uint64_t m[NW];
+                    load_row<NW>(rows, static_cast<uint32_t>(r), m);
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) m[k] ^= gd.g[k];
+                    e = mpa::keep_partner<NW, PAULI>(m, tr.cutoff, tr.cutoff_type) ||
+                        (tr.check_upper && gd.abs_s * fabs(v[it]) >= tr.upper);
+                }
+            }
+            const uint64_t em = __ballot(e);
+            if (lane == 0) sem[j] = em;
+            emit |= (e ? 1u : 0u) << it;
+        }
+        __syncthreads();
+        // ---- C: wave 0 leaders, wave 1 followers
+        if (wave < 2) {
+            const uint64_t em = sem[lane], pw = sp[lane];
+            const uint64_t mine = wave == 0 ? (em & ~pw) : (em & pw);
+            const uint32_t cnt = static_cast<uint32_t>(__popcll(mine));
+            const uint32_t incl = wave_incl_scan(cnt);
+            (wave == 0 ? pre_l : pre_f)[lane] = incl - cnt;
+            const uint32_t agg = __shfl(incl, 63, 64);
+            uint64_t *desc = wave == 0 ? desc_l : desc_f;
+            uint32_t excl = 0;
+            if (tile == 0) {
+                if (lane == 0) desc_publish(desc, ep, kDescIncl, agg);
+            } else {
+                if (lane == 0) desc_publish(desc + tile, ep, kDescAgg, agg);
+                excl = look_back(desc, tile, ep, counters);
+                if (lane == 0) desc_publish(desc + tile, ep, kDescIncl, excl + agg);
+            }
+            if (lane == 0) s_base[wave] = excl;
+            if (wave == 0) {
+                const uint64_t w = w0 + lane;
+                if (w < nwords) elead[w] = mine;
+                const uint32_t na = wave_sum(static_cast<uint32_t>(__popcll(sa[lane])));
+                const uint32_t ne = wave_sum(static_cast<uint32_t>(__popcll(em)));
+                int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
+                if (lane == 0) {
+                    if (na) atomicAdd(reinterpret_cast<unsigned long long *>(st + 1), static_cast<unsigned long long>(na));
+                    if (ne) atomicAdd(reinterpret_cast<unsigned long long *>(st + 2), static_cast<unsigned long long>(ne));
+                    if (tile == ntiles - 1) {
+                        counters[C_NL] = excl + agg;
+                        st[0] = static_cast<int64_t>(N);
+                        st[4] = static_cast<int64_t>(excl + agg);
+                    }
+                }
+            } else if (lane == 0 && tile == ntiles - 1) {
+                counters[C_NF] = excl + agg;
+            }
+        }
+        __syncthreads();
+        // ---- D
+        const uint64_t below = (uint64_t{1} << lane) - 1;
+#pragma unroll
+        for (int it = 0; it < kRowsPerThread; ++it) {
+            if (!((emit >> it) & 1u)) continue;
+            const int j = it * kWaves + static_cast<int>(wave);
+            const uint64_t r = (w0 + j) * 64 + lane;
+            const uint64_t em = sem[j], pw = sp[j];
+            uint64_t slot;
+            if ((pw >> lane) & 1u) {
+                slot = cap_items - 1 - (s_base[1] + pre_f[j] + __popcll(em & pw & below));
+            } else {
+                slot = s_base[0] + pre_l[j] + __popcll(em & ~pw & below);
+            }
+            item[slot] = static_cast<uint32_t>(r);
+            ival[slot] = v[it];
+        }
     }
 }
 
-// One emitter's partner resolution (Engine.h:364-387, 465-513).  A follower whose partner leader
-// emits is rotated by that leader (the reference's matched-follower skip).  A hit applies the rotation
-// against the swept partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a
-// miss stages the new term's coefficient k*0 + (s ph) v.  Returns true on a hit.
-template <int NW, int PAULI>
-__device__ __forceinline__ bool resolve_one(const GateDesc &gd, const uint64_t *__restrict__ rows,
-                                            double *__restrict__ coef, const double *__restrict__ vsrc,
-                                            const uint64_t *__restrict__ elead, const uint64_t *__restrict__ gidx,
-                                            uint64_t gmask, uint32_t S, bool leader, uint32_t *miss_out,
-                                            double *ins_out) {
-    uint64_t m[NW], t[NW];
-    load_row<NW>(rows, S, m);
+// Global row index lookup restricted to rows [0, limit): rows appended while a gate resolves are never
+// the partner of another emitter, so their entries are skipped without reading them.
+template <int NW>
+__device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                   const uint64_t *__restrict__ rows, const uint64_t *key,
+                                                   uint64_t limit) {
+    const uint64_t h = mpa::hash_key<NW>(key);
+    const uint64_t fp = h & 0xffffffff00000000ull;
+    uint64_t pos = h & gmask;
+    for (uint64_t probe = 0; probe <= gmask; ++probe) {
+        const uint64_t e = gidx[pos];
+        if (e == 0) return -1;
+        const uint64_t r = (e & 0xffffffffull) - 1;
+        if ((e & 0xffffffff00000000ull) == fp && r < limit) {
+            const uint64_t *pr = rows + r * NW;
+            bool eq = true;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-    const int64_t T = gidx_find<NW>(gidx, gmask, rows, t);
-    if (T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u)) {
-        *miss_out = 0;  // the emitting leader T rotates this pair
-        return false;
+            for (int w = 0; w < NW; ++w) eq = eq && (pr[w] == key[w]);
+            if (eq) return static_cast<int64_t>(r);
+        }
+        pos = (pos + 1) & gmask;
     }
-    const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
-    const double vS = vsrc[S];
-    if (T >= 0) {
-        const double wS = coef[S], wT = coef[T];
-        const double vTu = wT * gd.inv_k;
-        coef[S] = wS + gd.s * static_cast<double>(-ph) * vTu;
-        coef[T] = wT + gd.s * static_cast<double>(ph) * vS;
-        *miss_out = 0;
-        return true;
-    }
-    coef[S] = coef[S] + gd.s * static_cast<double>(-ph) * 0.0;
-    *ins_out = gd.k * 0.0 + gd.s * static_cast<double>(ph) * vS;
-    *miss_out = 1u;
-    return false;
+    return -1;
 }
 
-// Resolution of every emitter, then (last block) the gate's tail: ordered append of the partners to
-// create (leader misses then follower misses, ascending, Engine.h:428-444), their index entries and
-// column bits, the per-gate counters, N += I and the stop decision for the next gate.  Block b covers
-// the same 256 column words as block b of k_sweep and writes its emitters at the offsets k_sweep
-// computed: leaders at AL[0, nEL), followers at AL[nEL, nEL + nEF).
+// Partner resolution of every emitter (Engine.h:364-387, 465-513) and the ordered append of the
+// partners to create (leader misses then follower misses, ascending, Engine.h:428-444).  One emitter
+// per thread, 256-emitter chunks by work ticket; a chunk's misses land at N + (misses of earlier chunks)
+// by look-back, so the append needs no second pass.  A follower whose partner leader emits is rotated
+// by that leader (the reference's matched-follower skip).  A hit applies the rotation against the swept
+// partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss writes the new
+// term's coefficient k*0 + (s ph) v.  The chunk holding the last emitter publishes N + I for the next
+// gate and stops the device before a gate whose worst case could outgrow the store or the index.
 template <int NW, int PAULI>
-__global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi,
+__global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                    const double *__restrict__ vsrc,
-                                                    const uint64_t *__restrict__ elead,
-                                                    const uint64_t *__restrict__ efoll,
-                                                    const uint32_t *__restrict__ blk_l,
-                                                    const uint32_t *__restrict__ blk_f, uint32_t *__restrict__ AL,
-                                                    uint32_t *__restrict__ miss, double *__restrict__ ins,
-                                                    uint64_t *__restrict__ gidx, uint64_t gmask,
-                                                    uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
+                                                    const uint32_t *__restrict__ item,
+                                                    const double *__restrict__ ival, uint64_t cap_items,
+                                                    const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
+                                                    uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
                                                     uint64_t cap_rows, uint64_t gidx_slots) {
     __shared__ uint32_t lds[kWaves];
-    __shared__ uint32_t last_flag;
-    __shared__ uint32_t sh_tot[2];
+    __shared__ uint32_t s_ticket, s_base;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[C_TK_SWEEP] = 0;
+        counters[C_RB_DONE] = 0;
+    }
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
-    const uint64_t N = counters[C_N];
-    const uint64_t nwords = (N + 63) / 64;
+    const int par = gi & 1, nxt = par ^ 1;
+    const uint64_t N = counters[C_NB + 2 * par];
     const uint64_t nEL = counters[C_NL], nEF = counters[C_NF];
-    const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    const uint64_t el = w < nwords ? elead[w] : 0, ef = w < nwords ? efoll[w] : 0;
-    uint32_t tl, tf;
-    const uint32_t ol = block_excl_scan(static_cast<uint32_t>(__popcll(el)), lds, &tl);
-    const uint32_t of = block_excl_scan(static_cast<uint32_t>(__popcll(ef)), lds, &tf);
-    const uint64_t base_l = blk_l[blockIdx.x], base_f = nEL + blk_f[blockIdx.x];
-    uint64_t pos = base_l + ol;
-    for (uint64_t bits = el; bits; bits &= bits - 1)
-        AL[pos++] = static_cast<uint32_t>(w * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
-    pos = base_f + of;
-    for (uint64_t bits = ef; bits; bits &= bits - 1)
-        AL[pos++] = static_cast<uint32_t>(w * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
-    __syncthreads();
-    uint32_t hits = 0;
-    for (uint32_t i = threadIdx.x; i < tl + tf; i += kBlock) {
-        const bool leader = i < tl;
-        const uint64_t item = leader ? base_l + i : base_f + (i - tl);
-        if (resolve_one<NW, PAULI>(gd, rows, coef, vsrc, elead, gidx, gmask, AL[item], leader, miss + item,
-                                   ins + item))
-            ++hits;
-    }
-    const uint32_t bh = block_sum(hits, lds);
-    if (threadIdx.x == 0 && bh)
-        atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR), static_cast<unsigned long long>(bh));
-    if (!last_block_done(counters + C_DONE_RESOLVE, &last_flag)) return;
-
-    // ---- tail (one block) ----
     const uint64_t n_items = nEL + nEF;
-    uint64_t running = 0;
-    for (uint64_t base = 0; base < n_items; base += kBlock) {
-        const uint64_t i = base + threadIdx.x;
-        const uint32_t v = (i < n_items) ? load_agent(miss + i) : 0u;
-        uint32_t tot;
-        const uint32_t rank = block_excl_scan(v, lds, &tot);
-        if (v) {
-            const uint64_t dest = N + running + rank;
-            const uint64_t *ps = rows + static_cast<uint64_t>(load_agent(AL + i)) * NW;
-            uint64_t *pd = rows + dest * NW;
-            uint64_t t[NW];
-#pragma unroll
-            for (int k = 0; k < NW; ++k) {
-                t[k] = ps[k] ^ gd.g[k];
-                pd[k] = t[k];
-            }
-            coef[dest] = load_agent(ins + i);
-            if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest)))
-                atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
+    const uint64_t nchunks = (n_items + kResChunk - 1) / kResChunk;
+    int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
+    if (nchunks == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            counters[C_NB + 2 * nxt] = N;
+            counters[C_NB + 2 * nxt + 1] = N;
         }
-        running += tot;
+        return;
     }
-    __threadfence_block();
-    __syncthreads();
-    // column bits of the new rows [N, N + running): one wave per 64-term word
-    const uint64_t start = N, count = running;
-    if (count) {
-        const uint64_t w0 = start / 64, wend = (start + count + 63) / 64;
-        const bool lane0 = lane_id() == 0;
-        for (uint64_t cw = w0 + (threadIdx.x >> 6); cw < wend; cw += kWaves) {
-            const uint64_t r = cw * 64 + lane_id();
+    for (;;) {
+        const uint64_t tk = take_ticket(counters + C_TK_RES, &s_ticket);
+        if (tk >= nchunks) break;
+        const uint32_t chunk = static_cast<uint32_t>(tk);
+        const uint64_t i = static_cast<uint64_t>(chunk) * kResChunk + threadIdx.x;
+        uint64_t t[NW];
+        double ins = 0.0;
+        uint32_t miss = 0, hit = 0;
+        if (i < n_items) {
+            const bool leader = i < nEL;
+            const uint64_t slot = leader ? i : cap_items - 1 - (i - nEL);
+            const uint32_t S = item[slot];
+            const double vS = ival[slot];
             uint64_t m[NW];
-            const bool live = r >= start && r < start + count;
+            load_row<NW>(rows, S, m);
 #pragma unroll
-            for (int k = 0; k < NW; ++k) m[k] = live ? rows[r * NW + k] : 0;
-            if (PAULI) {
-                constexpr int L = NW / 2;
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                    for (int bit = 0; bit < 64; ++bit) {
-                        const int q = l * 64 + bit;
-                        if (q >= n_modes) break;
-                        const uint64_t bx = __ballot((m[l] >> bit) & 1u);
-                        const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
-                        if (lane0) {
-                            if (bx) cols[static_cast<uint64_t>(q) * colstride + cw] |= bx;
-                            if (bz) cols[static_cast<uint64_t>(n_modes + q) * colstride + cw] |= bz;
-                        }
-                    }
+            for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+            const int64_t T = gidx_find_below<NW>(gidx, gmask, rows, t, N);
+            const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
+            if (!skip) {
+                const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
+                const double wS = vS * gd.k;
+                if (T >= 0) {
+                    const double wT = coef[T];
+                    const double vTu = wT * gd.inv_k;
+                    coef[S] = wS + gd.s * static_cast<double>(-ph) * vTu;
+                    coef[T] = wT + gd.s * static_cast<double>(ph) * vS;
+                    hit = 1;
+                } else {
+                    coef[S] = wS + gd.s * static_cast<double>(-ph) * 0.0;
+                    ins = gd.k * 0.0 + gd.s * static_cast<double>(ph) * vS;
+                    miss = 1;
                 }
-            } else {
-                int pop = 0;
-#pragma unroll
-                for (int k = 0; k < NW; ++k) {
-                    pop += __popcll(m[k]);
-                    for (int bit = 0; bit < 64; ++bit) {
-                        const int sl = k * 64 + bit;
-                        if (sl >= 2 * n_modes) break;
-                        const uint64_t bb = __ballot((m[k] >> bit) & 1u);
-                        if (lane0 && bb) cols[static_cast<uint64_t>(sl) * colstride + cw] |= bb;
-                    }
-                }
-                const uint64_t bp = __ballot(pop & 1);
-                if (lane0 && bp) cols[static_cast<uint64_t>(2 * n_modes) * colstride + cw] |= bp;
             }
         }
+        uint32_t n_miss;
+        const uint32_t rank = block_excl_scan(miss, lds, &n_miss);
+        const uint32_t n_hit = block_sum(hit, lds);
+        if (threadIdx.x < 64) {
+            uint32_t excl = 0;
+            if (chunk == 0) {
+                if (threadIdx.x == 0) desc_publish(desc_m, ep, kDescIncl, n_miss);
+            } else {
+                if (threadIdx.x == 0) desc_publish(desc_m + chunk, ep, kDescAgg, n_miss);
+                excl = look_back(desc_m, chunk, ep, counters);
+                if (threadIdx.x == 0) desc_publish(desc_m + chunk, ep, kDescIncl, excl + n_miss);
+            }
+            if (threadIdx.x == 0) {
+                s_base = excl;
+                if (n_hit) atomicAdd(reinterpret_cast<unsigned long long *>(st + 5), static_cast<unsigned long long>(n_hit));
+                if (chunk == nchunks - 1) {
+                    const uint64_t I = excl + n_miss, n = N + I;
+                    st[3] = static_cast<int64_t>(I);
+                    counters[C_NB + 2 * nxt] = n;
+                    counters[C_NB + 2 * nxt + 1] = N;
+                    if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+                }
+            }
+        }
+        __syncthreads();
+        if (miss) {
+            const uint64_t dest = N + s_base + rank;
+            uint64_t *pd = rows + dest * NW;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) pd[k] = t[k];
+            coef[dest] = ins;
+            if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest)))
+                atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
+        }
     }
-    if (threadIdx.x == 0) {
-        int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
-        st[0] = static_cast<int64_t>(N);
-        st[1] = static_cast<int64_t>(load_agent(counters + C_NA));
-        st[2] = static_cast<int64_t>(load_agent(counters + C_NEMIT));
-        st[3] = static_cast<int64_t>(running);
-        st[4] = static_cast<int64_t>(nEL);
-        st[5] = static_cast<int64_t>(load_agent(counters + C_NPAIR));
-        st[6] = st[7] = 0;
-        const uint64_t n = N + running;
-        counters[C_N] = n;
-        counters[C_NINS] = 0;
-        counters[C_NEMIT] = 0;
-        counters[C_NPAIR] = 0;
-        counters[C_NA] = 0;
-        counters[C_DONE_RESOLVE] = 0;
-        if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
-    }
-    (void)sh_tot;
 }
 
-// End of a fused gate: per-gate counters, N += I, and the stop decision for the next gate -- it runs
-// only while its worst case (every live term anticommuting and creating a partner) fits the store and
-// keeps the index at most half full.
-__global__ void k_advance_fused(uint64_t *__restrict__ counters, int64_t *__restrict__ stats, int gi, int identity,
-                                uint64_t cap_rows, uint64_t gidx_slots) {
+// Identity gate of the fused pipeline: the live count carries over to the next gate parity.
+__global__ void k_advance_fused(uint64_t *__restrict__ counters, int64_t *__restrict__ stats, int gi) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (counters[C_STOP]) return;
+    const int par = gi & 1, nxt = par ^ 1;
     int64_t *s = stats + static_cast<int64_t>(gi) * kStatW;
-    s[0] = static_cast<int64_t>(counters[C_N]);
-    if (identity) {
-        s[1] = s[2] = s[3] = s[4] = s[5] = s[6] = s[7] = 0;
-    } else {
-        s[1] = static_cast<int64_t>(counters[C_NA]);
-        s[2] = static_cast<int64_t>(counters[C_NEMIT]);
-        s[3] = static_cast<int64_t>(counters[C_NINS]);
-        s[4] = static_cast<int64_t>(counters[C_NL]);
-        s[5] = static_cast<int64_t>(counters[C_NPAIR]);
-        s[6] = s[7] = 0;
-        counters[C_N] += counters[C_NINS];
-    }
-    counters[C_NINS] = 0;
-    counters[C_NEMIT] = 0;
-    counters[C_NPAIR] = 0;
-    counters[C_NA] = 0;
-    const uint64_t n = counters[C_N];
-    if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+    s[0] = static_cast<int64_t>(counters[C_NB + 2 * par]);
+    counters[C_NB + 2 * nxt] = counters[C_NB + 2 * par];
+    counters[C_NB + 2 * nxt + 1] = counters[C_NB + 2 * par + 1];
 }
 
 
@@ -1189,11 +1321,14 @@ struct mpa_engine {
     uint64_t gcos_total = 0, grot_total = 0;  // records stored
     bool building = false;                    // seedc tracks capacity while a graph is built
 
-    // Fused pipeline state: global row index and the pre-gate values of emitters.
-    DevArray gidx, vsrc;
+    // Fused pipeline state: global row index, the emitter list with pre-gate values (cap / 2 entries:
+    // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
+    DevArray gidx, f_item, f_ival, f_elead, desc_l, desc_f, desc_m;
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
+    uint64_t cap_items = 0;
+    uint32_t desc_epoch = 0;    // per fused gate; descriptors of other epochs read as unpublished
     static constexpr size_t kChunk = 256;  // gates enqueued between host synchronisations
     bool fused_enabled = true;             // mpa_set_pipeline(e, 0) forces the per-gate pipeline
     uint64_t table_slots = 0;
@@ -1619,15 +1754,32 @@ struct mpa_engine {
         for (size_t l = nl; l-- > 0;) grad[layers[l].param_index] += layers[l].gen_coeff * dl[l];
     }
 
-    // Per-row arrays of the fused pipeline sized to the store, and a global index with >= 2 slots per
-    // row of capacity covering every stored row.
+    // Arrays of the fused pipeline sized to the store (a gate runs only while N <= cap / 2), and a global
+    // index with >= 2 slots per row of capacity covering every stored row.
     template <int NW>
     void ensure_fused_arrays() {
         if (fused_cap != cap) {
-            vsrc.reserve(cap * 8);
-            miss.reserve(cap * 4);
-            ins.reserve(cap * 8);
+            cap_items = cap / 2;
+            f_item.reserve(cap_items * 4);
+            f_ival.reserve(cap_items * 8);
+            f_elead.reserve((cap / 64) * 8);
+            const uint64_t tiles = cap_items / (kTileWords * 64) + 2;
+            const uint64_t chunks = cap_items / kResChunk + 2;
+            desc_l.reserve(tiles * 8);
+            desc_f.reserve(tiles * 8);
+            desc_m.reserve(chunks * 8);
+            // fresh descriptor memory reads as epoch 0, which no gate uses
+            HIP_TRY(hipMemsetAsync(desc_l.p, 0, desc_l.bytes, stream));
+            HIP_TRY(hipMemsetAsync(desc_f.p, 0, desc_f.bytes, stream));
+            HIP_TRY(hipMemsetAsync(desc_m.p, 0, desc_m.bytes, stream));
+            desc_epoch = 0;
             fused_cap = cap;
+        }
+        if (desc_epoch >= (1u << 30) - kChunk - 1) {
+            HIP_TRY(hipMemsetAsync(desc_l.p, 0, desc_l.bytes, stream));
+            HIP_TRY(hipMemsetAsync(desc_f.p, 0, desc_f.bytes, stream));
+            HIP_TRY(hipMemsetAsync(desc_m.p, 0, desc_m.bytes, stream));
+            desc_epoch = 0;
         }
         uint64_t want = 1024;
         while (want < 2 * cap) want <<= 1;
@@ -1648,28 +1800,44 @@ struct mpa_engine {
         }
     }
 
+    // Algorithmic bytes of the fused kernels from the per-gate counters: fold + pivot column words, the
+    // anticommuting coefficients (read + write), the emitters' rows and list entries, leader masks; the
+    // column rebuild of the previous gate's appended rows; per emitter its list entry, row, index slot,
+    // candidate row, leader-mask word and coefficients; per appended row the row, coefficient and slot.
     void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
         const double NW = nw;
+        double prev_inserted = 0.0;
         for (size_t g = 0; g < stat_n.size(); ++g) {
             if (fold_plus_piv[g] == 0) continue;
             const double words = std::ceil(static_cast<double>(stat_n[g]) / 64.0);
-            const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], EL = stat_l[g], hits = stat_p[g];
-            const double EF = E - EL;
-            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 2) + nA * 16.0 + E * (8 * NW + 8);
-            prof_bytes[K_SCANSUM] += 2.0 * 8.0 * std::ceil(words / kBlock);
-            prof_bytes[K_EXPAND] += words * 16.0 + E * 4.0;
-            // resolve: emit masks, emitter list, row + vsrc + index slot + candidate row per emitter, the
-            // pair's coefficients, miss flag / staged value; tail: appended rows, index slots, column bits
-            (void)EL;
-            (void)EF;
-            prof_bytes[K_RESOLVE] += words * 16.0 + E * (4 + 8 * NW + 8 + 8 + 8 * NW + 16 + 4) + hits * 16 +
-                                     (E - hits) * 8 + E * 8 + I * (4 + 16 * NW + 8 + 8 + 8 + 8 * NW + ncols / 8.0);
+            const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], hits = stat_p[g];
+            const double rb_words = prev_inserted > 0 ? std::ceil(prev_inserted / 64.0) + 1 : 0.0;
+            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + nA * 16.0 + E * (8 * NW + 12) +
+                                   rb_words * (64 * 8 * NW + ncols * 8.0);
+            prof_bytes[K_RESOLVE] += E * (12 + 8 * NW + 8 + 8 * NW + 8 + 8) + hits * 16 + I * (8 * NW + 8 + 8);
+            prev_inserted = I;
         }
     }
 
-    // Contract-immediately propagation without host synchronisation between gates: gates are
-    // enqueued in chunks; the device stops before a gate whose worst case could outgrow the store,
-    // the host grows it and resumes there.
+    // Per gate parity {N, N_cols} and the work tickets, as the device pipeline expects them before gate
+    // `g`: every column bit current, nothing stopped.
+    void reset_fused_counters(size_t g) {
+        uint64_t h[C_COUNT];
+        HIP_TRY(hipMemcpyAsync(h, counters.p, C_COUNT * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        const int par = static_cast<int>(g & 1);
+        h[C_NB + 2 * par] = N;
+        h[C_NB + 2 * par + 1] = N;
+        h[C_N] = N;
+        h[C_STOP] = 0;
+        h[C_TK_SWEEP] = h[C_TK_RES] = h[C_RB_DONE] = 0;
+        HIP_TRY(hipMemcpyAsync(counters.p, h, C_COUNT * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+
+    // Contract-immediately propagation without host synchronisation between gates: two persistent
+    // launches per gate, enqueued in chunks; the device stops before a gate whose worst case could
+    // outgrow the store, the host grows it and resumes there.
     template <int NW, int PAULI>
     void propagate_fused(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds) {
         const size_t ng = gds.size();
@@ -1679,9 +1847,11 @@ struct mpa_engine {
         if (ng) HIP_TRY(hipMemcpyAsync(gdesc.p, gds.data(), ng * sizeof(GateDesc), hipMemcpyHostToDevice, stream));
         if (!folds.empty())
             HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemsetAsync(stats.p, 0, std::max<size_t>(ng, 1) * kStatW * 8, stream));
         std::vector<int32_t> fold_plus_piv(ng, 0);
         for (size_t g = 0; g < ng; ++g)
             if (!gds[g].identity) fold_plus_piv[g] = gds[g].fold_cnt + (gds[g].piv0 >= 0) + (gds[g].piv1 >= 0);
+        reset_fused_counters(0);
         hipEvent_t ev[2];
         HIP_TRY(hipEventCreate(&ev[0]));
         HIP_TRY(hipEventCreate(&ev[1]));
@@ -1691,8 +1861,10 @@ struct mpa_engine {
             if (2 * N > cap) ensure_capacity(std::max<uint64_t>(4 * N, 64ull * kBlock));
             ensure_fused_arrays<NW>();
             const size_t end = std::min(ng, gi + kChunk);
-            const uint64_t words_ub = (cap / 2 + 63) / 64;
-            const uint32_t sblocks = static_cast<uint32_t>(std::max<uint64_t>((words_ub + kBlock - 1) / kBlock, 1));
+            const uint64_t tiles_ub = (cap_items + kTileWords * 64 - 1) / (kTileWords * 64);
+            const uint32_t sgrid = static_cast<uint32_t>(std::clamp<uint64_t>(tiles_ub, 1, kMaxGrid));
+            const uint32_t rgrid =
+                static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, kMaxGrid));
             const uint64_t gmask = gidx_slots - 1;
             for (size_t g = gi; g < end; ++g) {
                 const GateDesc &gd = gds[g];
@@ -1701,47 +1873,56 @@ struct mpa_engine {
                 if (gd.identity) {
                     pa = prof_begin();
                     hipLaunchKernelGGL(k_advance_fused, dim3(1), dim3(64), 0, stream, counters.as<uint64_t>(),
-                                       stats.as<int64_t>(), ig, 1, cap, gidx_slots);
+                                       stats.as<int64_t>(), ig);
                     HIP_TRY(hipGetLastError());
                     prof_end(K_ADVANCE, pa);
                     continue;
                 }
+                const uint32_t ep = ++desc_epoch;
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sblocks), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                   ig, tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off,
-                                   gd.fold_cnt, gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(),
-                                   vsrc.as<double>(), lead.as<uint64_t>(), foll.as<uint64_t>(), blk_a.as<uint32_t>(),
-                                   blk_b.as<uint32_t>(), counters.as<uint64_t>());
+                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                                   ig, ep, tr, cols.as<uint64_t>(), colstride(), n_modes,
+                                   fold.as<int32_t>() + gd.fold_off, gd.fold_cnt, gd.piv0, gd.piv1,
+                                   rows.as<uint64_t>(), coef.as<double>(), f_item.as<uint32_t>(),
+                                   f_ival.as<double>(), cap_items, f_elead.as<uint64_t>(), desc_l.as<uint64_t>(),
+                                   desc_f.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>());
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(sblocks), dim3(kBlock), 0, stream,
-                                   gdesc.as<GateDesc>(), ig, rows.as<uint64_t>(), coef.as<double>(), vsrc.as<double>(),
-                                   lead.as<uint64_t>(), foll.as<uint64_t>(), blk_a.as<uint32_t>(),
-                                   blk_b.as<uint32_t>(), AL.as<uint32_t>(), miss.as<uint32_t>(), ins.as<double>(),
-                                   gidx.as<uint64_t>(), gmask, cols.as<uint64_t>(), colstride(), n_modes,
-                                   counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots);
+                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), 0, stream,
+                                   gdesc.as<GateDesc>(), ig, ep, rows.as<uint64_t>(), coef.as<double>(),
+                                   f_item.as<uint32_t>(), f_ival.as<double>(), cap_items, f_elead.as<uint64_t>(),
+                                   gidx.as<uint64_t>(), gmask, desc_m.as<uint64_t>(), counters.as<uint64_t>(),
+                                   stats.as<int64_t>(), cap, gidx_slots);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
             }
             sync_counters();
             prof_resolve();
-            if (h_counters[C_ERR]) throw RunError("device index overflow");
-            N = static_cast<uint64_t>(h_counters[C_N]);
-            gidx_rows = N;
-            const uint64_t stop = static_cast<uint64_t>(h_counters[C_STOP]);
-            if (stop) {
-                gi = static_cast<size_t>(stop - 1);
-                const uint64_t zero = 0;
-                HIP_TRY(hipMemcpyAsync(counters.as<uint64_t>() + C_STOP, &zero, 8, hipMemcpyHostToDevice, stream));
-                HIP_TRY(hipStreamSynchronize(stream));
-                if (N >= (uint64_t{1} << 31)) throw RunError("term count exceeds the 32-bit row index");
-            } else {
-                gi = end;
+            if (h_counters[C_ERR]) {
+                const int64_t err = h_counters[C_ERR];
+                throw RunError(err & static_cast<int64_t>(kErrIndex) ? "device index overflow"
+                                                                     : "device pipeline wait limit exceeded");
             }
+            const uint64_t stop = static_cast<uint64_t>(h_counters[C_STOP]);
+            const size_t next = stop ? static_cast<size_t>(stop - 1) : end;
+            const int par = static_cast<int>(next & 1);
+            N = static_cast<uint64_t>(h_counters[C_NB + 2 * par]);
+            const uint64_t ncols_done = static_cast<uint64_t>(h_counters[C_NB + 2 * par + 1]);
+            if (ncols_done < N) {
+                hipEvent_t pa = prof_begin();
+                launch_cols_update<NW, PAULI>(ncols_done, N - ncols_done, false, N - ncols_done);
+                prof_end(K_COLS, pa);
+                if (prof_on) prof_bytes[K_COLS] += (N - ncols_done) * (8.0 * nw + ncols / 8.0);
+            }
+            gidx_rows = N;
+            if (N >= (uint64_t{1} << 31)) throw RunError("term count exceeds the 32-bit row index");
+            reset_fused_counters(next);
+            gi = next;
         }
         HIP_TRY(hipEventRecord(ev[1], stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        prof_resolve();
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
         last_propagate_ms = ms;
