@@ -55,11 +55,10 @@ enum : int {
     C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6,
     C_STOP = 7,              // fused pipeline: 1 + index of the gate to resume at (0 = running)
     C_GCOS = 8, C_GROT = 9,  // graph record counts (cumulative over a build)
-    // fused pipeline: work tickets of the sweep and resolve kernels (each kernel resets the other's),
-    // rebuilt column words of the sweep, and per gate parity {N, N_cols}: the live count and the
-    // number of leading rows whose column bits are current
-    C_TK_SWEEP = 10, C_TK_RES = 11, C_RB_DONE = 12,
-    C_NB = 16,  // C_NB + 2 * parity: N;  C_NB + 2 * parity + 1: N_cols
+    // fused pipeline: work tickets of the index and resolve kernels (each resets the other's), and the
+    // live count per gate parity (C_NB + (gate & 1))
+    C_TK_IDX = 10, C_TK_RES = 11,
+    C_NB = 16,
     C_COUNT = 24
 };
 // per-gate stats: N, nA, emitted, inserted, leaders, pairs, graph cos-record end, graph rotation end
@@ -68,10 +67,10 @@ constexpr uint32_t kPhaseBit = 0x80000000u;  // rotation record: target row | (p
 
 // Kernel classes timed by the profiler (mpa_kernel_profile).
 enum KernelKind : int { K_SCAN, K_SCANSUM, K_EXPAND, K_LEADERS, K_FOLLOWERS, K_UNMATCHED, K_COUNT, K_INSERT,
-                        K_COLS, K_ADVANCE, K_SWEEP, K_RESOLVE, K_KINDS };
+                        K_COLS, K_ADVANCE, K_SWEEP, K_INDEX, K_RESOLVE, K_KINDS };
 const char *const kKindNames[K_KINDS] = {"k_scan", "k_scan_sums", "k_expand", "k_leaders", "k_followers",
                                          "k_leaders_unmatched", "k_count_miss", "k_insert", "k_cols_update",
-                                         "k_advance", "k_sweep", "k_resolve"};
+                                         "k_advance", "k_sweep", "k_index", "k_resolve"};
 
 thread_local std::string g_error;
 
@@ -705,22 +704,12 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Block-wide work ticket (every thread gets the same value).
-__device__ __forceinline__ uint32_t take_ticket(uint64_t *ctr, uint32_t *slot) {
-    __syncthreads();
-    if (threadIdx.x == 0)
-        *slot = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(ctr), 1ull));
-    __syncthreads();
-    return *slot;
-}
-
 // Ordered prefix sums across the tiles of one launch (single-pass scan with decoupled look-back).
-// Tiles are numbered by work ticket, so every tile a tile waits on is already owned by a running
-// block.  Descriptor word: epoch (30 bits) | status (2 bits) | value (32 bits); a descriptor of another
-// epoch (another gate) reads as not yet published, so the arrays are never cleared between gates.
+// Descriptor word: epoch (30 bits) | status (2 bits) | value (32 bits); a descriptor of another epoch
+// (another gate) reads as not yet published, so the arrays are never cleared between gates.
 constexpr uint64_t kDescAgg = 1, kDescIncl = 2;
 constexpr uint32_t kSpinLimit = 1u << 22;  // bounded waits: a broken invariant flags an error, never hangs
-constexpr uint64_t kErrIndex = 2, kErrLookBack = 4, kErrRebuild = 8;
+constexpr uint64_t kErrIndex = 2, kErrLookBack = 4;
 
 __device__ __forceinline__ uint64_t desc_pack(uint32_t ep, uint64_t status, uint32_t value) {
     return (static_cast<uint64_t>(ep) << 34) | (status << 32) | value;
@@ -730,269 +719,381 @@ __device__ __forceinline__ void desc_publish(uint64_t *d, uint32_t ep, uint64_t 
     __hip_atomic_store(d, desc_pack(ep, status, value), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive prefix of tile t over tiles [0, t); called by all 64 lanes of one wave.
+// Exclusive prefix of tile t over tiles [0, t); called by all 64 lanes of one wave.  Each lane reads
+// kLookSpan descriptors per round trip (512 per wave), so a tile whose predecessors have published
+// only aggregates still finds its prefix in one or two round trips.
+constexpr int kLookSpan = 8;
+
 __device__ uint32_t look_back(const uint64_t *desc, uint32_t t, uint32_t ep, uint64_t *counters) {
     const uint32_t lane = lane_id();
     uint32_t sum = 0, spins = 0;
     int64_t pos = static_cast<int64_t>(t) - 1;
     while (pos >= 0) {
-        const int64_t q = pos - static_cast<int64_t>(lane);
-        uint64_t d = desc_pack(ep, kDescIncl, 0);  // before tile 0: an inclusive prefix of 0
-        if (q >= 0) d = load_agent(desc + q);
-        const uint32_t st = (d >> 34) == ep ? static_cast<uint32_t>((d >> 32) & 3u) : 0u;
-        const uint32_t val = static_cast<uint32_t>(d);
-        const uint64_t incl = __ballot(st == kDescIncl);
-        const uint64_t stop = incl | __ballot(st == 0);
-        if (stop == 0) {  // 64 aggregates
-            sum += wave_sum(val);
-            pos -= 64;
+        // lane l covers tiles pos - kLookSpan l - i, i = 0 .. kLookSpan - 1, nearest first
+        uint64_t d[kLookSpan];
+#pragma unroll
+        for (int i = 0; i < kLookSpan; ++i) {
+            const int64_t q = pos - static_cast<int64_t>(kLookSpan * lane) - i;
+            d[i] = q >= 0 ? load_agent(desc + q) : desc_pack(ep, kDescIncl, 0);  // before tile 0: prefix 0
+        }
+        uint32_t part = 0;
+        int stop_i = kLookSpan;
+        bool stop_incl = false;
+#pragma unroll
+        for (int i = 0; i < kLookSpan; ++i) {
+            const uint32_t st = (d[i] >> 34) == ep ? static_cast<uint32_t>((d[i] >> 32) & 3u) : 0u;
+            if (stop_i == kLookSpan) {
+                if (st == kDescAgg) {
+                    part += static_cast<uint32_t>(d[i]);
+                } else {
+                    stop_i = i;
+                    stop_incl = st == kDescIncl;
+                    if (stop_incl) part += static_cast<uint32_t>(d[i]);
+                }
+            }
+        }
+        const uint64_t has_stop = __ballot(stop_i < kLookSpan);
+        if (has_stop == 0) {  // 64 kLookSpan aggregates
+            sum += wave_sum(part);
+            pos -= 64 * kLookSpan;
             continue;
         }
-        const uint32_t first = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(stop)) - 1);
-        if ((incl >> first) & 1u) {
-            sum += wave_sum(lane <= first ? val : 0u);
-            break;
+        const int first = __ffsll(static_cast<unsigned long long>(has_stop)) - 1;
+        const int f_incl = __shfl(stop_incl ? 1 : 0, first, 64);
+        const int f_i = __shfl(stop_i, first, 64);
+        sum += wave_sum(static_cast<int>(lane) <= first ? part : 0u);
+        if (f_incl) break;
+        pos -= static_cast<int64_t>(kLookSpan) * first + f_i;  // the first tile not yet published
+        // wait on that one descriptor only, then read the window again from it
+        int ok = 1;
+        if (lane == 0) {
+            for (;;) {
+                const uint64_t dd = load_agent(desc + pos);
+                if ((dd >> 34) == ep && ((dd >> 32) & 3u) != 0) break;
+                if (++spins > kSpinLimit) {
+                    atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrLookBack);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
         }
-        sum += wave_sum(lane < first ? val : 0u);  // aggregates before the first unpublished tile
-        pos -= first;
-        if (++spins > kSpinLimit) {
-            if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrLookBack);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+        if (!__shfl(ok, 0, 64)) break;
     }
     return sum;
 }
 
-// Column bits of one 64-term word rebuilt from the rows (one wave; lane = term).  The whole word is
-// written, so terms appended since the last rebuild get their bits without read-modify-write.
+// Column bits of an appended term, gathered in LDS per (column, word) for the chunk's words
+// [wlo, wlo + 5) (at most 256 consecutive new terms) before one global OR per nonzero word.
 template <int NW, int PAULI>
-__device__ __forceinline__ void rebuild_word(const uint64_t *__restrict__ rows, uint64_t *__restrict__ cols,
-                                             uint64_t colstride, int n_modes, uint64_t N, uint64_t w) {
-    const uint32_t lane = lane_id();
-    const uint64_t r = w * 64 + lane;
-    uint64_t m[NW];
-#pragma unroll
-    for (int k = 0; k < NW; ++k) m[k] = r < N ? rows[r * NW + k] : 0;
+constexpr int max_cols() {
+    return PAULI ? 64 * NW : 64 * NW + 1;
+}
+
+template <int NW, int PAULI>
+__device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, int n_modes, const uint64_t *m,
+                                                   uint64_t rel_word, uint64_t bit) {
+    uint64_t *base = lcol + rel_word * ncols;
     if (PAULI) {
         constexpr int L = NW / 2;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            uint64_t mx = 0, mz = 0;
-            for (int bit = 0; bit < 64 && l * 64 + bit < n_modes; ++bit) {
-                const uint64_t bx = __ballot((m[l] >> bit) & 1u);
-                const uint64_t bz = __ballot((m[L + l] >> bit) & 1u);
-                if (lane == static_cast<uint32_t>(bit)) {
-                    mx = bx;
-                    mz = bz;
-                }
-            }
-            const int q = l * 64 + static_cast<int>(lane);
-            if (q < n_modes) {
-                cols[static_cast<uint64_t>(q) * colstride + w] = mx;
-                cols[static_cast<uint64_t>(n_modes + q) * colstride + w] = mz;
-            }
+            for (uint64_t b = m[l]; b; b &= b - 1)
+                atomicOr(reinterpret_cast<unsigned long long *>(base + l * 64 + __ffsll(static_cast<unsigned long long>(b)) - 1), bit);
+            for (uint64_t b = m[L + l]; b; b &= b - 1)
+                atomicOr(reinterpret_cast<unsigned long long *>(base + n_modes + l * 64 +
+                                                                __ffsll(static_cast<unsigned long long>(b)) - 1),
+                         bit);
         }
     } else {
         int pop = 0;
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             pop += __popcll(m[k]);
-            uint64_t mk = 0;
-            for (int bit = 0; bit < 64 && k * 64 + bit < 2 * n_modes; ++bit) {
-                const uint64_t bb = __ballot((m[k] >> bit) & 1u);
-                if (lane == static_cast<uint32_t>(bit)) mk = bb;
-            }
-            const int sl = k * 64 + static_cast<int>(lane);
-            if (sl < 2 * n_modes) cols[static_cast<uint64_t>(sl) * colstride + w] = mk;
+            for (uint64_t b = m[k]; b; b &= b - 1)
+                atomicOr(reinterpret_cast<unsigned long long *>(base + k * 64 + __ffsll(static_cast<unsigned long long>(b)) - 1), bit);
         }
-        const uint64_t bp = __ballot(pop & 1);
-        if (lane == 0) cols[static_cast<uint64_t>(2 * n_modes) * colstride + w] = bp;
+        if (pop & 1) atomicOr(reinterpret_cast<unsigned long long *>(base + 2 * n_modes), bit);
     }
 }
 
-constexpr int kTileWords = 64;                          // sweep tile: 64 column words = 4096 terms
-constexpr int kRowsPerThread = kTileWords * 64 / kBlock;  // 16
-constexpr int kResChunk = kBlock;                       // resolve: one emitter per thread
-constexpr uint32_t kMaxGrid = 1024;                     // persistent grids (tickets hand out the work)
+constexpr int kWTileWords = 8;                           // sweep: one wave per 8 column words
+constexpr int kWTileRows = kWTileWords * 64;              // = 512 terms
+constexpr int kIdxTilesPerThread = 4;
+constexpr int kIdxTiles = kBlock * kIdxTilesPerThread;    // index: 1024 sweep tiles per block
+constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
+constexpr uint32_t kMaxGrid = 1024;                       // resolve blocks that take chunks
+// per-tile count word: leaders | followers << 10 | anticommuting << 20 (each <= 512)
+constexpr uint32_t kCntBits = 10, kCntMask = (1u << kCntBits) - 1;
 
-// Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate, and the ordered
-// emitter lists.  Work tickets [0, R) rebuild the column words of the terms appended by the previous
-// gate (one word per wave); tickets [R, R + tiles) are 4096-term tiles.  Per tile:
-//   A  XOR of G's fold columns -> anticommuting mask, pivot mask (one lane per column word)
-//   B  c <- c k on every anticommuting term (the reference's in-place sweep; 16 coalesced loads in
-//      flight per thread), then the emission test of the terms past the lower_atol gate against the
-//      structural cutoff of their partner
-//   C  tile counts of leader / follower emitters (pivot clear / set), their global offsets by look-back
-//   D  emitters with their pre-gate values: leaders ascending at ITEM[0, nEL), followers ascending at
-//      ITEM[cap - 1 - k]; leader emit masks per word for the followers' skip test
+// Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate.  One wave per
+// 512-term tile, no dependence between tiles:
+//   A  G's fold and pivot columns of the tile's 8 words, one load per lane (lane = column * 8 + word),
+//      XOR-reduced across lanes -> anticommuting mask a_j, pivot mask p_j
+//   B  c <- c k on every anticommuting term (the reference's in-place sweep; 8 loads in flight per lane);
+//      terms past the lower_atol gate go to a per-wave candidate list in LDS with their pre-gate value
+//   E  candidates, one per lane: partner key m ^ G against the structural cutoff (or the upper_atol
+//      rescue) -> emit bits
+//   D  the tile's emitters with pre-gate value and partner key into the tile's own region
+//      [512 tile, 512 tile + n): leaders ascending, then followers ascending (pivot clear / set);
+//      the tile's counts; leader emit masks per word for the followers' skip test
 template <int NW, int PAULI>
-__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep, Trunc tr,
-                                                  uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
+__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
+                                                  const uint64_t *__restrict__ cols, uint64_t colstride,
                                                   const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                  uint32_t *__restrict__ item, double *__restrict__ ival,
-                                                  uint64_t cap_items, uint64_t *__restrict__ elead,
-                                                  uint64_t *__restrict__ desc_l, uint64_t *__restrict__ desc_f,
-                                                  uint64_t *__restrict__ counters, int64_t *__restrict__ stats) {
-    __shared__ uint64_t sa[kTileWords], sp[kTileWords], sem[kTileWords];
-    __shared__ uint32_t pre_l[kTileWords], pre_f[kTileWords];
-    __shared__ uint32_t s_ticket, s_base[2];
+                                                  uint32_t *__restrict__ t_item, double *__restrict__ t_val,
+                                                  uint64_t *__restrict__ t_key, uint64_t *__restrict__ elead,
+                                                  uint32_t *__restrict__ tcnt, const uint64_t *__restrict__ counters) {
+    __shared__ uint16_t cidx[kWaves][kWTileRows];
+    __shared__ double cval[kWaves][kWTileRows];
+    __shared__ uint64_t sem[kWaves][kWTileWords], s_pw[kWaves][kWTileWords];
+    __shared__ uint2 s_pre[kWaves][kWTileWords];
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
-    const int par = gi & 1;
-    const uint64_t N = counters[C_NB + 2 * par], Ncols = counters[C_NB + 2 * par + 1];
+    const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t nwords = (N + 63) / 64;
-    const uint64_t ntiles = (nwords + kTileWords - 1) / kTileWords;
-    const uint64_t rb_lo = Ncols < N ? Ncols / 64 : nwords;
-    const uint64_t nrb = nwords - rb_lo;
-    const uint64_t R = (nrb + kWaves - 1) / kWaves;
+    const uint64_t ntiles = (nwords + kWTileWords - 1) / kWTileWords;
+    if (static_cast<uint64_t>(blockIdx.x) * kWaves >= ntiles) return;  // whole block idle
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
+    const bool live = tile < ntiles;  // idle waves of the last block still take part in the barriers
+    const uint64_t row0 = tile * kWTileRows;
+    // ---- A
+    uint64_t a = 0, p = 0;
+    {
+        const int j = static_cast<int>(lane & 7), c = static_cast<int>(lane >> 3);
+        const uint64_t w = tile * kWTileWords + j;
+        const int npiv = (piv0 >= 0) + (piv1 >= 0);
+        const int ncol = nfold + npiv;
+        for (int c0 = 0; c0 < ncol; c0 += 8) {
+            const int cc = c0 + c;
+            uint64_t x = 0;
+            if (live && w < nwords && cc < ncol) {
+                const int col = cc < nfold ? fold[cc] : (cc == nfold ? piv0 : piv1);
+                x = cols[static_cast<uint64_t>(col) * colstride + w];
+            }
+            if (cc < nfold) a ^= x;
+            else p ^= x;
+        }
+#pragma unroll
+        for (int d = 8; d < 64; d <<= 1) {
+            a ^= __shfl_xor(a, d, 64);
+            p ^= __shfl_xor(p, d, 64);
+        }
+        if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
+        if (!live || w >= nwords) a = 0;
+        if (lane < kWTileWords) sem[wave][lane] = 0;
+    }
+    // lanes j < 8 hold word j's masks
+    // ---- B
+    double v[kWTileWords];
+    uint32_t cand = 0;
+    {
+        uint32_t act = 0;
+#pragma unroll
+        for (int j = 0; j < kWTileWords; ++j) {
+            const uint64_t aj = __shfl(a, j, 64);
+            v[j] = 0.0;
+            if ((aj >> lane) & 1u) {
+                v[j] = coef[row0 + static_cast<uint64_t>(j) * 64 + lane];
+                act |= 1u << j;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kWTileWords; ++j) {
+            if (!((act >> j) & 1u)) continue;
+            coef[row0 + static_cast<uint64_t>(j) * 64 + lane] = v[j] * gd.k;
+            if (!(tr.check_lower && gd.abs_s * fabs(v[j]) <= tr.lower)) cand |= 1u << j;
+        }
+    }
+    const uint32_t mine = static_cast<uint32_t>(__popc(cand));
+    const uint32_t incl = wave_incl_scan(mine);
+    const uint32_t ncand = __shfl(incl, 63, 64);
+    {
+        uint32_t pos = incl - mine;
+#pragma unroll
+        for (int j = 0; j < kWTileWords; ++j) {
+            if (!((cand >> j) & 1u)) continue;
+            cidx[wave][pos] = static_cast<uint16_t>(j * 64 + lane);
+            cval[wave][pos] = v[j];
+            ++pos;
+        }
+    }
+    __syncthreads();
+    // ---- E: keys of the first two rounds stay in registers for D
+    uint64_t key0[NW], key1[NW];
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
+        const uint32_t c = lane + round * 64;
+        if (c < ncand) {
+            const uint32_t idx = cidx[wave][c];
+            uint64_t *key = round == 0 ? key0 : key1;
+            load_row<NW>(rows, static_cast<uint32_t>(row0 + idx), key);
+#pragma unroll
+            for (int k = 0; k < NW; ++k) key[k] ^= gd.g[k];
+            if (mpa::keep_partner<NW, PAULI>(key, tr.cutoff, tr.cutoff_type) ||
+                (tr.check_upper && gd.abs_s * fabs(cval[wave][c]) >= tr.upper))
+                atomicOr(reinterpret_cast<unsigned long long *>(&sem[wave][idx >> 6]), uint64_t{1} << (idx & 63));
+        }
+    }
+    for (uint32_t c = lane + 128; c < ncand; c += 64) {
+        const uint32_t idx = cidx[wave][c];
+        uint64_t key[NW];
+        load_row<NW>(rows, static_cast<uint32_t>(row0 + idx), key);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) key[k] ^= gd.g[k];
+        if (mpa::keep_partner<NW, PAULI>(key, tr.cutoff, tr.cutoff_type) ||
+            (tr.check_upper && gd.abs_s * fabs(cval[wave][c]) >= tr.upper))
+            atomicOr(reinterpret_cast<unsigned long long *>(&sem[wave][idx >> 6]), uint64_t{1} << (idx & 63));
+    }
+    __syncthreads();
+    // ---- counts: lane j < 8 owns word j
+    const uint64_t em_mine = lane < kWTileWords ? sem[wave][lane] : 0;
+    const uint64_t sel = em_mine & ~p, sef = em_mine & p;
+    const uint32_t cl = lane < kWTileWords ? static_cast<uint32_t>(__popcll(sel)) : 0u;
+    const uint32_t cf = lane < kWTileWords ? static_cast<uint32_t>(__popcll(sef)) : 0u;
+    const uint32_t il = wave_incl_scan(cl), iff = wave_incl_scan(cf);
+    const uint32_t nl = __shfl(il, 63, 64), nf = __shfl(iff, 63, 64);
+    const uint32_t na = wave_sum(lane < kWTileWords ? static_cast<uint32_t>(__popcll(a)) : 0u);
+    if (lane < kWTileWords) {
+        s_pw[wave][lane] = p;
+        s_pre[wave][lane] = make_uint2(il - cl, iff - cf);
+        const uint64_t w = tile * kWTileWords + lane;
+        if (live && w < nwords) elead[w] = sel;
+    }
+    if (live && lane == 0) tcnt[tile] = nl | (nf << kCntBits) | (na << (2 * kCntBits));
+    __syncthreads();
+    if (!live) return;
+    // ---- D
+    for (uint32_t c = lane; c < ncand; c += 64) {
+        const uint32_t idx = cidx[wave][c];
+        const int j = static_cast<int>(idx >> 6);
+        const uint32_t b = idx & 63;
+        const uint64_t em = sem[wave][j], pw = s_pw[wave][j];
+        const uint32_t pl = s_pre[wave][j].x, pf = s_pre[wave][j].y;
+        if (!((em >> b) & 1u)) continue;
+        const uint64_t bl = (uint64_t{1} << b) - 1;
+        const uint32_t rank = ((pw >> b) & 1u) ? nl + pf + static_cast<uint32_t>(__popcll(em & pw & bl))
+                                               : pl + static_cast<uint32_t>(__popcll(em & ~pw & bl));
+        const uint64_t slot = row0 + rank;
+        t_item[slot] = static_cast<uint32_t>(row0 + idx);
+        t_val[slot] = cval[wave][c];
+        uint64_t *pk = t_key + slot * NW;
+        if (c < 128) {
+            const bool first = c < 64;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) pk[k] = first ? key0[k] : key1[k];
+        } else {
+            const uint64_t *pr = rows + (row0 + idx) * NW;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) pk[k] = pr[k] ^ gd.g[k];
+        }
+    }
+}
+
+// Ordered emitter lists from the sweep's tiles: exclusive prefixes of the per-tile counts across the
+// launch (ticket-ordered blocks, decoupled look-back) turn into a map from list position to tile-region
+// slot: leaders at MAP[0, nEL), followers at MAP[cap - 1 - k], each ascending.  The map is written
+// with no loads, spread evenly over the block whatever the tiles' emitter counts (a binary search
+// over the block's tile prefixes in LDS).  The block holding the last tile publishes the totals and
+// the gate's statistics.
+__device__ __forceinline__ uint32_t upper_tile(const uint32_t *pre, uint32_t n, uint32_t f) {
+    // largest t < n with pre[t] <= f (pre ascending, pre[0] = 0)
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uint32_t *__restrict__ tcnt,
+                                                  uint32_t *__restrict__ map, uint64_t cap_items,
+                                                  uint64_t *__restrict__ desc_l, uint64_t *__restrict__ desc_f,
+                                                  uint64_t *__restrict__ desc_a, uint64_t *__restrict__ counters,
+                                                  int64_t *__restrict__ stats) {
+    __shared__ uint32_t lds[kWaves];
+    __shared__ uint32_t s_ticket, s_base[3], s_tot[3];
+    __shared__ uint32_t pre_l[kIdxTiles], pre_f[kIdxTiles], cnt_l[kIdxTiles];
+    if (counters[C_STOP]) return;
+    const uint64_t N = counters[C_NB + (gi & 1)];
+    const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
+    const uint64_t nblk = (ntiles + kIdxTiles - 1) / kIdxTiles;
+    int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counters[C_TK_RES] = 0;
-        if (ntiles == 0) counters[C_NL] = counters[C_NF] = 0;
+        if (nblk == 0) {
+            counters[C_NL] = counters[C_NF] = 0;
+            st[0] = static_cast<int64_t>(N);
+        }
     }
-    for (;;) {
-        const uint64_t tk = take_ticket(counters + C_TK_SWEEP, &s_ticket);
-        if (tk >= R + ntiles) break;
-        if (tk < R) {
-            const uint64_t w = rb_lo + tk * kWaves + wave;
-            if (w < nwords) rebuild_word<NW, PAULI>(rows, cols, colstride, n_modes, N, w);
-            __threadfence();
-            __syncthreads();
-            if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_RB_DONE), 1ull);
-            continue;
-        }
-        const uint32_t tile = static_cast<uint32_t>(tk - R);
-        const uint64_t w0 = static_cast<uint64_t>(tile) * kTileWords;
-        const bool after_rb = nrb > 0 && w0 + kTileWords > rb_lo;
-        // ---- A
-        if (wave == 0) {
-            if (after_rb) {
-                uint32_t spins = 0;
-                while (load_agent(counters + C_RB_DONE) < R) {
-                    if (++spins > kSpinLimit) {
-                        if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrRebuild);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            const uint64_t w = w0 + lane;
-            uint64_t a = 0, p = 0;
-            if (w < nwords) {
-                if (after_rb) {
-                    for (int j = 0; j < nfold; ++j) a ^= load_agent(cols + static_cast<uint64_t>(fold[j]) * colstride + w);
-                    if (piv0 >= 0) p = load_agent(cols + static_cast<uint64_t>(piv0) * colstride + w);
-                    if (piv1 >= 0) p ^= load_agent(cols + static_cast<uint64_t>(piv1) * colstride + w);
-                } else {
-                    for (int j = 0; j < nfold; ++j) a ^= cols[static_cast<uint64_t>(fold[j]) * colstride + w];
-                    if (piv0 >= 0) p = cols[static_cast<uint64_t>(piv0) * colstride + w];
-                    if (piv1 >= 0) p ^= cols[static_cast<uint64_t>(piv1) * colstride + w];
-                }
-                if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
-            }
-            sa[lane] = a;
-            sp[lane] = p;
-        }
-        __syncthreads();
-        // ---- B: thread (wave, lane) owns term lane of words it * kWaves + wave
-        double v[kRowsPerThread];
-        uint32_t act = 0, cand = 0, emit = 0;
+    if (blockIdx.x >= nblk) return;
+    if (threadIdx.x == 0)
+        s_ticket = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_IDX), 1ull));
+    __syncthreads();
+    const uint32_t b = s_ticket;
+    const uint64_t tb = static_cast<uint64_t>(b) * kIdxTiles;
+    const uint32_t nt = static_cast<uint32_t>(std::min<uint64_t>(kIdxTiles, ntiles - tb));
+    const uint32_t t0 = threadIdx.x * kIdxTilesPerThread;
+    uint32_t cl[kIdxTilesPerThread], cf[kIdxTilesPerThread];
+    uint32_t sl = 0, sf = 0, sa = 0;
 #pragma unroll
-        for (int it = 0; it < kRowsPerThread; ++it) {
-            const int j = it * kWaves + static_cast<int>(wave);
-            const uint64_t r = (w0 + j) * 64 + lane;
-            v[it] = 0.0;
-            if ((sa[j] >> lane) & 1u) {
-                v[it] = coef[r];
-                act |= 1u << it;
-            }
-        }
+    for (int k = 0; k < kIdxTilesPerThread; ++k) {
+        const uint32_t c = t0 + k < nt ? tcnt[tb + t0 + k] : 0u;
+        cl[k] = c & kCntMask;
+        cf[k] = (c >> kCntBits) & kCntMask;
+        sl += cl[k];
+        sf += cf[k];
+        sa += c >> (2 * kCntBits);
+    }
+    uint32_t agg_l, agg_f, agg_a;
+    uint32_t xl = block_excl_scan(sl, lds, &agg_l);
+    uint32_t xf = block_excl_scan(sf, lds, &agg_f);
+    (void)block_excl_scan(sa, lds, &agg_a);
 #pragma unroll
-        for (int it = 0; it < kRowsPerThread; ++it) {
-            if (!((act >> it) & 1u)) continue;
-            const uint64_t r = (w0 + it * kWaves + wave) * 64 + lane;
-            coef[r] = v[it] * gd.k;
-            if (!(tr.check_lower && gd.abs_s * fabs(v[it]) <= tr.lower)) cand |= 1u << it;
+    for (int k = 0; k < kIdxTilesPerThread; ++k) {
+        pre_l[t0 + k] = xl;
+        pre_f[t0 + k] = xf;
+        cnt_l[t0 + k] = cl[k];
+        xl += cl[k];
+        xf += cf[k];
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if (wave < 3) {
+        uint64_t *desc = wave == 0 ? desc_l : (wave == 1 ? desc_f : desc_a);
+        const uint32_t agg = wave == 0 ? agg_l : (wave == 1 ? agg_f : agg_a);
+        uint32_t excl = 0;
+        if (b == 0) {
+            if (lane_id() == 0) desc_publish(desc, ep, kDescIncl, agg);
+        } else {
+            if (lane_id() == 0) desc_publish(desc + b, ep, kDescAgg, agg);
+            excl = look_back(desc, b, ep, counters);
+            if (lane_id() == 0) desc_publish(desc + b, ep, kDescIncl, excl + agg);
         }
-#pragma unroll
-        for (int it = 0; it < kRowsPerThread; ++it) {
-            const int j = it * kWaves + static_cast<int>(wave);
-            const bool c = (cand >> it) & 1u;
-            bool e = false;
-            if (__ballot(c)) {
-                if (c) {
-                    const uint64_t r = (w0 + j) * 64 + lane;
-                    uint64_t m[NW];
-                    load_row<NW>(rows, static_cast<uint32_t>(r), m);
-#pragma unroll
-                    for (int k = 0; k < NW; ++k) m[k] ^= gd.g[k];
-                    e = mpa::keep_partner<NW, PAULI>(m, tr.cutoff, tr.cutoff_type) ||
-                        (tr.check_upper && gd.abs_s * fabs(v[it]) >= tr.upper);
-                }
-            }
-            const uint64_t em = __ballot(e);
-            if (lane == 0) sem[j] = em;
-            emit |= (e ? 1u : 0u) << it;
+        if (lane_id() == 0) {
+            s_base[wave] = excl;
+            s_tot[wave] = excl + agg;
         }
-        __syncthreads();
-        // ---- C: wave 0 leaders, wave 1 followers
-        if (wave < 2) {
-            const uint64_t em = sem[lane], pw = sp[lane];
-            const uint64_t mine = wave == 0 ? (em & ~pw) : (em & pw);
-            const uint32_t cnt = static_cast<uint32_t>(__popcll(mine));
-            const uint32_t incl = wave_incl_scan(cnt);
-            (wave == 0 ? pre_l : pre_f)[lane] = incl - cnt;
-            const uint32_t agg = __shfl(incl, 63, 64);
-            uint64_t *desc = wave == 0 ? desc_l : desc_f;
-            uint32_t excl = 0;
-            if (tile == 0) {
-                if (lane == 0) desc_publish(desc, ep, kDescIncl, agg);
-            } else {
-                if (lane == 0) desc_publish(desc + tile, ep, kDescAgg, agg);
-                excl = look_back(desc, tile, ep, counters);
-                if (lane == 0) desc_publish(desc + tile, ep, kDescIncl, excl + agg);
-            }
-            if (lane == 0) s_base[wave] = excl;
-            if (wave == 0) {
-                const uint64_t w = w0 + lane;
-                if (w < nwords) elead[w] = mine;
-                const uint32_t na = wave_sum(static_cast<uint32_t>(__popcll(sa[lane])));
-                const uint32_t ne = wave_sum(static_cast<uint32_t>(__popcll(em)));
-                int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
-                if (lane == 0) {
-                    if (na) atomicAdd(reinterpret_cast<unsigned long long *>(st + 1), static_cast<unsigned long long>(na));
-                    if (ne) atomicAdd(reinterpret_cast<unsigned long long *>(st + 2), static_cast<unsigned long long>(ne));
-                    if (tile == ntiles - 1) {
-                        counters[C_NL] = excl + agg;
-                        st[0] = static_cast<int64_t>(N);
-                        st[4] = static_cast<int64_t>(excl + agg);
-                    }
-                }
-            } else if (lane == 0 && tile == ntiles - 1) {
-                counters[C_NF] = excl + agg;
-            }
-        }
-        __syncthreads();
-        // ---- D
-        const uint64_t below = (uint64_t{1} << lane) - 1;
-#pragma unroll
-        for (int it = 0; it < kRowsPerThread; ++it) {
-            if (!((emit >> it) & 1u)) continue;
-            const int j = it * kWaves + static_cast<int>(wave);
-            const uint64_t r = (w0 + j) * 64 + lane;
-            const uint64_t em = sem[j], pw = sp[j];
-            uint64_t slot;
-            if ((pw >> lane) & 1u) {
-                slot = cap_items - 1 - (s_base[1] + pre_f[j] + __popcll(em & pw & below));
-            } else {
-                slot = s_base[0] + pre_l[j] + __popcll(em & ~pw & below);
-            }
-            item[slot] = static_cast<uint32_t>(r);
-            ival[slot] = v[it];
-        }
+    }
+    __syncthreads();
+    if (b == nblk - 1 && threadIdx.x == 0) {
+        counters[C_NL] = s_tot[0];
+        counters[C_NF] = s_tot[1];
+        st[0] = static_cast<int64_t>(N);
+        st[1] = static_cast<int64_t>(s_tot[2]);
+        st[2] = static_cast<int64_t>(s_tot[0]) + static_cast<int64_t>(s_tot[1]);
+        st[4] = static_cast<int64_t>(s_tot[0]);
+    }
+    const uint64_t base_l = s_base[0], base_f = s_base[1];
+    for (uint32_t f = threadIdx.x; f < agg_l; f += kBlock) {
+        const uint32_t t = upper_tile(pre_l, nt, f);
+        map[base_l + f] = static_cast<uint32_t>((tb + t) * kWTileRows + (f - pre_l[t]));
+    }
+    for (uint32_t f = threadIdx.x; f < agg_f; f += kBlock) {
+        const uint32_t t = upper_tile(pre_f, nt, f);
+        map[cap_items - 1 - (base_f + f)] = static_cast<uint32_t>((tb + t) * kWTileRows + cnt_l[t] + (f - pre_f[t]));
     }
 }
 
@@ -1024,58 +1125,65 @@ __device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ 
 // Partner resolution of every emitter (Engine.h:364-387, 465-513) and the ordered append of the
 // partners to create (leader misses then follower misses, ascending, Engine.h:428-444).  One emitter
 // per thread, 256-emitter chunks by work ticket; a chunk's misses land at N + (misses of earlier chunks)
-// by look-back, so the append needs no second pass.  A follower whose partner leader emits is rotated
-// by that leader (the reference's matched-follower skip).  A hit applies the rotation against the swept
-// partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss writes the new
-// term's coefficient k*0 + (s ph) v.  The chunk holding the last emitter publishes N + I for the next
-// gate and stops the device before a gate whose worst case could outgrow the store or the index.
+// by look-back, so the append needs no second pass; appended terms enter the row index, and their
+// column bits are gathered per word in LDS before one OR per word.  A follower whose partner leader emits is rotated by that leader (the
+// reference's matched-follower skip).  A hit applies the rotation against the swept partner value
+// recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss writes the new term's
+// coefficient k*0 + (s ph) v.  The chunk holding the last emitter publishes N + I for the next gate and
+// stops the device before a gate whose worst case could outgrow the store or the index.
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                    const uint32_t *__restrict__ item,
-                                                    const double *__restrict__ ival, uint64_t cap_items,
+                                                    uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
+                                                    const uint32_t *__restrict__ map,
+                                                    const uint32_t *__restrict__ t_item,
+                                                    const double *__restrict__ t_val,
+                                                    const uint64_t *__restrict__ t_key, uint64_t cap_items,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
                                                     uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
                                                     uint64_t cap_rows, uint64_t gidx_slots) {
+    constexpr int NC = max_cols<NW, PAULI>();
+    constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
+    __shared__ uint64_t lcol[kColWords * NC];
     __shared__ uint32_t lds[kWaves];
     __shared__ uint32_t s_ticket, s_base;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[C_TK_SWEEP] = 0;
-        counters[C_RB_DONE] = 0;
-    }
+    const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
-    const int par = gi & 1, nxt = par ^ 1;
-    const uint64_t N = counters[C_NB + 2 * par];
+    const int nxt = (gi & 1) ^ 1;
+    const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t nEL = counters[C_NL], nEF = counters[C_NF];
     const uint64_t n_items = nEL + nEF;
     const uint64_t nchunks = (n_items + kResChunk - 1) / kResChunk;
     int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
-    if (nchunks == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            counters[C_NB + 2 * nxt] = N;
-            counters[C_NB + 2 * nxt + 1] = N;
-        }
-        return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[C_TK_IDX] = 0;
+        if (nchunks == 0) counters[C_NB + nxt] = N;
     }
+    if (blockIdx.x >= nchunks) return;  // only blocks with work take tickets
     for (;;) {
-        const uint64_t tk = take_ticket(counters + C_TK_RES, &s_ticket);
-        if (tk >= nchunks) break;
-        const uint32_t chunk = static_cast<uint32_t>(tk);
+        __syncthreads();  // LDS of the previous chunk fully consumed
+        if (threadIdx.x == 0)
+            s_ticket = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
+        __syncthreads();
+        const uint32_t chunk = s_ticket;
+        if (chunk >= nchunks) break;
         const uint64_t i = static_cast<uint64_t>(chunk) * kResChunk + threadIdx.x;
         uint64_t t[NW];
         double ins = 0.0;
         uint32_t miss = 0, hit = 0;
         if (i < n_items) {
             const bool leader = i < nEL;
-            const uint64_t slot = leader ? i : cap_items - 1 - (i - nEL);
-            const uint32_t S = item[slot];
-            const double vS = ival[slot];
+            const uint64_t slot = map[leader ? i : cap_items - 1 - (i - nEL)];
+            const uint32_t S = t_item[slot];
+            const double vS = t_val[slot];
             uint64_t m[NW];
-            load_row<NW>(rows, S, m);
 #pragma unroll
-            for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+            for (int k = 0; k < NW; ++k) {
+                t[k] = t_key[slot * NW + k];
+                m[k] = t[k] ^ gd.g[k];
+            }
             const int64_t T = gidx_find_below<NW>(gidx, gmask, rows, t, N);
             const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
             if (!skip) {
@@ -1094,6 +1202,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                 }
             }
         }
+        for (int k = threadIdx.x; k < kColWords * NC; k += kBlock) lcol[k] = 0;
         uint32_t n_miss;
         const uint32_t rank = block_excl_scan(miss, lds, &n_miss);
         const uint32_t n_hit = block_sum(hit, lds);
@@ -1112,21 +1221,33 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                 if (chunk == nchunks - 1) {
                     const uint64_t I = excl + n_miss, n = N + I;
                     st[3] = static_cast<int64_t>(I);
-                    counters[C_NB + 2 * nxt] = n;
-                    counters[C_NB + 2 * nxt + 1] = N;
+                    counters[C_NB + nxt] = n;
                     if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
                 }
             }
         }
         __syncthreads();
+        const uint64_t first_row = N + s_base, wlo = first_row >> 6;
         if (miss) {
-            const uint64_t dest = N + s_base + rank;
+            const uint64_t dest = first_row + rank;
             uint64_t *pd = rows + dest * NW;
 #pragma unroll
             for (int k = 0; k < NW; ++k) pd[k] = t[k];
             coef[dest] = ins;
+            gather_column_bits<NW, PAULI>(lcol, ncols, n_modes, t, (dest >> 6) - wlo, uint64_t{1} << (dest & 63));
             if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest)))
                 atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
+        }
+        __syncthreads();
+        if (n_miss) {
+            const int nwt = static_cast<int>(((first_row + n_miss - 1) >> 6) - wlo + 1);
+            for (int k = threadIdx.x; k < nwt * ncols; k += kBlock) {
+                const uint64_t bits = lcol[k];
+                if (bits)
+                    atomicOr(reinterpret_cast<unsigned long long *>(cols + static_cast<uint64_t>(k % ncols) * colstride +
+                                                                    wlo + k / ncols),
+                             bits);
+            }
         }
     }
 }
@@ -1135,11 +1256,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
 __global__ void k_advance_fused(uint64_t *__restrict__ counters, int64_t *__restrict__ stats, int gi) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (counters[C_STOP]) return;
-    const int par = gi & 1, nxt = par ^ 1;
     int64_t *s = stats + static_cast<int64_t>(gi) * kStatW;
-    s[0] = static_cast<int64_t>(counters[C_NB + 2 * par]);
-    counters[C_NB + 2 * nxt] = counters[C_NB + 2 * par];
-    counters[C_NB + 2 * nxt + 1] = counters[C_NB + 2 * par + 1];
+    s[0] = static_cast<int64_t>(counters[C_NB + (gi & 1)]);
+    counters[C_NB + ((gi & 1) ^ 1)] = counters[C_NB + (gi & 1)];
 }
 
 
@@ -1323,7 +1442,7 @@ struct mpa_engine {
 
     // Fused pipeline state: global row index, the emitter list with pre-gate values (cap / 2 entries:
     // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
-    DevArray gidx, f_item, f_ival, f_elead, desc_l, desc_f, desc_m;
+    DevArray gidx, f_map, f_elead, t_item, t_val, t_key, t_cnt, desc_l, desc_f, desc_a, desc_m;
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
@@ -1754,33 +1873,35 @@ struct mpa_engine {
         for (size_t l = nl; l-- > 0;) grad[layers[l].param_index] += layers[l].gen_coeff * dl[l];
     }
 
+    // Fresh descriptor memory reads as epoch 0, which no gate uses.
+    void clear_descriptors() {
+        for (DevArray *d : {&desc_l, &desc_f, &desc_a, &desc_m}) HIP_TRY(hipMemsetAsync(d->p, 0, d->bytes, stream));
+        desc_epoch = 0;
+    }
+
     // Arrays of the fused pipeline sized to the store (a gate runs only while N <= cap / 2), and a global
     // index with >= 2 slots per row of capacity covering every stored row.
     template <int NW>
     void ensure_fused_arrays() {
         if (fused_cap != cap) {
             cap_items = cap / 2;
-            f_item.reserve(cap_items * 4);
-            f_ival.reserve(cap_items * 8);
+            const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
+            f_map.reserve(cap_items * 4);
+            t_item.reserve(tiles * kWTileRows * 4);
+            t_val.reserve(tiles * kWTileRows * 8);
+            t_key.reserve(tiles * kWTileRows * nw * 8);
+            t_cnt.reserve(tiles * 4);
             f_elead.reserve((cap / 64) * 8);
-            const uint64_t tiles = cap_items / (kTileWords * 64) + 2;
+            const uint64_t iblocks = tiles / kIdxTiles + 2;
             const uint64_t chunks = cap_items / kResChunk + 2;
-            desc_l.reserve(tiles * 8);
-            desc_f.reserve(tiles * 8);
+            desc_l.reserve(iblocks * 8);
+            desc_f.reserve(iblocks * 8);
+            desc_a.reserve(iblocks * 8);
             desc_m.reserve(chunks * 8);
-            // fresh descriptor memory reads as epoch 0, which no gate uses
-            HIP_TRY(hipMemsetAsync(desc_l.p, 0, desc_l.bytes, stream));
-            HIP_TRY(hipMemsetAsync(desc_f.p, 0, desc_f.bytes, stream));
-            HIP_TRY(hipMemsetAsync(desc_m.p, 0, desc_m.bytes, stream));
-            desc_epoch = 0;
+            clear_descriptors();
             fused_cap = cap;
         }
-        if (desc_epoch >= (1u << 30) - kChunk - 1) {
-            HIP_TRY(hipMemsetAsync(desc_l.p, 0, desc_l.bytes, stream));
-            HIP_TRY(hipMemsetAsync(desc_f.p, 0, desc_f.bytes, stream));
-            HIP_TRY(hipMemsetAsync(desc_m.p, 0, desc_m.bytes, stream));
-            desc_epoch = 0;
-        }
+        if (desc_epoch >= (1u << 30) - kChunk - 1) clear_descriptors();
         uint64_t want = 1024;
         while (want < 2 * cap) want <<= 1;
         if (want > gidx_slots) {
@@ -1800,37 +1921,34 @@ struct mpa_engine {
         }
     }
 
-    // Algorithmic bytes of the fused kernels from the per-gate counters: fold + pivot column words, the
-    // anticommuting coefficients (read + write), the emitters' rows and list entries, leader masks; the
-    // column rebuild of the previous gate's appended rows; per emitter its list entry, row, index slot,
-    // candidate row, leader-mask word and coefficients; per appended row the row, coefficient and slot.
+    // Algorithmic bytes of the fused kernels from the per-gate counters.  Sweep: fold + pivot column
+    // words, leader masks, tile counts, the anticommuting coefficients (read + write), per emitter its
+    // row and its tile entry (row, value, key).  Index: tile counts, one map entry per emitter.
+    // Resolve: per emitter its map entry, tile entry, index slot, partner row and coefficient writes; per hit the partner
+    // coefficient (read + write); per appended row the row, coefficient, index slot and column bits.
     void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
-        const double NW = nw;
-        double prev_inserted = 0.0;
+        const double NW = nw, entry = 12 + 8 * NW;
         for (size_t g = 0; g < stat_n.size(); ++g) {
             if (fold_plus_piv[g] == 0) continue;
             const double words = std::ceil(static_cast<double>(stat_n[g]) / 64.0);
+            const double tiles = std::ceil(words / kWTileWords);
             const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], hits = stat_p[g];
-            const double rb_words = prev_inserted > 0 ? std::ceil(prev_inserted / 64.0) + 1 : 0.0;
-            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + nA * 16.0 + E * (8 * NW + 12) +
-                                   rb_words * (64 * 8 * NW + ncols * 8.0);
-            prof_bytes[K_RESOLVE] += E * (12 + 8 * NW + 8 + 8 * NW + 8 + 8) + hits * 16 + I * (8 * NW + 8 + 8);
-            prev_inserted = I;
+            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + tiles * 4 + nA * 16.0 + E * (8 * NW + entry);
+            prof_bytes[K_INDEX] += tiles * 4 + E * 4;
+            prof_bytes[K_RESOLVE] += E * (4 + entry + 8 + 8 * NW + 8) + hits * 16 + I * (8 * NW + 8 + 8 + ncols / 8.0);
         }
     }
 
-    // Per gate parity {N, N_cols} and the work tickets, as the device pipeline expects them before gate
-    // `g`: every column bit current, nothing stopped.
+    // Live count of gate `g`'s parity and the work tickets, as the device pipeline expects them before
+    // gate `g`; nothing stopped.
     void reset_fused_counters(size_t g) {
         uint64_t h[C_COUNT];
         HIP_TRY(hipMemcpyAsync(h, counters.p, C_COUNT * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        const int par = static_cast<int>(g & 1);
-        h[C_NB + 2 * par] = N;
-        h[C_NB + 2 * par + 1] = N;
+        h[C_NB + (g & 1)] = N;
         h[C_N] = N;
         h[C_STOP] = 0;
-        h[C_TK_SWEEP] = h[C_TK_RES] = h[C_RB_DONE] = 0;
+        h[C_TK_IDX] = h[C_TK_RES] = 0;
         HIP_TRY(hipMemcpyAsync(counters.p, h, C_COUNT * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
     }
@@ -1861,8 +1979,9 @@ struct mpa_engine {
             if (2 * N > cap) ensure_capacity(std::max<uint64_t>(4 * N, 64ull * kBlock));
             ensure_fused_arrays<NW>();
             const size_t end = std::min(ng, gi + kChunk);
-            const uint64_t tiles_ub = (cap_items + kTileWords * 64 - 1) / (kTileWords * 64);
-            const uint32_t sgrid = static_cast<uint32_t>(std::clamp<uint64_t>(tiles_ub, 1, kMaxGrid));
+            const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
+            const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
+            const uint32_t igrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kIdxTiles - 1) / kIdxTiles, 1));
             const uint32_t rgrid =
                 static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, kMaxGrid));
             const uint64_t gmask = gidx_slots - 1;
@@ -1880,20 +1999,26 @@ struct mpa_engine {
                 }
                 const uint32_t ep = ++desc_epoch;
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                   ig, ep, tr, cols.as<uint64_t>(), colstride(), n_modes,
-                                   fold.as<int32_t>() + gd.fold_off, gd.fold_cnt, gd.piv0, gd.piv1,
-                                   rows.as<uint64_t>(), coef.as<double>(), f_item.as<uint32_t>(),
-                                   f_ival.as<double>(), cap_items, f_elead.as<uint64_t>(), desc_l.as<uint64_t>(),
-                                   desc_f.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>());
+                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), ig,
+                                   tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
+                                   gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), t_item.as<uint32_t>(),
+                                   t_val.as<double>(), t_key.as<uint64_t>(), f_elead.as<uint64_t>(),
+                                   t_cnt.as<uint32_t>(), counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), 0, stream,
-                                   gdesc.as<GateDesc>(), ig, ep, rows.as<uint64_t>(), coef.as<double>(),
-                                   f_item.as<uint32_t>(), f_ival.as<double>(), cap_items, f_elead.as<uint64_t>(),
-                                   gidx.as<uint64_t>(), gmask, desc_m.as<uint64_t>(), counters.as<uint64_t>(),
-                                   stats.as<int64_t>(), cap, gidx_slots);
+                hipLaunchKernelGGL(k_index, dim3(igrid), dim3(kBlock), 0, stream, ig, ep, t_cnt.as<uint32_t>(),
+                                   f_map.as<uint32_t>(), cap_items, desc_l.as<uint64_t>(), desc_f.as<uint64_t>(),
+                                   desc_a.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>());
+                HIP_TRY(hipGetLastError());
+                prof_end(K_INDEX, pa);
+                pa = prof_begin();
+                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                                   ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
+                                   n_modes, f_map.as<uint32_t>(), t_item.as<uint32_t>(), t_val.as<double>(),
+                                   t_key.as<uint64_t>(), cap_items,
+                                   f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask, desc_m.as<uint64_t>(),
+                                   counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
             }
@@ -1906,15 +2031,7 @@ struct mpa_engine {
             }
             const uint64_t stop = static_cast<uint64_t>(h_counters[C_STOP]);
             const size_t next = stop ? static_cast<size_t>(stop - 1) : end;
-            const int par = static_cast<int>(next & 1);
-            N = static_cast<uint64_t>(h_counters[C_NB + 2 * par]);
-            const uint64_t ncols_done = static_cast<uint64_t>(h_counters[C_NB + 2 * par + 1]);
-            if (ncols_done < N) {
-                hipEvent_t pa = prof_begin();
-                launch_cols_update<NW, PAULI>(ncols_done, N - ncols_done, false, N - ncols_done);
-                prof_end(K_COLS, pa);
-                if (prof_on) prof_bytes[K_COLS] += (N - ncols_done) * (8.0 * nw + ncols / 8.0);
-            }
+            N = static_cast<uint64_t>(h_counters[C_NB + (next & 1)]);
             gidx_rows = N;
             if (N >= (uint64_t{1} << 31)) throw RunError("term count exceeds the 32-bit row index");
             reset_fused_counters(next);
