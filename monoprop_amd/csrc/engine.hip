@@ -817,79 +817,101 @@ __device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, in
     }
 }
 
-constexpr int kWTileWords = 8;                           // sweep: one wave per 8 column words
-constexpr int kWTileRows = kWTileWords * 64;              // = 512 terms
+constexpr int kWTileWords = 16;                          // sweep: one wavefront per 16 column words
+constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
 constexpr int kIdxTilesPerThread = 4;
 constexpr int kIdxTiles = kBlock * kIdxTilesPerThread;    // index: 1024 sweep tiles per block
+constexpr uint32_t kOwnWin = 8192;                        // index: list positions per LDS window
 constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
 constexpr uint32_t kMaxGrid = 1024;                       // resolve blocks that take chunks
-// per-tile count word: leaders | followers << 10 | anticommuting << 20 (each <= 512)
-constexpr uint32_t kCntBits = 10, kCntMask = (1u << kCntBits) - 1;
+// per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
+constexpr uint32_t kCntBits = 16;
+constexpr uint64_t kCntMask = (uint64_t{1} << kCntBits) - 1;
 
-// Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate.  One wave per
-// 512-term tile, no dependence between tiles:
-//   A  G's fold and pivot columns of the tile's 8 words, one load per lane (lane = column * 8 + word),
+// Wave-level LDS hand-off between lanes of one wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate.  One wavefront
+// (one 64-thread block) per 1024-term tile, no dependence between tiles:
+//   A  G's fold and pivot columns of the tile's 16 words, all loads in flight (lane = slot * 16 + word),
 //      XOR-reduced across lanes -> anticommuting mask a_j, pivot mask p_j
-//   B  c <- c k on every anticommuting term (the reference's in-place sweep; 8 loads in flight per lane);
-//      terms past the lower_atol gate go to a per-wave candidate list in LDS with their pre-gate value
-//   E  candidates, one per lane: partner key m ^ G against the structural cutoff (or the upper_atol
-//      rescue) -> emit bits
-//   D  the tile's emitters with pre-gate value and partner key into the tile's own region
-//      [512 tile, 512 tile + n): leaders ascending, then followers ascending (pivot clear / set);
-//      the tile's counts; leader emit masks per word for the followers' skip test
+//   B  c <- c k on every anticommuting term (the reference's in-place sweep); lane b owns bit b of every
+//      word, so at most 16 loads per lane, all in flight.  Terms past the lower_atol gate keep their
+//      pre-gate value in vsrc[row] and go to the tile's candidate list (LDS)
+//   E  candidates dense over the lanes: partner key m ^ G against the structural cutoff (or the
+//      upper_atol rescue) -> emit bit (LDS)
+//   D  lane j lists word j's emitters into the tile's region [1024 tile, ...): leaders ascending, then
+//      followers ascending (pivot clear / set); the tile's counts; leader emit masks per word
 template <int NW, int PAULI>
-__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
-                                                  const uint64_t *__restrict__ cols, uint64_t colstride,
-                                                  const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
-                                                  const uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                  uint32_t *__restrict__ t_item, double *__restrict__ t_val,
-                                                  uint64_t *__restrict__ t_key, uint64_t *__restrict__ elead,
-                                                  uint32_t *__restrict__ tcnt, const uint64_t *__restrict__ counters) {
-    __shared__ uint16_t cidx[kWaves][kWTileRows];
-    __shared__ double cval[kWaves][kWTileRows];
-    __shared__ uint64_t sem[kWaves][kWTileWords], s_pw[kWaves][kWTileWords];
-    __shared__ uint2 s_pre[kWaves][kWTileWords];
+__global__ __launch_bounds__(64) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
+                                              const uint64_t *__restrict__ cols, uint64_t colstride,
+                                              const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
+                                              const uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                              double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
+                                              uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
+                                              const uint64_t *__restrict__ counters) {
+    __shared__ uint64_t sem[kWTileWords];
+    __shared__ uint16_t cidx[kWTileRows];
     if (counters[C_STOP]) return;
     const GateDesc &gd = gdesc[gi];
     const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t nwords = (N + 63) / 64;
-    const uint64_t ntiles = (nwords + kWTileWords - 1) / kWTileWords;
-    if (static_cast<uint64_t>(blockIdx.x) * kWaves >= ntiles) return;  // whole block idle
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
-    const bool live = tile < ntiles;  // idle waves of the last block still take part in the barriers
+    const uint64_t tile = blockIdx.x;
+    const uint64_t w0 = tile * kWTileWords;
+    if (w0 >= nwords) return;
+    const uint32_t lane = lane_id();
     const uint64_t row0 = tile * kWTileRows;
-    // ---- A
+    // ---- A: lanes hold (column slot s = lane / 16, word j = lane % 16)
     uint64_t a = 0, p = 0;
     {
-        const int j = static_cast<int>(lane & 7), c = static_cast<int>(lane >> 3);
-        const uint64_t w = tile * kWTileWords + j;
+        const int j = static_cast<int>(lane & (kWTileWords - 1)), sl = static_cast<int>(lane / kWTileWords);
+        const uint64_t w = w0 + j;
         const int npiv = (piv0 >= 0) + (piv1 >= 0);
         const int ncol = nfold + npiv;
-        for (int c0 = 0; c0 < ncol; c0 += 8) {
-            const int cc = c0 + c;
-            uint64_t x = 0;
-            if (live && w < nwords && cc < ncol) {
+        constexpr int kSlots = 64 / kWTileWords;  // 4 column slots per pass
+        uint64_t x[2];
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            const int cc = pass * kSlots + sl;
+            x[pass] = 0;
+            if (w < nwords && cc < ncol) {
                 const int col = cc < nfold ? fold[cc] : (cc == nfold ? piv0 : piv1);
-                x = cols[static_cast<uint64_t>(col) * colstride + w];
+                x[pass] = cols[static_cast<uint64_t>(col) * colstride + w];
             }
-            if (cc < nfold) a ^= x;
-            else p ^= x;
         }
 #pragma unroll
-        for (int d = 8; d < 64; d <<= 1) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const int cc = pass * kSlots + sl;
+            if (cc < nfold) a ^= x[pass];
+            else p ^= x[pass];
+        }
+        for (int cc0 = 2 * kSlots; cc0 < ncol; cc0 += kSlots) {  // generators wider than 8 columns
+            const int cc = cc0 + sl;
+            uint64_t y = 0;
+            if (w < nwords && cc < ncol) {
+                const int col = cc < nfold ? fold[cc] : (cc == nfold ? piv0 : piv1);
+                y = cols[static_cast<uint64_t>(col) * colstride + w];
+            }
+            if (cc < nfold) a ^= y;
+            else p ^= y;
+        }
+#pragma unroll
+        for (int d = kWTileWords; d < 64; d <<= 1) {
             a ^= __shfl_xor(a, d, 64);
             p ^= __shfl_xor(p, d, 64);
         }
         if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
-        if (!live || w >= nwords) a = 0;
-        if (lane < kWTileWords) sem[wave][lane] = 0;
+        if (w >= nwords) a = 0;
     }
-    // lanes j < 8 hold word j's masks
-    // ---- B
-    double v[kWTileWords];
-    uint32_t cand = 0;
+    if (lane < kWTileWords) sem[lane] = 0;
+    // ---- B: lane owns bit `lane` of each word
+    uint32_t cand = 0, resc = 0;  // candidates; those the upper_atol rescue emits regardless
     {
+        double v[kWTileWords];
         uint32_t act = 0;
 #pragma unroll
         for (int j = 0; j < kWTileWords; ++j) {
@@ -903,8 +925,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
 #pragma unroll
         for (int j = 0; j < kWTileWords; ++j) {
             if (!((act >> j) & 1u)) continue;
-            coef[row0 + static_cast<uint64_t>(j) * 64 + lane] = v[j] * gd.k;
-            if (!(tr.check_lower && gd.abs_s * fabs(v[j]) <= tr.lower)) cand |= 1u << j;
+            const uint64_t r = row0 + static_cast<uint64_t>(j) * 64 + lane;
+            coef[r] = v[j] * gd.k;
+            if (!(tr.check_lower && gd.abs_s * fabs(v[j]) <= tr.lower)) {
+                cand |= 1u << j;
+                vsrc[r] = v[j];
+                if (tr.check_upper && gd.abs_s * fabs(v[j]) >= tr.upper) resc |= 1u << j;
+            }
         }
     }
     const uint32_t mine = static_cast<uint32_t>(__popc(cand));
@@ -913,103 +940,62 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
     {
         uint32_t pos = incl - mine;
 #pragma unroll
-        for (int j = 0; j < kWTileWords; ++j) {
-            if (!((cand >> j) & 1u)) continue;
-            cidx[wave][pos] = static_cast<uint16_t>(j * 64 + lane);
-            cval[wave][pos] = v[j];
-            ++pos;
+        for (int j = 0; j < kWTileWords; ++j)
+            if ((cand >> j) & 1u) cidx[pos++] = static_cast<uint16_t>(j * 64 + lane) | (((resc >> j) & 1u) << 15);
+    }
+    wave_lds_sync();
+    // ---- E
+    for (uint32_t c0 = 0; c0 < ncand; c0 += 128) {
+        uint64_t m[2][NW];
+        uint32_t id[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t c = c0 + q * 64 + lane;
+            id[q] = c < ncand ? cidx[c] : 0xffffu;
+            if (id[q] < 0x8000u) load_row<NW>(rows, static_cast<uint32_t>(row0 + id[q]), m[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (id[q] == 0xffffu) continue;
+            bool e = true;  // rescued
+            if (id[q] < 0x8000u) {
+#pragma unroll
+                for (int k = 0; k < NW; ++k) m[q][k] ^= gd.g[k];
+                e = mpa::keep_partner<NW, PAULI>(m[q], tr.cutoff, tr.cutoff_type);
+            }
+            const uint32_t i = id[q] & 0x7fffu;
+            if (e) atomicOr(reinterpret_cast<unsigned long long *>(&sem[i >> 6]), uint64_t{1} << (i & 63));
         }
     }
-    __syncthreads();
-    // ---- E: keys of the first two rounds stay in registers for D
-    uint64_t key0[NW], key1[NW];
-#pragma unroll
-    for (int round = 0; round < 2; ++round) {
-        const uint32_t c = lane + round * 64;
-        if (c < ncand) {
-            const uint32_t idx = cidx[wave][c];
-            uint64_t *key = round == 0 ? key0 : key1;
-            load_row<NW>(rows, static_cast<uint32_t>(row0 + idx), key);
-#pragma unroll
-            for (int k = 0; k < NW; ++k) key[k] ^= gd.g[k];
-            if (mpa::keep_partner<NW, PAULI>(key, tr.cutoff, tr.cutoff_type) ||
-                (tr.check_upper && gd.abs_s * fabs(cval[wave][c]) >= tr.upper))
-                atomicOr(reinterpret_cast<unsigned long long *>(&sem[wave][idx >> 6]), uint64_t{1} << (idx & 63));
-        }
-    }
-    for (uint32_t c = lane + 128; c < ncand; c += 64) {
-        const uint32_t idx = cidx[wave][c];
-        uint64_t key[NW];
-        load_row<NW>(rows, static_cast<uint32_t>(row0 + idx), key);
-#pragma unroll
-        for (int k = 0; k < NW; ++k) key[k] ^= gd.g[k];
-        if (mpa::keep_partner<NW, PAULI>(key, tr.cutoff, tr.cutoff_type) ||
-            (tr.check_upper && gd.abs_s * fabs(cval[wave][c]) >= tr.upper))
-            atomicOr(reinterpret_cast<unsigned long long *>(&sem[wave][idx >> 6]), uint64_t{1} << (idx & 63));
-    }
-    __syncthreads();
-    // ---- counts: lane j < 8 owns word j
-    const uint64_t em_mine = lane < kWTileWords ? sem[wave][lane] : 0;
-    const uint64_t sel = em_mine & ~p, sef = em_mine & p;
-    const uint32_t cl = lane < kWTileWords ? static_cast<uint32_t>(__popcll(sel)) : 0u;
-    const uint32_t cf = lane < kWTileWords ? static_cast<uint32_t>(__popcll(sef)) : 0u;
+    wave_lds_sync();
+    // ---- D: lane j < 16 owns word j
+    const uint64_t em = lane < kWTileWords ? sem[lane] : 0;
+    const uint64_t sel = em & ~p, sef = em & p;
+    const uint32_t cl = static_cast<uint32_t>(__popcll(sel)), cf = static_cast<uint32_t>(__popcll(sef));
     const uint32_t il = wave_incl_scan(cl), iff = wave_incl_scan(cf);
     const uint32_t nl = __shfl(il, 63, 64), nf = __shfl(iff, 63, 64);
     const uint32_t na = wave_sum(lane < kWTileWords ? static_cast<uint32_t>(__popcll(a)) : 0u);
-    if (lane < kWTileWords) {
-        s_pw[wave][lane] = p;
-        s_pre[wave][lane] = make_uint2(il - cl, iff - cf);
-        const uint64_t w = tile * kWTileWords + lane;
-        if (live && w < nwords) elead[w] = sel;
-    }
-    if (live && lane == 0) tcnt[tile] = nl | (nf << kCntBits) | (na << (2 * kCntBits));
-    __syncthreads();
-    if (!live) return;
-    // ---- D
-    for (uint32_t c = lane; c < ncand; c += 64) {
-        const uint32_t idx = cidx[wave][c];
-        const int j = static_cast<int>(idx >> 6);
-        const uint32_t b = idx & 63;
-        const uint64_t em = sem[wave][j], pw = s_pw[wave][j];
-        const uint32_t pl = s_pre[wave][j].x, pf = s_pre[wave][j].y;
-        if (!((em >> b) & 1u)) continue;
-        const uint64_t bl = (uint64_t{1} << b) - 1;
-        const uint32_t rank = ((pw >> b) & 1u) ? nl + pf + static_cast<uint32_t>(__popcll(em & pw & bl))
-                                               : pl + static_cast<uint32_t>(__popcll(em & ~pw & bl));
-        const uint64_t slot = row0 + rank;
-        t_item[slot] = static_cast<uint32_t>(row0 + idx);
-        t_val[slot] = cval[wave][c];
-        uint64_t *pk = t_key + slot * NW;
-        if (c < 128) {
-            const bool first = c < 64;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) pk[k] = first ? key0[k] : key1[k];
-        } else {
-            const uint64_t *pr = rows + (row0 + idx) * NW;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) pk[k] = pr[k] ^ gd.g[k];
-        }
-    }
+    if (lane < kWTileWords && w0 + lane < nwords) elead[w0 + lane] = sel;
+    if (lane == 0)
+        tcnt[tile] = static_cast<uint64_t>(nl) | (static_cast<uint64_t>(nf) << kCntBits) |
+                     (static_cast<uint64_t>(na) << (2 * kCntBits));
+    uint32_t *region = t_item + tile * kWTileRows;
+    uint32_t k = il - cl;
+    for (uint64_t bits = sel; bits; bits &= bits - 1)
+        region[k++] = static_cast<uint32_t>(row0 + lane * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
+    k = nl + iff - cf;
+    for (uint64_t bits = sef; bits; bits &= bits - 1)
+        region[k++] = static_cast<uint32_t>(row0 + lane * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
 }
 
 // Ordered emitter lists from the sweep's tiles: exclusive prefixes of the per-tile counts across the
 // launch (ticket-ordered blocks, decoupled look-back) turn into a map from list position to tile-region
 // slot: leaders at MAP[0, nEL), followers at MAP[cap - 1 - k], each ascending.  The map is written
-// with no loads, spread evenly over the block whatever the tiles' emitter counts (a binary search
-// over the block's tile prefixes in LDS).  The block holding the last tile publishes the totals and
+// with no global loads and coalesced, whatever the tiles' emitter counts (positions are assigned to
+// their tiles through an LDS window first).  The block holding the last tile publishes the totals and
 // the gate's statistics.
-__device__ __forceinline__ uint32_t upper_tile(const uint32_t *pre, uint32_t n, uint32_t f) {
-    // largest t < n with pre[t] <= f (pre ascending, pre[0] = 0)
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= f) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
-__global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uint32_t *__restrict__ tcnt,
+__global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uint64_t *__restrict__ tcnt,
                                                   uint32_t *__restrict__ map, uint64_t cap_items,
                                                   uint64_t *__restrict__ desc_l, uint64_t *__restrict__ desc_f,
                                                   uint64_t *__restrict__ desc_a, uint64_t *__restrict__ counters,
@@ -1017,6 +1003,7 @@ __global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uin
     __shared__ uint32_t lds[kWaves];
     __shared__ uint32_t s_ticket, s_base[3], s_tot[3];
     __shared__ uint32_t pre_l[kIdxTiles], pre_f[kIdxTiles], cnt_l[kIdxTiles];
+    __shared__ uint16_t owner[kOwnWin];
     if (counters[C_STOP]) return;
     const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
@@ -1041,12 +1028,12 @@ __global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uin
     uint32_t sl = 0, sf = 0, sa = 0;
 #pragma unroll
     for (int k = 0; k < kIdxTilesPerThread; ++k) {
-        const uint32_t c = t0 + k < nt ? tcnt[tb + t0 + k] : 0u;
-        cl[k] = c & kCntMask;
-        cf[k] = (c >> kCntBits) & kCntMask;
+        const uint64_t c = t0 + k < nt ? tcnt[tb + t0 + k] : 0u;
+        cl[k] = static_cast<uint32_t>(c & kCntMask);
+        cf[k] = static_cast<uint32_t>((c >> kCntBits) & kCntMask);
         sl += cl[k];
         sf += cf[k];
-        sa += c >> (2 * kCntBits);
+        sa += static_cast<uint32_t>(c >> (2 * kCntBits));
     }
     uint32_t agg_l, agg_f, agg_a;
     uint32_t xl = block_excl_scan(sl, lds, &agg_l);
@@ -1086,14 +1073,34 @@ __global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uin
         st[2] = static_cast<int64_t>(s_tot[0]) + static_cast<int64_t>(s_tot[1]);
         st[4] = static_cast<int64_t>(s_tot[0]);
     }
+    // map entries in windows of kOwnWin positions: each thread marks the positions of its own tiles in
+    // LDS (owner), then the block writes the window's map entries coalesced
     const uint64_t base_l = s_base[0], base_f = s_base[1];
-    for (uint32_t f = threadIdx.x; f < agg_l; f += kBlock) {
-        const uint32_t t = upper_tile(pre_l, nt, f);
-        map[base_l + f] = static_cast<uint32_t>((tb + t) * kWTileRows + (f - pre_l[t]));
-    }
-    for (uint32_t f = threadIdx.x; f < agg_f; f += kBlock) {
-        const uint32_t t = upper_tile(pre_f, nt, f);
-        map[cap_items - 1 - (base_f + f)] = static_cast<uint32_t>((tb + t) * kWTileRows + cnt_l[t] + (f - pre_f[t]));
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t agg = side == 0 ? agg_l : agg_f;
+        const uint32_t *pre = side == 0 ? pre_l : pre_f;
+        for (uint32_t W0 = 0; W0 < agg; W0 += kOwnWin) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kIdxTilesPerThread; ++k) {
+                const uint32_t t = t0 + k;
+                if (t >= nt) break;
+                const uint32_t n = side == 0 ? cl[k] : cf[k];
+                const uint32_t lo = std::max(pre[t], W0), hi = std::min(pre[t] + n, W0 + kOwnWin);
+                for (uint32_t f = lo; f < hi; ++f) owner[f - W0] = static_cast<uint16_t>(t);
+            }
+            __syncthreads();
+            const uint32_t wend = std::min(agg, W0 + kOwnWin);
+            for (uint32_t f = W0 + threadIdx.x; f < wend; f += kBlock) {
+                const uint32_t t = owner[f - W0];
+                if (side == 0) {
+                    map[base_l + f] = static_cast<uint32_t>((tb + t) * kWTileRows + (f - pre[t]));
+                } else {
+                    map[cap_items - 1 - (base_f + f)] =
+                        static_cast<uint32_t>((tb + t) * kWTileRows + cnt_l[t] + (f - pre[t]));
+                }
+            }
+        }
     }
 }
 
@@ -1137,8 +1144,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
                                                     const uint32_t *__restrict__ map,
                                                     const uint32_t *__restrict__ t_item,
-                                                    const double *__restrict__ t_val,
-                                                    const uint64_t *__restrict__ t_key, uint64_t cap_items,
+                                                    const double *__restrict__ vsrc, uint64_t cap_items,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
                                                     uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
@@ -1175,15 +1181,12 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
         uint32_t miss = 0, hit = 0;
         if (i < n_items) {
             const bool leader = i < nEL;
-            const uint64_t slot = map[leader ? i : cap_items - 1 - (i - nEL)];
-            const uint32_t S = t_item[slot];
-            const double vS = t_val[slot];
+            const uint32_t S = t_item[map[leader ? i : cap_items - 1 - (i - nEL)]];
+            const double vS = vsrc[S];
             uint64_t m[NW];
+            load_row<NW>(rows, S, m);
 #pragma unroll
-            for (int k = 0; k < NW; ++k) {
-                t[k] = t_key[slot * NW + k];
-                m[k] = t[k] ^ gd.g[k];
-            }
+            for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
             const int64_t T = gidx_find_below<NW>(gidx, gmask, rows, t, N);
             const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
             if (!skip) {
@@ -1442,7 +1445,7 @@ struct mpa_engine {
 
     // Fused pipeline state: global row index, the emitter list with pre-gate values (cap / 2 entries:
     // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
-    DevArray gidx, f_map, f_elead, t_item, t_val, t_key, t_cnt, desc_l, desc_f, desc_a, desc_m;
+    DevArray gidx, f_map, f_elead, f_vsrc, t_item, t_cnt, desc_l, desc_f, desc_a, desc_m;
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
@@ -1888,9 +1891,8 @@ struct mpa_engine {
             const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
             f_map.reserve(cap_items * 4);
             t_item.reserve(tiles * kWTileRows * 4);
-            t_val.reserve(tiles * kWTileRows * 8);
-            t_key.reserve(tiles * kWTileRows * nw * 8);
-            t_cnt.reserve(tiles * 4);
+            t_cnt.reserve(tiles * 8);
+            f_vsrc.reserve(cap * 8);
             f_elead.reserve((cap / 64) * 8);
             const uint64_t iblocks = tiles / kIdxTiles + 2;
             const uint64_t chunks = cap_items / kResChunk + 2;
@@ -1923,19 +1925,19 @@ struct mpa_engine {
 
     // Algorithmic bytes of the fused kernels from the per-gate counters.  Sweep: fold + pivot column
     // words, leader masks, tile counts, the anticommuting coefficients (read + write), per emitter its
-    // row and its tile entry (row, value, key).  Index: tile counts, one map entry per emitter.
-    // Resolve: per emitter its map entry, tile entry, index slot, partner row and coefficient writes; per hit the partner
+    // row, pre-gate value and tile entry.  Index: tile counts, one map entry per emitter.  Resolve: per
+    // emitter its map and tile entries, pre-gate value, row, index slot, partner row and coefficient writes; per hit the partner
     // coefficient (read + write); per appended row the row, coefficient, index slot and column bits.
     void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
-        const double NW = nw, entry = 12 + 8 * NW;
+        const double NW = nw;
         for (size_t g = 0; g < stat_n.size(); ++g) {
             if (fold_plus_piv[g] == 0) continue;
             const double words = std::ceil(static_cast<double>(stat_n[g]) / 64.0);
             const double tiles = std::ceil(words / kWTileWords);
             const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], hits = stat_p[g];
-            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + tiles * 4 + nA * 16.0 + E * (8 * NW + entry);
-            prof_bytes[K_INDEX] += tiles * 4 + E * 4;
-            prof_bytes[K_RESOLVE] += E * (4 + entry + 8 + 8 * NW + 8) + hits * 16 + I * (8 * NW + 8 + 8 + ncols / 8.0);
+            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + tiles * 8 + nA * 16.0 + E * (8 * NW + 12);
+            prof_bytes[K_INDEX] += tiles * 8 + E * 4;
+            prof_bytes[K_RESOLVE] += E * (4 + 4 + 8 + 8 * NW + 8 + 8 * NW + 8) + hits * 16 + I * (8 * NW + 8 + 8 + ncols / 8.0);
         }
     }
 
@@ -1980,7 +1982,7 @@ struct mpa_engine {
             ensure_fused_arrays<NW>();
             const size_t end = std::min(ng, gi + kChunk);
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
-            const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
+            const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(tiles_ub, 1));
             const uint32_t igrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kIdxTiles - 1) / kIdxTiles, 1));
             const uint32_t rgrid =
                 static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, kMaxGrid));
@@ -1999,15 +2001,15 @@ struct mpa_engine {
                 }
                 const uint32_t ep = ++desc_epoch;
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), ig,
-                                   tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
-                                   gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), t_item.as<uint32_t>(),
-                                   t_val.as<double>(), t_key.as<uint64_t>(), f_elead.as<uint64_t>(),
-                                   t_cnt.as<uint32_t>(), counters.as<uint64_t>());
+                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(64), 0, stream, gdesc.as<GateDesc>(), ig, tr,
+                                   cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
+                                   gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), f_vsrc.as<double>(),
+                                   t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(),
+                                   counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL(k_index, dim3(igrid), dim3(kBlock), 0, stream, ig, ep, t_cnt.as<uint32_t>(),
+                hipLaunchKernelGGL(k_index, dim3(igrid), dim3(kBlock), 0, stream, ig, ep, t_cnt.as<uint64_t>(),
                                    f_map.as<uint32_t>(), cap_items, desc_l.as<uint64_t>(), desc_f.as<uint64_t>(),
                                    desc_a.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>());
                 HIP_TRY(hipGetLastError());
@@ -2015,8 +2017,7 @@ struct mpa_engine {
                 pa = prof_begin();
                 hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
-                                   n_modes, f_map.as<uint32_t>(), t_item.as<uint32_t>(), t_val.as<double>(),
-                                   t_key.as<uint64_t>(), cap_items,
+                                   n_modes, f_map.as<uint32_t>(), t_item.as<uint32_t>(), f_vsrc.as<double>(), cap_items,
                                    f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask, desc_m.as<uint64_t>(),
                                    counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots);
                 HIP_TRY(hipGetLastError());
