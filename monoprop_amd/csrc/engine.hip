@@ -55,9 +55,9 @@ enum : int {
     C_N = 0, C_NL = 1, C_NF = 2, C_NINS = 3, C_NEMIT = 4, C_NPAIR = 5, C_ERR = 6,
     C_STOP = 7,              // fused pipeline: 1 + index of the gate to resume at (0 = running)
     C_GCOS = 8, C_GROT = 9,  // graph record counts (cumulative over a build)
-    // fused pipeline: work tickets of the index and resolve kernels (each resets the other's), and the
-    // live count per gate parity (C_NB + (gate & 1))
-    C_TK_IDX = 10, C_TK_RES = 11,
+    // fused pipeline: work ticket of the resolve kernel (reset by the sweep) and the live count per
+    // gate parity (C_NB + (gate & 1))
+    C_TK_RES = 11,
     C_NB = 16,
     C_COUNT = 24
 };
@@ -67,10 +67,10 @@ constexpr uint32_t kPhaseBit = 0x80000000u;  // rotation record: target row | (p
 
 // Kernel classes timed by the profiler (mpa_kernel_profile).
 enum KernelKind : int { K_SCAN, K_SCANSUM, K_EXPAND, K_LEADERS, K_FOLLOWERS, K_UNMATCHED, K_COUNT, K_INSERT,
-                        K_COLS, K_ADVANCE, K_SWEEP, K_INDEX, K_RESOLVE, K_KINDS };
+                        K_COLS, K_ADVANCE, K_SWEEP, K_RESOLVE, K_KINDS };
 const char *const kKindNames[K_KINDS] = {"k_scan", "k_scan_sums", "k_expand", "k_leaders", "k_followers",
                                          "k_leaders_unmatched", "k_count_miss", "k_insert", "k_cols_update",
-                                         "k_advance", "k_sweep", "k_index", "k_resolve"};
+                                         "k_advance", "k_sweep", "k_resolve"};
 
 thread_local std::string g_error;
 
@@ -640,38 +640,50 @@ __global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__
 // Fused contract-immediately pipeline (the default propagate: no orbital cap, cos != 0).
 //
 // A global open-addressing index over every stored row answers "is M^G stored?" for the emitting
-// terms only.  Slot = (fingerprint << 32) | (row + 1), 0 = empty; a fingerprint match is confirmed
-// against the stored row.
+// terms only.  Slot = NW + 1 words: tag (fingerprint << 32) | (row + 1), 0 = empty, then the key
+// itself, so a probe reads one slot and needs no second access to the store.  The tag is claimed
+// first and the key written after; lookups skip rows appended during the current gate (they are
+// nobody's partner), so a key still being written is never compared.
 
 template <int NW>
 __device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_t gmask, const uint64_t *key,
                                             uint32_t row) {
+    constexpr int SW = NW + 1;
     const uint64_t h = mpa::hash_key<NW>(key);
     const uint64_t val = (h & 0xffffffff00000000ull) | (static_cast<uint64_t>(row) + 1);
     uint64_t pos = h & gmask;
     for (uint64_t probe = 0; probe <= gmask; ++probe) {
-        const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(gidx + pos), 0ull, val);
-        if (old == 0ull) return true;
+        uint64_t *slot = gidx + pos * SW;
+        const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(slot), 0ull, val);
+        if (old == 0ull) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) slot[1 + w] = key[w];
+            return true;
+        }
         pos = (pos + 1) & gmask;
     }
     return false;
 }
 
+// Row of `key` among rows [0, limit), or -1.
 template <int NW>
-__device__ __forceinline__ int64_t gidx_find(const uint64_t *__restrict__ gidx, uint64_t gmask,
-                                             const uint64_t *__restrict__ rows, const uint64_t *key) {
+__device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                   const uint64_t *key, uint64_t limit) {
+    constexpr int SW = NW + 1;
     const uint64_t h = mpa::hash_key<NW>(key);
     const uint64_t fp = h & 0xffffffff00000000ull;
     uint64_t pos = h & gmask;
     for (uint64_t probe = 0; probe <= gmask; ++probe) {
-        const uint64_t v = gidx[pos];
-        if (v == 0) return -1;
-        if ((v & 0xffffffff00000000ull) == fp) {
-            const uint64_t r = (v & 0xffffffffull) - 1;
-            const uint64_t *pr = rows + r * NW;
+        const uint64_t *slot = gidx + pos * SW;
+        uint64_t e[SW];
+#pragma unroll
+        for (int w = 0; w < SW; ++w) e[w] = slot[w];
+        if (e[0] == 0) return -1;
+        const uint64_t r = (e[0] & 0xffffffffull) - 1;
+        if ((e[0] & 0xffffffff00000000ull) == fp && r < limit) {
             bool eq = true;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) eq = eq && (pr[w] == key[w]);
+            for (int w = 0; w < NW; ++w) eq = eq && (e[1 + w] == key[w]);
             if (eq) return static_cast<int64_t>(r);
         }
         pos = (pos + 1) & gmask;
@@ -819,9 +831,11 @@ __device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, in
 
 constexpr int kWTileWords = 16;                          // sweep: one wavefront per 16 column words
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
-constexpr int kIdxTilesPerThread = 4;
-constexpr int kIdxTiles = kBlock * kIdxTilesPerThread;    // index: 1024 sweep tiles per block
-constexpr uint32_t kOwnWin = 8192;                        // index: list positions per LDS window
+constexpr int kGroupTiles = 64;                           // tiles per count group (65536 terms): 64 atomics per word
+constexpr uint32_t kMaxGroups = 4096;                     // groups whose prefixes fit in LDS (2^28 terms)
+// per-group count word: leaders | followers << 20 | anticommuting << 40 (each <= 2^16)
+constexpr uint32_t kGrpBits = 20;
+constexpr uint64_t kGrpMask = (uint64_t{1} << kGrpBits) - 1;
 constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
 constexpr uint32_t kMaxGrid = 1024;                       // resolve blocks that take chunks
 // per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
@@ -845,24 +859,18 @@ __device__ __forceinline__ void wave_lds_sync() {
 //   E  candidates dense over the lanes: partner key m ^ G against the structural cutoff (or the
 //      upper_atol rescue) -> emit bit (LDS)
 //   D  lane j lists word j's emitters into the tile's region [1024 tile, ...): leaders ascending, then
-//      followers ascending (pivot clear / set); the tile's counts; leader emit masks per word
+//      followers ascending (pivot clear / set); the tile's counts, also summed per 64-tile group
+//      (one 64-bit atomic per tile); leader emit masks per word
 template <int NW, int PAULI>
-__global__ __launch_bounds__(64) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
-                                              const uint64_t *__restrict__ cols, uint64_t colstride,
-                                              const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
-                                              const uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                              double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
-                                              uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
-                                              const uint64_t *__restrict__ counters) {
-    __shared__ uint64_t sem[kWTileWords];
-    __shared__ uint16_t cidx[kWTileRows];
-    if (counters[C_STOP]) return;
-    const GateDesc &gd = gdesc[gi];
-    const uint64_t N = counters[C_NB + (gi & 1)];
+__device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
+                                               uint64_t colstride, const int32_t *__restrict__ fold, int nfold,
+                                               int piv0, int piv1, const uint64_t *__restrict__ rows,
+                                               double *__restrict__ coef, double *__restrict__ vsrc,
+                                               uint32_t *__restrict__ t_item, uint64_t *__restrict__ elead,
+                                               uint64_t *__restrict__ tcnt, uint64_t N, uint64_t tile,
+                                               uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx) {
     const uint64_t nwords = (N + 63) / 64;
-    const uint64_t tile = blockIdx.x;
     const uint64_t w0 = tile * kWTileWords;
-    if (w0 >= nwords) return;
     const uint32_t lane = lane_id();
     const uint64_t row0 = tile * kWTileRows;
     // ---- A: lanes hold (column slot s = lane / 16, word j = lane % 16)
@@ -986,208 +994,199 @@ __global__ __launch_bounds__(64) void k_sweep(const GateDesc *__restrict__ gdesc
     k = nl + iff - cf;
     for (uint64_t bits = sef; bits; bits &= bits - 1)
         region[k++] = static_cast<uint32_t>(row0 + lane * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
+    return static_cast<uint64_t>(nl) | (static_cast<uint64_t>(nf) << kGrpBits) |
+           (static_cast<uint64_t>(na) << (2 * kGrpBits));
 }
 
-// Ordered emitter lists from the sweep's tiles: exclusive prefixes of the per-tile counts across the
-// launch (ticket-ordered blocks, decoupled look-back) turn into a map from list position to tile-region
-// slot: leaders at MAP[0, nEL), followers at MAP[cap - 1 - k], each ascending.  The map is written
-// with no global loads and coalesced, whatever the tiles' emitter counts (positions are assigned to
-// their tiles through an LDS window first).  The block holding the last tile publishes the totals and
-// the gate's statistics.
-
-__global__ __launch_bounds__(kBlock) void k_index(int gi, uint32_t ep, const uint64_t *__restrict__ tcnt,
-                                                  uint32_t *__restrict__ map, uint64_t cap_items,
-                                                  uint64_t *__restrict__ desc_l, uint64_t *__restrict__ desc_f,
-                                                  uint64_t *__restrict__ desc_a, uint64_t *__restrict__ counters,
-                                                  int64_t *__restrict__ stats) {
-    __shared__ uint32_t lds[kWaves];
-    __shared__ uint32_t s_ticket, s_base[3], s_tot[3];
-    __shared__ uint32_t pre_l[kIdxTiles], pre_f[kIdxTiles], cnt_l[kIdxTiles];
-    __shared__ uint16_t owner[kOwnWin];
+// One 1024-term tile per wavefront, four per block; the block adds its tiles' counts into their group
+// word with one atomic (groups are 64 tiles, so 16 atomics per word).
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
+                                                  const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                  const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
+                                                  const uint64_t *__restrict__ rows, double *__restrict__ coef,
+                                                  double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
+                                                  uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
+                                                  uint64_t *__restrict__ gsum, uint64_t *__restrict__ counters) {
+    __shared__ uint64_t sem[kWaves][kWTileWords];
+    __shared__ uint16_t cidx[kWaves][kWTileRows];
+    __shared__ uint64_t s_grp[kWaves];
     if (counters[C_STOP]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_TK_RES] = 0;
     const uint64_t N = counters[C_NB + (gi & 1)];
-    const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
-    const uint64_t nblk = (ntiles + kIdxTiles - 1) / kIdxTiles;
-    int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[C_TK_RES] = 0;
-        if (nblk == 0) {
-            counters[C_NL] = counters[C_NF] = 0;
-            st[0] = static_cast<int64_t>(N);
-        }
-    }
-    if (blockIdx.x >= nblk) return;
-    if (threadIdx.x == 0)
-        s_ticket = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_IDX), 1ull));
-    __syncthreads();
-    const uint32_t b = s_ticket;
-    const uint64_t tb = static_cast<uint64_t>(b) * kIdxTiles;
-    const uint32_t nt = static_cast<uint32_t>(std::min<uint64_t>(kIdxTiles, ntiles - tb));
-    const uint32_t t0 = threadIdx.x * kIdxTilesPerThread;
-    uint32_t cl[kIdxTilesPerThread], cf[kIdxTilesPerThread];
-    uint32_t sl = 0, sf = 0, sa = 0;
-#pragma unroll
-    for (int k = 0; k < kIdxTilesPerThread; ++k) {
-        const uint64_t c = t0 + k < nt ? tcnt[tb + t0 + k] : 0u;
-        cl[k] = static_cast<uint32_t>(c & kCntMask);
-        cf[k] = static_cast<uint32_t>((c >> kCntBits) & kCntMask);
-        sl += cl[k];
-        sf += cf[k];
-        sa += static_cast<uint32_t>(c >> (2 * kCntBits));
-    }
-    uint32_t agg_l, agg_f, agg_a;
-    uint32_t xl = block_excl_scan(sl, lds, &agg_l);
-    uint32_t xf = block_excl_scan(sf, lds, &agg_f);
-    (void)block_excl_scan(sa, lds, &agg_a);
-#pragma unroll
-    for (int k = 0; k < kIdxTilesPerThread; ++k) {
-        pre_l[t0 + k] = xl;
-        pre_f[t0 + k] = xf;
-        cnt_l[t0 + k] = cl[k];
-        xl += cl[k];
-        xf += cf[k];
-    }
+    const uint64_t nwords = (N + 63) / 64;
+    const uint64_t tile0 = static_cast<uint64_t>(blockIdx.x) * kWaves;
+    if (tile0 * kWTileWords >= nwords) return;  // whole block idle
     const uint32_t wave = threadIdx.x >> 6;
-    if (wave < 3) {
-        uint64_t *desc = wave == 0 ? desc_l : (wave == 1 ? desc_f : desc_a);
-        const uint32_t agg = wave == 0 ? agg_l : (wave == 1 ? agg_f : agg_a);
-        uint32_t excl = 0;
-        if (b == 0) {
-            if (lane_id() == 0) desc_publish(desc, ep, kDescIncl, agg);
-        } else {
-            if (lane_id() == 0) desc_publish(desc + b, ep, kDescAgg, agg);
-            excl = look_back(desc, b, ep, counters);
-            if (lane_id() == 0) desc_publish(desc + b, ep, kDescIncl, excl + agg);
-        }
-        if (lane_id() == 0) {
-            s_base[wave] = excl;
-            s_tot[wave] = excl + agg;
-        }
-    }
+    const uint64_t tile = tile0 + wave;
+    uint64_t grp = 0;
+    if (tile * kWTileWords < nwords)
+        grp = sweep_tile<NW, PAULI>(gdesc[gi], tr, cols, colstride, fold, nfold, piv0, piv1, rows, coef, vsrc, t_item,
+                                    elead, tcnt, N, tile, sem[wave], cidx[wave]);
+    if (lane_id() == 0) s_grp[wave] = grp;
     __syncthreads();
-    if (b == nblk - 1 && threadIdx.x == 0) {
-        counters[C_NL] = s_tot[0];
-        counters[C_NF] = s_tot[1];
-        st[0] = static_cast<int64_t>(N);
-        st[1] = static_cast<int64_t>(s_tot[2]);
-        st[2] = static_cast<int64_t>(s_tot[0]) + static_cast<int64_t>(s_tot[1]);
-        st[4] = static_cast<int64_t>(s_tot[0]);
+    if (threadIdx.x == 0) {
+        uint64_t g = 0;
+        for (int w = 0; w < kWaves; ++w) g += s_grp[w];
+        if (g) atomicAdd(reinterpret_cast<unsigned long long *>(gsum + tile0 / kGroupTiles), g);
     }
-    // map entries in windows of kOwnWin positions: each thread marks the positions of its own tiles in
-    // LDS (owner), then the block writes the window's map entries coalesced
-    const uint64_t base_l = s_base[0], base_f = s_base[1];
-    for (int side = 0; side < 2; ++side) {
-        const uint32_t agg = side == 0 ? agg_l : agg_f;
-        const uint32_t *pre = side == 0 ? pre_l : pre_f;
-        for (uint32_t W0 = 0; W0 < agg; W0 += kOwnWin) {
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < kIdxTilesPerThread; ++k) {
-                const uint32_t t = t0 + k;
-                if (t >= nt) break;
-                const uint32_t n = side == 0 ? cl[k] : cf[k];
-                const uint32_t lo = std::max(pre[t], W0), hi = std::min(pre[t] + n, W0 + kOwnWin);
-                for (uint32_t f = lo; f < hi; ++f) owner[f - W0] = static_cast<uint16_t>(t);
-            }
-            __syncthreads();
-            const uint32_t wend = std::min(agg, W0 + kOwnWin);
-            for (uint32_t f = W0 + threadIdx.x; f < wend; f += kBlock) {
-                const uint32_t t = owner[f - W0];
-                if (side == 0) {
-                    map[base_l + f] = static_cast<uint32_t>((tb + t) * kWTileRows + (f - pre[t]));
-                } else {
-                    map[cap_items - 1 - (base_f + f)] =
-                        static_cast<uint32_t>((tb + t) * kWTileRows + cnt_l[t] + (f - pre[t]));
-                }
-            }
-        }
-    }
-}
-
-// Global row index lookup restricted to rows [0, limit): rows appended while a gate resolves are never
-// the partner of another emitter, so their entries are skipped without reading them.
-template <int NW>
-__device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gmask,
-                                                   const uint64_t *__restrict__ rows, const uint64_t *key,
-                                                   uint64_t limit) {
-    const uint64_t h = mpa::hash_key<NW>(key);
-    const uint64_t fp = h & 0xffffffff00000000ull;
-    uint64_t pos = h & gmask;
-    for (uint64_t probe = 0; probe <= gmask; ++probe) {
-        const uint64_t e = gidx[pos];
-        if (e == 0) return -1;
-        const uint64_t r = (e & 0xffffffffull) - 1;
-        if ((e & 0xffffffff00000000ull) == fp && r < limit) {
-            const uint64_t *pr = rows + r * NW;
-            bool eq = true;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) eq = eq && (pr[w] == key[w]);
-            if (eq) return static_cast<int64_t>(r);
-        }
-        pos = (pos + 1) & gmask;
-    }
-    return -1;
 }
 
 // Partner resolution of every emitter (Engine.h:364-387, 465-513) and the ordered append of the
-// partners to create (leader misses then follower misses, ascending, Engine.h:428-444).  One emitter
-// per thread, 256-emitter chunks by work ticket; a chunk's misses land at N + (misses of earlier chunks)
-// by look-back, so the append needs no second pass; appended terms enter the row index, and their
-// column bits are gathered per word in LDS before one OR per word.  A follower whose partner leader emits is rotated by that leader (the
-// reference's matched-follower skip).  A hit applies the rotation against the swept partner value
-// recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss writes the new term's
-// coefficient k*0 + (s ph) v.  The chunk holding the last emitter publishes N + I for the next gate and
-// stops the device before a gate whose worst case could outgrow the store or the index.
+// partners to create (leader misses then follower misses, ascending, Engine.h:428-444).
+//   positions  every block turns the sweep's per-group counts into prefixes (LDS); the chunk's list
+//              positions start in group g (binary search); the tile counts from there on, 1024 tiles at
+//              a time, place position f in tile t at offset f - prefix: the tile's region holds its row
+//   resolution one emitter per thread, 256-emitter chunks (leader chunks, then follower chunks) by work
+//              ticket.  A follower whose partner leader emits is rotated by that leader (the
+//              reference's matched-follower skip).  A hit applies the rotation against the swept
+//              partner value recovered as (v k)(1/k) (Engine.h:209-211, FusedApply.h:61-72); a miss
+//              writes the new term's coefficient k*0 + (s ph) v
+//   append     a chunk's misses land at N + (misses of earlier chunks) by look-back; new terms enter
+//              the row index, and their column bits are gathered per word in LDS before one OR per
+//              word
+// The chunk holding the last emitter publishes N + I for the next gate and stops the device before a
+// gate whose worst case could outgrow the store or the index.  Every block also clears its share of
+// the other gate parity's group counts.
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
-                                                    const uint32_t *__restrict__ map,
                                                     const uint32_t *__restrict__ t_item,
-                                                    const double *__restrict__ vsrc, uint64_t cap_items,
+                                                    const uint64_t *__restrict__ tcnt,
+                                                    const uint64_t *__restrict__ gsum, uint64_t *__restrict__ gsum_next,
+                                                    uint64_t n_groups_next, const double *__restrict__ vsrc,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
                                                     uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
-                                                    uint64_t cap_rows, uint64_t gidx_slots) {
+                                                    uint64_t cap_rows, uint64_t gidx_slots, uint32_t max_groups) {
     constexpr int NC = max_cols<NW, PAULI>();
     constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
     __shared__ uint64_t lcol[kColWords * NC];
+    extern __shared__ uint32_t dyn_lds[];  // group prefixes: leaders [0, max_groups], followers after
+    constexpr uint32_t kPosWindow = 4 * kBlock;
+    __shared__ uint32_t tp[kPosWindow], tnl[kPosWindow];
     __shared__ uint32_t lds[kWaves];
-    __shared__ uint32_t s_ticket, s_base;
+    __shared__ uint32_t s_ticket, s_base, s_t0, s_wbase;
+    uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
     if (counters[C_STOP]) return;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; g < n_groups_next;
+         g += static_cast<uint64_t>(gridDim.x) * kBlock)
+        gsum_next[g] = 0;
     const GateDesc &gd = gdesc[gi];
     const int nxt = (gi & 1) ^ 1;
     const uint64_t N = counters[C_NB + (gi & 1)];
-    const uint64_t nEL = counters[C_NL], nEF = counters[C_NF];
-    const uint64_t n_items = nEL + nEF;
-    const uint64_t nchunks = (n_items + kResChunk - 1) / kResChunk;
+    const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
+    const uint32_t ng = static_cast<uint32_t>((ntiles + kGroupTiles - 1) / kGroupTiles);
+    // ---- group prefixes (every block)
+    uint32_t nEL = 0, nEF = 0, nA = 0;
+    for (uint32_t base = 0; base < ng; base += kBlock) {
+        const uint32_t g = base + threadIdx.x;
+        const uint64_t v = g < ng ? gsum[g] : 0;
+        uint32_t tl, tf, ta;
+        const uint32_t xl = block_excl_scan(static_cast<uint32_t>(v & kGrpMask), lds, &tl);
+        const uint32_t xf = block_excl_scan(static_cast<uint32_t>((v >> kGrpBits) & kGrpMask), lds, &tf);
+        (void)block_excl_scan(static_cast<uint32_t>(v >> (2 * kGrpBits)), lds, &ta);
+        if (g < ng) {
+            gpl[g] = nEL + xl;
+            gpf[g] = nEF + xf;
+        }
+        nEL += tl;
+        nEF += tf;
+        nA += ta;
+    }
+    if (threadIdx.x == 0) {
+        gpl[ng] = nEL;
+        gpf[ng] = nEF;
+    }
+    const uint32_t cL = (nEL + kResChunk - 1) / kResChunk;
+    const uint32_t nchunks = cL + (nEF + kResChunk - 1) / kResChunk;
     int64_t *st = stats + static_cast<int64_t>(gi) * kStatW;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[C_TK_IDX] = 0;
+        counters[C_NL] = nEL;
+        counters[C_NF] = nEF;
+        st[0] = static_cast<int64_t>(N);
+        st[1] = static_cast<int64_t>(nA);
+        st[2] = static_cast<int64_t>(nEL) + static_cast<int64_t>(nEF);
+        st[4] = static_cast<int64_t>(nEL);
         if (nchunks == 0) counters[C_NB + nxt] = N;
     }
     if (blockIdx.x >= nchunks) return;  // only blocks with work take tickets
     for (;;) {
-        __syncthreads();  // LDS of the previous chunk fully consumed
+        __syncthreads();  // LDS of the previous chunk fully consumed (and the group prefixes written)
         if (threadIdx.x == 0)
             s_ticket = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
         __syncthreads();
         const uint32_t chunk = s_ticket;
         if (chunk >= nchunks) break;
-        const uint64_t i = static_cast<uint64_t>(chunk) * kResChunk + threadIdx.x;
+        // ---- positions: this thread's emitter f of side L (leaders) or F (followers)
+        const bool leader = chunk < cL;
+        const uint32_t *gp = leader ? gpl : gpf;
+        const uint32_t total = leader ? nEL : nEF;
+        const uint32_t first = (leader ? chunk : chunk - cL) * kResChunk;
+        const uint32_t cnt = std::min<uint32_t>(kResChunk, total - first);
+        const uint32_t f = first + threadIdx.x;
+        const bool valid = threadIdx.x < cnt;
+        if (threadIdx.x == 0) {
+            // group holding position `first`: largest g with gp[g] <= first
+            uint32_t lo = 0, hi = ng;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (gp[mid] <= first) lo = mid;
+                else hi = mid;
+            }
+            s_t0 = lo * kGroupTiles;
+            s_wbase = gp[lo];
+        }
+        __syncthreads();
+        // windows of 1024 tiles (4 per thread) from the group's first tile until the chunk's positions
+        // are placed
+        uint32_t S = 0;
+        for (uint64_t w0 = s_t0, wbase = s_wbase; wbase < first + cnt && w0 < ntiles; w0 += kPosWindow) {
+            uint32_t own[4], nl4[4], sum4 = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t tt = w0 + 4 * threadIdx.x + k;
+                const uint64_t c = tt < ntiles ? tcnt[tt] : 0;
+                nl4[k] = static_cast<uint32_t>(c & kCntMask);
+                own[k] = leader ? nl4[k] : static_cast<uint32_t>((c >> kCntBits) & kCntMask);
+                sum4 += own[k];
+            }
+            uint32_t wsum;
+            uint32_t x = block_excl_scan(sum4, lds, &wsum);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                tp[4 * threadIdx.x + k] = x;
+                tnl[4 * threadIdx.x + k] = nl4[k];
+                x += own[k];
+            }
+            __syncthreads();
+            if (valid && wbase <= f && f < wbase + wsum) {
+                const uint32_t local = static_cast<uint32_t>(f - wbase);
+                uint32_t lo = 0, hi = kPosWindow;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (tp[mid] <= local) lo = mid;
+                    else hi = mid;
+                }
+                const uint64_t slot = (w0 + lo) * kWTileRows + (leader ? 0u : tnl[lo]) + (local - tp[lo]);
+                S = t_item[slot];
+            }
+            wbase += wsum;
+            __syncthreads();
+        }
+        // ---- resolution
         uint64_t t[NW];
         double ins = 0.0;
         uint32_t miss = 0, hit = 0;
-        if (i < n_items) {
-            const bool leader = i < nEL;
-            const uint32_t S = t_item[map[leader ? i : cap_items - 1 - (i - nEL)]];
+        if (valid) {
             const double vS = vsrc[S];
             uint64_t m[NW];
             load_row<NW>(rows, S, m);
 #pragma unroll
             for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-            const int64_t T = gidx_find_below<NW>(gidx, gmask, rows, t, N);
+            const int64_t T = gidx_find_below<NW>(gidx, gmask, t, N);
             const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
             if (!skip) {
                 const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
@@ -1445,7 +1444,8 @@ struct mpa_engine {
 
     // Fused pipeline state: global row index, the emitter list with pre-gate values (cap / 2 entries:
     // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
-    DevArray gidx, f_map, f_elead, f_vsrc, t_item, t_cnt, desc_l, desc_f, desc_a, desc_m;
+    DevArray gidx, f_elead, f_vsrc, t_item, t_cnt, t_gsum, desc_m;
+    uint64_t gsum_stride = 0;  // per gate parity
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
@@ -1878,7 +1878,7 @@ struct mpa_engine {
 
     // Fresh descriptor memory reads as epoch 0, which no gate uses.
     void clear_descriptors() {
-        for (DevArray *d : {&desc_l, &desc_f, &desc_a, &desc_m}) HIP_TRY(hipMemsetAsync(d->p, 0, d->bytes, stream));
+        HIP_TRY(hipMemsetAsync(desc_m.p, 0, desc_m.bytes, stream));
         desc_epoch = 0;
     }
 
@@ -1889,17 +1889,17 @@ struct mpa_engine {
         if (fused_cap != cap) {
             cap_items = cap / 2;
             const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
-            f_map.reserve(cap_items * 4);
             t_item.reserve(tiles * kWTileRows * 4);
             t_cnt.reserve(tiles * 8);
+            gsum_stride = (tiles + kGroupTiles - 1) / kGroupTiles + 1;
+            if (gsum_stride > kMaxGroups)
+                throw RunError("store of " + std::to_string(cap) + " rows exceeds the fused pipeline's limit; "
+                               "use the per-gate pipeline (mpa_set_pipeline(e, 0))");
+            t_gsum.reserve(2 * gsum_stride * 8);
+            HIP_TRY(hipMemsetAsync(t_gsum.p, 0, t_gsum.bytes, stream));  // the sweep accumulates into it
             f_vsrc.reserve(cap * 8);
             f_elead.reserve((cap / 64) * 8);
-            const uint64_t iblocks = tiles / kIdxTiles + 2;
-            const uint64_t chunks = cap_items / kResChunk + 2;
-            desc_l.reserve(iblocks * 8);
-            desc_f.reserve(iblocks * 8);
-            desc_a.reserve(iblocks * 8);
-            desc_m.reserve(chunks * 8);
+            desc_m.reserve((cap_items / kResChunk + 2) * 8);
             clear_descriptors();
             fused_cap = cap;
         }
@@ -1907,12 +1907,12 @@ struct mpa_engine {
         uint64_t want = 1024;
         while (want < 2 * cap) want <<= 1;
         if (want > gidx_slots) {
-            gidx.reserve(want * 8);
+            gidx.reserve(want * (nw + 1) * 8);
             gidx_slots = want;
             gidx_rows = static_cast<uint64_t>(-1);  // force a rebuild
         }
         if (gidx_rows != N) {
-            HIP_TRY(hipMemsetAsync(gidx.p, 0, gidx_slots * 8, stream));
+            HIP_TRY(hipMemsetAsync(gidx.p, 0, gidx_slots * (nw + 1) * 8, stream));
             if (N) {
                 hipLaunchKernelGGL((k_gidx_build<NW>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)),
                                    dim3(kBlock), 0, stream, rows.as<uint64_t>(), 0, N, gidx.as<uint64_t>(),
@@ -1925,8 +1925,8 @@ struct mpa_engine {
 
     // Algorithmic bytes of the fused kernels from the per-gate counters.  Sweep: fold + pivot column
     // words, leader masks, tile counts, the anticommuting coefficients (read + write), per emitter its
-    // row, pre-gate value and tile entry.  Index: tile counts, one map entry per emitter.  Resolve: per
-    // emitter its map and tile entries, pre-gate value, row, index slot, partner row and coefficient writes; per hit the partner
+    // row, pre-gate value and tile entry.  Resolve: group and tile counts, per emitter its tile entry,
+    // pre-gate value, row, index slot (key inline) and coefficient writes; per hit the partner
     // coefficient (read + write); per appended row the row, coefficient, index slot and column bits.
     void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
         const double NW = nw;
@@ -1936,21 +1936,22 @@ struct mpa_engine {
             const double tiles = std::ceil(words / kWTileWords);
             const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], hits = stat_p[g];
             prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + tiles * 8 + nA * 16.0 + E * (8 * NW + 12);
-            prof_bytes[K_INDEX] += tiles * 8 + E * 4;
-            prof_bytes[K_RESOLVE] += E * (4 + 4 + 8 + 8 * NW + 8 + 8 * NW + 8) + hits * 16 + I * (8 * NW + 8 + 8 + ncols / 8.0);
+            prof_bytes[K_RESOLVE] += tiles * 8 + E * (4 + 8 + 8 * NW + 8 * (NW + 1) + 8) + hits * 16 +
+                                     I * (8 * NW + 8 + 8 * (NW + 1) + ncols / 8.0);
         }
     }
 
-    // Live count of gate `g`'s parity and the work tickets, as the device pipeline expects them before
-    // gate `g`; nothing stopped.
+    // Live count of gate `g`'s parity, the work ticket and both parities' group counts, as the device
+    // pipeline expects them before gate `g`; nothing stopped.
     void reset_fused_counters(size_t g) {
+        if (t_gsum.p) HIP_TRY(hipMemsetAsync(t_gsum.p, 0, t_gsum.bytes, stream));
         uint64_t h[C_COUNT];
         HIP_TRY(hipMemcpyAsync(h, counters.p, C_COUNT * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         h[C_NB + (g & 1)] = N;
         h[C_N] = N;
         h[C_STOP] = 0;
-        h[C_TK_IDX] = h[C_TK_RES] = 0;
+        h[C_TK_RES] = 0;
         HIP_TRY(hipMemcpyAsync(counters.p, h, C_COUNT * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
     }
@@ -1982,11 +1983,11 @@ struct mpa_engine {
             ensure_fused_arrays<NW>();
             const size_t end = std::min(ng, gi + kChunk);
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
-            const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>(tiles_ub, 1));
-            const uint32_t igrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kIdxTiles - 1) / kIdxTiles, 1));
+            const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
             const uint32_t rgrid =
                 static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, kMaxGrid));
             const uint64_t gmask = gidx_slots - 1;
+            const size_t rlds = 2 * (gsum_stride + 1) * sizeof(uint32_t);
             for (size_t g = gi; g < end; ++g) {
                 const GateDesc &gd = gds[g];
                 const int ig = static_cast<int>(g);
@@ -2000,26 +2001,23 @@ struct mpa_engine {
                     continue;
                 }
                 const uint32_t ep = ++desc_epoch;
+                uint64_t *gsum_cur = t_gsum.as<uint64_t>() + (g & 1) * gsum_stride;
+                uint64_t *gsum_nxt = t_gsum.as<uint64_t>() + ((g + 1) & 1) * gsum_stride;
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(64), 0, stream, gdesc.as<GateDesc>(), ig, tr,
+                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), ig, tr,
                                    cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
                                    gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), f_vsrc.as<double>(),
-                                   t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(),
+                                   t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), gsum_cur,
                                    counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL(k_index, dim3(igrid), dim3(kBlock), 0, stream, ig, ep, t_cnt.as<uint64_t>(),
-                                   f_map.as<uint32_t>(), cap_items, desc_l.as<uint64_t>(), desc_f.as<uint64_t>(),
-                                   desc_a.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>());
-                HIP_TRY(hipGetLastError());
-                prof_end(K_INDEX, pa);
-                pa = prof_begin();
-                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
-                                   n_modes, f_map.as<uint32_t>(), t_item.as<uint32_t>(), f_vsrc.as<double>(), cap_items,
-                                   f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask, desc_m.as<uint64_t>(),
-                                   counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots);
+                                   n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), gsum_cur, gsum_nxt,
+                                   gsum_stride, f_vsrc.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
+                                   desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
+                                   static_cast<uint32_t>(gsum_stride));
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
             }
