@@ -1,0 +1,41 @@
+"""PyTorch autograd bridge for the graph-mode expectation value.
+
+``expectation(prop, theta)`` returns ``<O>(theta)`` of a propagator whose graph was built with
+``prop.build_graph(circuit)`` as a differentiable scalar tensor: the forward pass is the engine's
+replay of the recorded layers, the backward pass is its exact adjoint sweep over the same layers
+(pre-layer snapshots, no ``1/cos`` recovery), so ``theta.grad`` equals
+``prop.expectation_value_and_gradient(theta)[1]`` scaled by the incoming gradient.  This is the
+engine-side counterpart of the reference's ``expval_and_grad`` (``MPFunctions.cpp:207-246``,
+``monomial_propagator.py:352-403``).
+
+The engine runs on the GPU whatever device ``theta`` lives on; the result comes back on ``theta``'s
+device and dtype (float64 expected).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class _GraphExpectation(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, prop) -> torch.Tensor:
+        params = theta.detach().to("cpu", torch.float64).numpy()
+        value, grad = prop.expectation_value_and_gradient(params)
+        ctx.save_for_backward(torch.as_tensor(np.asarray(grad, dtype=np.float64)).to(theta.device))
+        return torch.tensor(value, dtype=theta.dtype, device=theta.device)
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        (grad,) = ctx.saved_tensors
+        return grad_out.to(grad.dtype) * grad, None
+
+
+def expectation(prop, theta: torch.Tensor) -> torch.Tensor:
+    """Differentiable ``<O>(theta)`` of ``prop`` (a propagator with a built graph)."""
+    if prop.graph_layers == 0:
+        raise RuntimeError("build_graph must be called before a differentiable expectation value")
+    if theta.dim() != 1 or theta.numel() != prop.n_parameters:
+        raise ValueError(f"theta must be a vector of {prop.n_parameters} parameters, got shape {tuple(theta.shape)}")
+    return _GraphExpectation.apply(theta, prop)

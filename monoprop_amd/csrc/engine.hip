@@ -831,10 +831,12 @@ __device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, in
 
 constexpr int kWTileWords = 16;                          // sweep: one wavefront per 16 column words
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
-constexpr int kGroupTiles = 64;                           // tiles per count group (65536 terms): 64 atomics per word
-constexpr uint32_t kMaxGroups = 4096;                     // groups whose prefixes fit in LDS (2^28 terms)
-// per-group count word: leaders | followers << 20 | anticommuting << 40 (each <= 2^16)
-constexpr uint32_t kGrpBits = 20;
+// Count groups: 2^group_log2 tiles (64 tiles = 65536 terms, 16 atomics per word, up to 1024 tiles for
+// the largest stores) so that at most kMaxGroups prefixes sit in LDS.
+constexpr int kMinGroupLog2 = 6, kMaxGroupLog2 = 10;
+constexpr uint32_t kMaxGroups = 4096;
+// per-group count word: leaders | followers << 21 | anticommuting << 42 (each <= 2^20)
+constexpr uint32_t kGrpBits = 21;
 constexpr uint64_t kGrpMask = (uint64_t{1} << kGrpBits) - 1;
 constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
 constexpr uint32_t kMaxGrid = 1024;                       // resolve blocks that take chunks
@@ -859,8 +861,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 //   E  candidates dense over the lanes: partner key m ^ G against the structural cutoff (or the
 //      upper_atol rescue) -> emit bit (LDS)
 //   D  lane j lists word j's emitters into the tile's region [1024 tile, ...): leaders ascending, then
-//      followers ascending (pivot clear / set); the tile's counts, also summed per 64-tile group
-//      (one 64-bit atomic per tile); leader emit masks per word
+//      followers ascending (pivot clear / set); the tile's counts, also summed per group of tiles;
+//      leader emit masks per word
 template <int NW, int PAULI>
 __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
                                                uint64_t colstride, const int32_t *__restrict__ fold, int nfold,
@@ -999,7 +1001,7 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
 }
 
 // One 1024-term tile per wavefront, four per block; the block adds its tiles' counts into their group
-// word with one atomic (groups are 64 tiles, so 16 atomics per word).
+// word with one atomic.
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
                                                   const uint64_t *__restrict__ cols, uint64_t colstride,
@@ -1007,7 +1009,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                   double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
                                                   uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
-                                                  uint64_t *__restrict__ gsum, uint64_t *__restrict__ counters) {
+                                                  uint64_t *__restrict__ gsum, int group_log2,
+                                                  uint64_t *__restrict__ counters) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
@@ -1028,7 +1031,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
     if (threadIdx.x == 0) {
         uint64_t g = 0;
         for (int w = 0; w < kWaves; ++w) g += s_grp[w];
-        if (g) atomicAdd(reinterpret_cast<unsigned long long *>(gsum + tile0 / kGroupTiles), g);
+        if (g) atomicAdd(reinterpret_cast<unsigned long long *>(gsum + (tile0 >> group_log2)), g);
     }
 }
 
@@ -1059,7 +1062,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
                                                     uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
-                                                    uint64_t cap_rows, uint64_t gidx_slots, uint32_t max_groups) {
+                                                    uint64_t cap_rows, uint64_t gidx_slots, uint32_t max_groups,
+                                                    int group_log2) {
     constexpr int NC = max_cols<NW, PAULI>();
     constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
     __shared__ uint64_t lcol[kColWords * NC];
@@ -1067,7 +1071,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
     constexpr uint32_t kPosWindow = 4 * kBlock;
     __shared__ uint32_t tp[kPosWindow], tnl[kPosWindow];
     __shared__ uint32_t lds[kWaves];
-    __shared__ uint32_t s_ticket, s_base, s_t0, s_wbase;
+    __shared__ uint32_t s_ticket, s_base, s_wbase;
+    __shared__ uint64_t s_t0;
     uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
     if (counters[C_STOP]) return;
@@ -1078,7 +1083,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
     const int nxt = (gi & 1) ^ 1;
     const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
-    const uint32_t ng = static_cast<uint32_t>((ntiles + kGroupTiles - 1) / kGroupTiles);
+    const uint32_t ng = static_cast<uint32_t>((ntiles + (uint64_t{1} << group_log2) - 1) >> group_log2);
     // ---- group prefixes (every block)
     uint32_t nEL = 0, nEF = 0, nA = 0;
     for (uint32_t base = 0; base < ng; base += kBlock) {
@@ -1136,7 +1141,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                 if (gp[mid] <= first) lo = mid;
                 else hi = mid;
             }
-            s_t0 = lo * kGroupTiles;
+            s_t0 = static_cast<uint64_t>(lo) << group_log2;
             s_wbase = gp[lo];
         }
         __syncthreads();
@@ -1224,7 +1229,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                     const uint64_t I = excl + n_miss, n = N + I;
                     st[3] = static_cast<int64_t>(I);
                     counters[C_NB + nxt] = n;
-                    if (2 * n > cap_rows || 4 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+                    if (2 * n > cap_rows || 2 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
                 }
             }
         }
@@ -1446,6 +1451,7 @@ struct mpa_engine {
     // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
     DevArray gidx, f_elead, f_vsrc, t_item, t_cnt, t_gsum, desc_m;
     uint64_t gsum_stride = 0;  // per gate parity
+    int group_log2 = kMinGroupLog2;
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
@@ -1544,6 +1550,15 @@ struct mpa_engine {
     // Grow the store to hold at least `need` rows, preserving rows, coefficients and columns.
     void ensure_capacity(uint64_t need) {
         if (need <= cap) return;
+        try {
+            grow_store(need);
+        } catch (const HipFailure &f) {
+            throw RunError("out of device memory growing the term store to " + std::to_string(need) + " rows (" +
+                           std::to_string(N) + " live terms): " + f.what);
+        }
+    }
+
+    void grow_store(uint64_t need) {
         uint64_t nc = std::max<uint64_t>(need, cap + cap / 2);
         const uint64_t q = 64ull * kBlock;
         nc = std::max<uint64_t>((nc + q - 1) / q * q, q);
@@ -1891,7 +1906,10 @@ struct mpa_engine {
             const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
             t_item.reserve(tiles * kWTileRows * 4);
             t_cnt.reserve(tiles * 8);
-            gsum_stride = (tiles + kGroupTiles - 1) / kGroupTiles + 1;
+            group_log2 = kMinGroupLog2;
+            while (group_log2 < kMaxGroupLog2 && ((tiles + (uint64_t{1} << group_log2) - 1) >> group_log2) + 1 > kMaxGroups)
+                ++group_log2;
+            gsum_stride = ((tiles + (uint64_t{1} << group_log2) - 1) >> group_log2) + 1;
             if (gsum_stride > kMaxGroups)
                 throw RunError("store of " + std::to_string(cap) + " rows exceeds the fused pipeline's limit; "
                                "use the per-gate pipeline (mpa_set_pipeline(e, 0))");
@@ -1905,7 +1923,7 @@ struct mpa_engine {
         }
         if (desc_epoch >= (1u << 30) - kChunk - 1) clear_descriptors();
         uint64_t want = 1024;
-        while (want < 2 * cap) want <<= 1;
+        while (want < cap) want <<= 1;  // a gate starts at N <= cap / 2: load <= 1/2 after it
         if (want > gidx_slots) {
             gidx.reserve(want * (nw + 1) * 8);
             gidx_slots = want;
@@ -1979,7 +1997,7 @@ struct mpa_engine {
         HIP_TRY(hipEventRecord(ev[0], stream));
         size_t gi = 0;
         while (gi < ng) {
-            if (2 * N > cap) ensure_capacity(std::max<uint64_t>(4 * N, 64ull * kBlock));
+            if (2 * N > cap) ensure_capacity(std::max<uint64_t>(3 * N, 64ull * kBlock));
             ensure_fused_arrays<NW>();
             const size_t end = std::min(ng, gi + kChunk);
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
@@ -2008,7 +2026,7 @@ struct mpa_engine {
                                    cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
                                    gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), f_vsrc.as<double>(),
                                    t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), gsum_cur,
-                                   counters.as<uint64_t>());
+                                   group_log2, counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
@@ -2017,7 +2035,7 @@ struct mpa_engine {
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), gsum_cur, gsum_nxt,
                                    gsum_stride, f_vsrc.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
-                                   static_cast<uint32_t>(gsum_stride));
+                                   static_cast<uint32_t>(gsum_stride), group_log2);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
             }

@@ -111,3 +111,24 @@ def test_graph_matches_propagate(oracle_mod):
     b.build_graph(circ)
     assert a.size() == b.size()
     assert abs(a.expectation_value() - b.expectation_value(circ.parameters)) <= 1e-12
+
+
+def test_torch_autograd_matches_adjoint():
+    """theta.grad through monoprop_amd.autograd equals the engine's adjoint gradient (times the
+    incoming gradient), and the forward value equals the replayed <O>."""
+    import torch
+
+    from monoprop_amd.autograd import expectation
+
+    circ = workloads.brickwork_ring(16, 2, seed=3)
+    prop = PauliPropagator(workloads.ring_zz_mean(16), [], cutoff=6, lower_atol=1e-6)
+    prop.build_graph(circ)
+    theta0 = np.asarray(circ.parameters)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    ev = expectation(prop, theta)
+    (3.0 * ev).backward()
+    v, g = prop.expectation_value_and_gradient(theta0)
+    assert abs(ev.item() - v) <= 1e-14
+    assert np.allclose(theta.grad.numpy(), 3.0 * np.asarray(g), rtol=0, atol=1e-14)
+    with pytest.raises(ValueError):
+        expectation(prop, theta[:-1])
