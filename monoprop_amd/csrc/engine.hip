@@ -36,6 +36,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1000,8 +1001,8 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
            (static_cast<uint64_t>(na) << (2 * kGrpBits));
 }
 
-// One 1024-term tile per wavefront, four per block; the block adds its tiles' counts into their group
-// word with one atomic.
+// One 1024-term tile per wavefront, four per block; the block stores its tiles' summed counts
+// (bsum[block]) and adds them into their group word with one atomic.
 template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
                                                   const uint64_t *__restrict__ cols, uint64_t colstride,
@@ -1009,8 +1010,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                   double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
                                                   uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
-                                                  uint64_t *__restrict__ gsum, int group_log2,
-                                                  uint64_t *__restrict__ counters) {
+                                                  uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
+                                                  int group_log2, uint64_t *__restrict__ counters) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
@@ -1031,15 +1032,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
     if (threadIdx.x == 0) {
         uint64_t g = 0;
         for (int w = 0; w < kWaves; ++w) g += s_grp[w];
+        bsum[blockIdx.x] = g;
         if (g) atomicAdd(reinterpret_cast<unsigned long long *>(gsum + (tile0 >> group_log2)), g);
     }
 }
 
 // Partner resolution of every emitter (Engine.h:364-387, 465-513) and the ordered append of the
 // partners to create (leader misses then follower misses, ascending, Engine.h:428-444).
-//   positions  every block turns the sweep's per-group counts into prefixes (LDS); the chunk's list
-//              positions start in group g (binary search); the tile counts from there on, 1024 tiles at
-//              a time, place position f in tile t at offset f - prefix: the tile's region holds its row
+//   positions  every block turns the sweep's per-group counts into prefixes (LDS); list position f
+//              lies in group g (binary search), in one of its sweep blocks (the blocks' count sums) and
+//              in one of that block's four tiles (the tile counts): the tile's region holds its row
 //   resolution one emitter per thread, 256-emitter chunks (leader chunks, then follower chunks) by work
 //              ticket.  A follower whose partner leader emits is rotated by that leader (the
 //              reference's matched-follower skip).  A hit applies the rotation against the swept
@@ -1056,7 +1058,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
                                                     const uint32_t *__restrict__ t_item,
-                                                    const uint64_t *__restrict__ tcnt,
+                                                    const uint64_t *__restrict__ tcnt, const uint64_t *__restrict__ bsum,
                                                     const uint64_t *__restrict__ gsum, uint64_t *__restrict__ gsum_next,
                                                     uint64_t n_groups_next, const double *__restrict__ vsrc,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
@@ -1068,11 +1070,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
     constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
     __shared__ uint64_t lcol[kColWords * NC];
     extern __shared__ uint32_t dyn_lds[];  // group prefixes: leaders [0, max_groups], followers after
-    constexpr uint32_t kPosWindow = 4 * kBlock;
-    __shared__ uint32_t tp[kPosWindow], tnl[kPosWindow];
     __shared__ uint32_t lds[kWaves];
-    __shared__ uint32_t s_ticket, s_base, s_wbase;
-    __shared__ uint64_t s_t0;
+    __shared__ uint32_t s_ticket, s_base;
     uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
     if (counters[C_STOP]) return;
@@ -1084,6 +1083,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
     const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
     const uint32_t ng = static_cast<uint32_t>((ntiles + (uint64_t{1} << group_log2) - 1) >> group_log2);
+    const uint64_t nblocks = (ntiles + kWaves - 1) / kWaves;  // sweep blocks of this gate
     // ---- group prefixes (every block)
     uint32_t nEL = 0, nEF = 0, nA = 0;
     for (uint32_t base = 0; base < ng; base += kBlock) {
@@ -1133,53 +1133,64 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
         const uint32_t cnt = std::min<uint32_t>(kResChunk, total - first);
         const uint32_t f = first + threadIdx.x;
         const bool valid = threadIdx.x < cnt;
-        if (threadIdx.x == 0) {
-            // group holding position `first`: largest g with gp[g] <= first
+        // group of f: binary search over the group prefixes (LDS); then the group's sweep-block sums
+        // and the block's tile counts (two rounds of independent loads, however sparse the emitters)
+        uint32_t S = 0;
+        if (valid) {
             uint32_t lo = 0, hi = ng;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (gp[mid] <= first) lo = mid;
+                if (gp[mid] <= f) lo = mid;
                 else hi = mid;
             }
-            s_t0 = static_cast<uint64_t>(lo) << group_log2;
-            s_wbase = gp[lo];
-        }
-        __syncthreads();
-        // windows of 1024 tiles (4 per thread) from the group's first tile until the chunk's positions
-        // are placed
-        uint32_t S = 0;
-        for (uint64_t w0 = s_t0, wbase = s_wbase; wbase < first + cnt && w0 < ntiles; w0 += kPosWindow) {
-            uint32_t own[4], nl4[4], sum4 = 0;
+            uint32_t rem = f - gp[lo];
+            const uint32_t blocks_per_group = 1u << (group_log2 - 2);
+            const uint64_t b0 = static_cast<uint64_t>(lo) << (group_log2 - 2);
+            uint64_t blk = b0;
+            for (uint32_t base = 0; base < blocks_per_group; base += 16) {
+                uint32_t cb[16];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint64_t tt = w0 + 4 * threadIdx.x + k;
-                const uint64_t c = tt < ntiles ? tcnt[tt] : 0;
-                nl4[k] = static_cast<uint32_t>(c & kCntMask);
-                own[k] = leader ? nl4[k] : static_cast<uint32_t>((c >> kCntBits) & kCntMask);
-                sum4 += own[k];
-            }
-            uint32_t wsum;
-            uint32_t x = block_excl_scan(sum4, lds, &wsum);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                tp[4 * threadIdx.x + k] = x;
-                tnl[4 * threadIdx.x + k] = nl4[k];
-                x += own[k];
-            }
-            __syncthreads();
-            if (valid && wbase <= f && f < wbase + wsum) {
-                const uint32_t local = static_cast<uint32_t>(f - wbase);
-                uint32_t lo = 0, hi = kPosWindow;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (tp[mid] <= local) lo = mid;
-                    else hi = mid;
+                for (int k = 0; k < 16; ++k) {
+                    const uint64_t bb = b0 + base + k;
+                    const uint64_t v = bb < nblocks ? bsum[bb] : 0;
+                    cb[k] = static_cast<uint32_t>(leader ? (v & kGrpMask) : ((v >> kGrpBits) & kGrpMask));
                 }
-                const uint64_t slot = (w0 + lo) * kWTileRows + (leader ? 0u : tnl[lo]) + (local - tp[lo]);
-                S = t_item[slot];
+                bool found = false;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    if (found) continue;
+                    if (rem < cb[k]) {
+                        blk = b0 + base + k;
+                        found = true;
+                    } else {
+                        rem -= cb[k];
+                    }
+                }
+                if (found) break;
             }
-            wbase += wsum;
-            __syncthreads();
+            uint64_t tc[kWaves];
+#pragma unroll
+            for (int k = 0; k < kWaves; ++k) {
+                const uint64_t tt = blk * kWaves + k;
+                tc[k] = tt < ntiles ? tcnt[tt] : 0;
+            }
+            uint64_t tile = blk * kWaves;
+            uint32_t nl_t = 0;
+            bool found = false;
+#pragma unroll
+            for (int k = 0; k < kWaves; ++k) {
+                if (found) continue;
+                const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
+                const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
+                if (rem < own) {
+                    tile = blk * kWaves + k;
+                    nl_t = nl1;
+                    found = true;
+                } else {
+                    rem -= own;
+                }
+            }
+            S = t_item[tile * kWTileRows + (leader ? 0u : nl_t) + rem];
         }
         // ---- resolution
         uint64_t t[NW];
@@ -1394,12 +1405,100 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__rest
 // ------------------------------------------------------------------------------------------------
 // host side
 
+// Device memory blocks are cached per process instead of returned: a regrowing store (capacities are
+// powers of two) and the next propagator get the same sizes back without mapping new memory, which
+// for blocks of many GB costs far more than the work on them.  A block freed on one stream and taken
+// on another waits for the freeing stream's work first (an event recorded at free time).  When HIP
+// cannot allocate, every cached block goes back to HIP and the allocation is retried once.
+class BlockCache {
+   public:
+    static BlockCache &get() {
+        static BlockCache c;
+        return c;
+    }
+    void *take(size_t &bytes, hipStream_t s) {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            size_t best = free_.size();
+            for (size_t i = 0; i < free_.size(); ++i) {
+                const Blk &b = free_[i];
+                if (b.dev != dev || b.bytes < bytes || b.bytes > 2 * bytes + (size_t{64} << 20)) continue;
+                if (best == free_.size() || b.bytes < free_[best].bytes) best = i;
+            }
+            if (best != free_.size()) {
+                Blk b = free_[best];
+                free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(best));
+                if (b.ev) {
+                    if (b.stream != s) HIP_TRY(hipStreamWaitEvent(s, b.ev, 0));
+                    (void)hipEventDestroy(b.ev);
+                }
+                bytes = b.bytes;
+                return b.p;
+            }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            release_all();
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess)
+            throw HipFailure(std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+        return p;
+    }
+    void give(void *p, size_t bytes, hipStream_t s) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, s) != hipSuccess) {
+            if (ev) (void)hipEventDestroy(ev);
+            (void)hipStreamSynchronize(s);
+            ev = nullptr;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.push_back({p, bytes, dev, s, ev});
+    }
+    void release_all() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (Blk &b : free_) {
+            if (b.ev) {
+                (void)hipEventSynchronize(b.ev);
+                (void)hipEventDestroy(b.ev);
+            }
+            (void)hipFree(b.p);
+        }
+        free_.clear();
+    }
+
+   private:
+    struct Blk {
+        void *p;
+        size_t bytes;
+        int dev;
+        hipStream_t stream;
+        hipEvent_t ev;
+    };
+    std::mutex mu_;
+    std::vector<Blk> free_;
+};
+
+// A device array owned by one engine: allocations and frees are ordered on the engine's stream (the
+// null stream once the engine has synchronised and destroyed it).
 struct DevArray {
     void *p = nullptr;
     size_t bytes = 0;
+    const hipStream_t *sp = nullptr;
+    DevArray() = default;
+    explicit DevArray(const hipStream_t *s) : sp(s) {}
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
     ~DevArray() {
-        if (p) (void)hipFree(p);
+        if (p) BlockCache::get().give(p, bytes, stream());
     }
+    hipStream_t stream() const { return sp ? *sp : nullptr; }
     template <typename T>
     T *as() const {
         return static_cast<T *>(p);
@@ -1409,10 +1508,10 @@ struct DevArray {
         if (need <= bytes) return;
         size_t nb = std::max(need, bytes + bytes / 2);
         nb = (nb + 4095) & ~size_t{4095};
-        if (p) HIP_TRY(hipFree(p));
+        if (p) BlockCache::get().give(p, bytes, stream());
         p = nullptr;
         bytes = 0;
-        HIP_TRY(hipMalloc(&p, nb));
+        p = BlockCache::get().take(nb, stream());
         bytes = nb;
     }
 };
@@ -1428,10 +1527,11 @@ struct mpa_engine {
     hipStream_t stream = nullptr;
 
     uint64_t N = 0;     // live terms (exact whenever the host has synchronised)
-    uint64_t cap = 0;   // row capacity (multiple of 64 * kBlock)
-    DevArray rows, coef, cols;
-    DevArray AL, lead, foll, blk_a, blk_b, rec, lv, match, miss, ins, table, counters, stats, gdesc, fold, occ_dev,
-        partial;
+    uint64_t cap = 0;   // row capacity (a power of two, at least 64 * kBlock)
+    DevArray rows{&stream}, coef{&stream}, cols{&stream};
+    DevArray AL{&stream}, lead{&stream}, foll{&stream}, blk_a{&stream}, blk_b{&stream}, rec{&stream}, lv{&stream},
+        match{&stream}, miss{&stream}, ins{&stream}, table{&stream}, counters{&stream}, stats{&stream},
+        gdesc{&stream}, fold{&stream}, occ_dev{&stream}, partial{&stream};
 
     // Graph (build_graph): per-layer cos-only rows and rotation records, the coefficient vector the
     // seeded build evolves, the operator coefficients the graph starts from, and replay scratch.
@@ -1442,14 +1542,16 @@ struct mpa_engine {
     };
     std::vector<LayerInfo> layers;
     int64_t graph_nparams = 0;
-    DevArray g_cos, g_rot, seedc, base, rep, snap_c, snap_r, lam, gpartial, goffs, gdl;
+    DevArray g_cos{&stream}, g_rot{&stream}, seedc{&stream}, base{&stream}, rep{&stream}, snap_c{&stream},
+        snap_r{&stream}, lam{&stream}, gpartial{&stream}, goffs{&stream}, gdl{&stream};
     uint64_t base_n = 0;
     uint64_t gcos_total = 0, grot_total = 0;  // records stored
     bool building = false;                    // seedc tracks capacity while a graph is built
 
     // Fused pipeline state: global row index, the emitter list with pre-gate values (cap / 2 entries:
     // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
-    DevArray gidx, f_elead, f_vsrc, t_item, t_cnt, t_gsum, desc_m;
+    DevArray gidx{&stream}, f_elead{&stream}, f_vsrc{&stream}, t_item{&stream}, t_cnt{&stream}, t_bsum{&stream},
+        t_gsum{&stream}, desc_m{&stream};
     uint64_t gsum_stride = 0;  // per gate parity
     int group_log2 = kMinGroupLog2;
     uint64_t gidx_slots = 0;
@@ -1487,7 +1589,11 @@ struct mpa_engine {
         }
         for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
         if (h_counters) (void)hipHostFree(h_counters);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+            stream = nullptr;  // the arrays below go back to the block cache on the null stream
+        }
     }
 
     hipEvent_t ev_get() {
@@ -1558,11 +1664,12 @@ struct mpa_engine {
         }
     }
 
+    // Capacities are powers of two (at least 16384 rows), so every per-row array of a store, and of the
+    // next propagator, recurs at the same sizes and the memory pool hands back cached blocks.
     void grow_store(uint64_t need) {
-        uint64_t nc = std::max<uint64_t>(need, cap + cap / 2);
-        const uint64_t q = 64ull * kBlock;
-        nc = std::max<uint64_t>((nc + q - 1) / q * q, q);
-        DevArray r2, c2, k2;
+        uint64_t nc = 64ull * kBlock;
+        while (nc < need || nc < 2 * cap) nc <<= 1;
+        DevArray r2{&stream}, c2{&stream}, k2{&stream};
         r2.reserve(nc * nw * 8);
         c2.reserve(nc * 8);
         k2.reserve(static_cast<size_t>(ncols) * (nc / 64) * 8);
@@ -1582,7 +1689,7 @@ struct mpa_engine {
         std::swap(cols.p, k2.p);
         std::swap(cols.bytes, k2.bytes);
         if (building) {
-            DevArray s2;
+            DevArray s2{&stream};
             s2.reserve(nc * 8);
             if (N > 0) HIP_TRY(hipMemcpyAsync(s2.p, seedc.p, N * 8, hipMemcpyDeviceToDevice, stream));
             HIP_TRY(hipStreamSynchronize(stream));
@@ -1592,7 +1699,7 @@ struct mpa_engine {
         cap = nc;
         // per-term scratch sized by capacity; the item list of the current gate is kept
         if (AL.bytes < cap * 4) {
-            DevArray a2;
+            DevArray a2{&stream};
             a2.reserve(cap * 4);
             if (AL.p) HIP_TRY(hipMemcpyAsync(a2.p, AL.p, AL.bytes, hipMemcpyDeviceToDevice, stream));
             HIP_TRY(hipStreamSynchronize(stream));
@@ -1626,7 +1733,7 @@ struct mpa_engine {
     // Grow `a` to `need` bytes keeping its contents.
     void grow_keep(DevArray &a, size_t need) {
         if (need <= a.bytes) return;
-        DevArray b;
+        DevArray b{&stream};
         b.reserve(std::max(need, a.bytes + a.bytes / 2));
         if (a.p) HIP_TRY(hipMemcpyAsync(b.p, a.p, a.bytes, hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -1906,6 +2013,7 @@ struct mpa_engine {
             const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
             t_item.reserve(tiles * kWTileRows * 4);
             t_cnt.reserve(tiles * 8);
+            t_bsum.reserve(((tiles + kWaves - 1) / kWaves + 1) * 8);
             group_log2 = kMinGroupLog2;
             while (group_log2 < kMaxGroupLog2 && ((tiles + (uint64_t{1} << group_log2) - 1) >> group_log2) + 1 > kMaxGroups)
                 ++group_log2;
@@ -1943,7 +2051,7 @@ struct mpa_engine {
 
     // Algorithmic bytes of the fused kernels from the per-gate counters.  Sweep: fold + pivot column
     // words, leader masks, tile counts, the anticommuting coefficients (read + write), per emitter its
-    // row, pre-gate value and tile entry.  Resolve: group and tile counts, per emitter its tile entry,
+    // row, pre-gate value and tile entry.  Resolve: per emitter its tile entry,
     // pre-gate value, row, index slot (key inline) and coefficient writes; per hit the partner
     // coefficient (read + write); per appended row the row, coefficient, index slot and column bits.
     void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
@@ -1954,7 +2062,7 @@ struct mpa_engine {
             const double tiles = std::ceil(words / kWTileWords);
             const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], hits = stat_p[g];
             prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + tiles * 8 + nA * 16.0 + E * (8 * NW + 12);
-            prof_bytes[K_RESOLVE] += tiles * 8 + E * (4 + 8 + 8 * NW + 8 * (NW + 1) + 8) + hits * 16 +
+            prof_bytes[K_RESOLVE] += E * (4 + 8 + 8 * NW + 8 * (NW + 1) + 8) + hits * 16 +
                                      I * (8 * NW + 8 + 8 * (NW + 1) + ncols / 8.0);
         }
     }
@@ -2025,14 +2133,15 @@ struct mpa_engine {
                 hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), ig, tr,
                                    cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
                                    gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), f_vsrc.as<double>(),
-                                   t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), gsum_cur,
-                                   group_log2, counters.as<uint64_t>());
+                                   t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(),
+                                   t_bsum.as<uint64_t>(), gsum_cur, group_log2, counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
                 hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
-                                   n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), gsum_cur, gsum_nxt,
+                                   n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
+                                   gsum_cur, gsum_nxt,
                                    gsum_stride, f_vsrc.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
                                    static_cast<uint32_t>(gsum_stride), group_log2);
