@@ -9,10 +9,13 @@ Metric: term-gate applications per second, T / t with T = sum over gates of the 
 the gate (counted on the device); t covers engine creation, propagation and <O> for the three
 circuits (their generator lists are built once, before timing).  The roofline figure is the dominant
 kernel's algorithmic bytes over its HIP-event-timed device duration, from one extra step of the same
-workload run with the per-launch profiler on (kept out of the timed steps).  The CPU baseline is the oracle
+workload run with the per-launch profiler on (kept out of the timed steps).  `gradient` reports the
+other half of the metric, the full-angle gradient's time against one forward replay, on the 64-qubit
+ring brickwork (2304 angles), measured after the timed steps.  The CPU baseline is the oracle
 (a serial restatement of the reference algorithm) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--trotter-steps 20] [--no-cpu-baseline]
+       [--no-gradient]
 """
 
 from __future__ import annotations
@@ -42,6 +45,11 @@ def parse_args():
     ap.add_argument("--trotter-steps", type=int, default=20)
     ap.add_argument("--lattice", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run the known-size copy (counter scale)")
+    ap.add_argument("--traffic-from", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "FETCH", "WRITE"),
+                    help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
+    ap.add_argument("--traffic-out", default=str(TRAFFIC_FILE))
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     return ap.parse_args()
 
@@ -114,8 +122,103 @@ def cpu_baseline(L, sample_steps):
     }
 
 
+def gradient_measurement(reps: int = 3):
+    """64-qubit ring brickwork, 12 layers (2304 independent angles), O = mean ZZ, cutoff 8, lower_atol
+    1e-6: build the graph once, then time the replay (<O>) against replay + adjoint (<O>, d<O>/dtheta)."""
+    from monoprop_amd import workloads
+    from monoprop_amd._engine import DeviceSimulator
+
+    circ = workloads.brickwork_ring(64, 12, seed=0)
+    monos, gcs, pm, gidx = layers_of(circ)
+    theta = np.asarray(circ.parameters, dtype=np.float64)
+    sim = DeviceSimulator(workloads.ring_zz_mean(64).get_local_operator().terms, 8, [], lower_atol=1e-6,
+                          cutoff_type="support", logical_num_modes=64, basis="pauli")
+    t0 = time.perf_counter()
+    sim.build_graph(monos, pm, gcs, gidx, list(theta))
+    build_s = time.perf_counter() - t0
+    fwd, grd = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        sim.expectation_value(theta)
+        fwd.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        sim.expectation_value_and_gradient(theta)
+        grd.append(time.perf_counter() - t0)
+    f, g = min(fwd), min(grd)
+    cos_records, rot_records = sim.graph_size()
+    return {
+        "workload": "64-qubit ring brickwork, 12 layers, 2304 angles, O=mean ZZ, cutoff 8, lower_atol 1e-6",
+        "terms": sim.size(),
+        "graph_records": {"cos": cos_records, "rotation": rot_records},
+        "graph_build_ms": build_s * 1e3,
+        "forward_ms": f * 1e3,
+        "gradient_ms": g * 1e3,
+        "gradient_over_forward": g / f,
+    }
+
+
+COPY_BYTES = 256 << 20
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_kicked_ising_10x10.json"
+
+
+def copy_calibration():
+    """A plain device copy of COPY_BYTES, three times: the known-size reference for the HBM counters."""
+    import torch
+
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {COPY_BYTES} bytes x3")
+
+
+def traffic_from_counters(copy_fetch, copy_write, fetch, write, out_path=TRAFFIC_FILE):
+    """Per-launch HBM bytes of the fused kernels from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
+    (KiB per dispatch) over this benchmark, each scaled by what the counter reads for the copy above."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(path):
+        out = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            for k in ("k_sweep", "k_resolve", "copyBuffer"):
+                if k in r["Kernel_Name"]:
+                    out[k].append(float(r["Counter_Value"]) * 1024.0)
+        return out
+
+    cf, cw = per_kernel(copy_fetch)["copyBuffer"], per_kernel(copy_write)["copyBuffer"]
+    out = {"fetch_scale": COPY_BYTES / (sum(cf) / len(cf)), "write_scale": COPY_BYTES / (sum(cw) / len(cw)),
+           "kernels": {}}
+    fk, wk = per_kernel(fetch), per_kernel(write)
+    for k in ("k_sweep", "k_resolve"):
+        fv = [x for x in fk[k] if x > 0]  # a stopped device leaves a few empty launches behind
+        wv = [x for x in wk[k] if x > 0]
+        fb = out["fetch_scale"] * sum(fv) / max(len(fv), 1)
+        wb = out["write_scale"] * sum(wv) / max(len(wv), 1)
+        out["kernels"][k] = {"fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                             "traffic_bytes_per_launch": fb + wb, "launches": len(fv)}
+    Path(out_path).write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out, indent=1))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        return json.loads(TRAFFIC_FILE.read_text())["kernels"][kernel]["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     args = parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.traffic_from:
+        traffic_from_counters(*args.traffic_from, out_path=args.traffic_out)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the engine holds the whole operator in "
               "one GPU's HBM and multi-GPU sharding is not part of this build.", file=sys.stderr)
@@ -189,7 +292,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dname),
             "avg_launch_ms": avg_ms,
             "launches": d["launches"],
         },
@@ -201,6 +304,8 @@ def main():
         "all_kernels_GBps": kern_bytes / (kern_ms * 1e6) if kern_ms > 0 else 0.0,
         "profiled_step": {"propagate_ms": round(prof_wall_ms, 2), "kernel_ms": round(kern_ms, 2)},
     }
+    if not args.no_gradient:
+        line["gradient"] = gradient_measurement()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.lattice, args.cpu_sample_steps)
     print(json.dumps(line))
