@@ -122,6 +122,41 @@ int mpa_last_propagate_ms(mpa_engine *e, double *ms);
 int mpa_set_truncation(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower_atol,
                        double upper_atol);
 
+/* Batched propagation.  One engine holds `n_members` independent propagations of one layer sequence:
+ * member q starts from terms [member_offsets[q], member_offsets[q + 1]) of the CSR term list and is
+ * propagated at its own parameter row.  It replaces n_members reference MonomialPropagator objects
+ * built with the same truncation and initial state (MonomialPropagator.inl:73-188, binder.h:60-100),
+ * each propagated through the same layers (MonomialPropagator.inl:692-733, binder.h:118-125) and
+ * read with expectation_value (MPFunctions.cpp:135-148,196-205).  Member q's rows, in store order,
+ * are the rows its own propagator would hold, with identical coefficients.  1 <= n_members <= 2048.
+ * The entry points above that take a lone propagator (propagate, build_graph, expectation_value*,
+ * contract, export_terms) refuse a batch engine with -1; mpa_size, mpa_read_rows, mpa_gate_stats
+ * and mpa_last_propagate_ms report the whole store. */
+int mpa_batch_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type,
+                     double lower_atol, double upper_atol, int32_t n_members,
+                     const int64_t *member_offsets, const int64_t *term_offsets,
+                     const int32_t *term_slots, const double *coeff_re, const double *coeff_im,
+                     const int32_t *initial_state, int64_t n_state, int device);
+
+/* params: n_members rows of n_params values (row-major); alpha of member q at layer i is
+ * params[q * n_params + param_map[i]] * gen_coeffs[i].  Runs the fused pipeline for every member
+ * (-2 if some member has cos 2a == 0 at a gate: propagate that member alone). */
+int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots,
+                        int64_t n_layers, const int64_t *param_map, const double *gen_coeffs,
+                        const double *params, int64_t n_params);
+
+/* Number of members. */
+int mpa_batch_members(mpa_engine *e, int32_t *out);
+
+/* <ref| O_q |ref> of every member into out[n_members] (each summed in a fixed order). */
+int mpa_batch_expectation_values(mpa_engine *e, double *out);
+
+/* Stored terms of every member into out[n_members]. */
+int mpa_batch_sizes(mpa_engine *e, int64_t *out);
+
+/* Member of every stored row (mpa_read_rows order) into out[size]. */
+int mpa_batch_row_members(mpa_engine *e, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,13 @@ _SIGNATURES = {
     "mpa_kernel_profile": (C.c_int, [_P, C.c_int, C.POINTER(C.c_char_p), _f64p, _f64p, _i64p]),
     "mpa_last_propagate_ms": (C.c_int, [_P, _f64p]),
     "mpa_set_truncation": (C.c_int, [_P, C.c_uint32, C.c_int, C.c_double, C.c_double]),
+    "mpa_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double, C.c_double,
+                                   C.c_int32, _i64p, _i64p, _i32p, _f64p, _f64p, _i32p, C.c_int64, C.c_int]),
+    "mpa_batch_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
+    "mpa_batch_members": (C.c_int, [_P, _i32p]),
+    "mpa_batch_expectation_values": (C.c_int, [_P, _f64p]),
+    "mpa_batch_sizes": (C.c_int, [_P, _i64p]),
+    "mpa_batch_row_members": (C.c_int, [_P, _i32p]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
@@ -437,3 +444,120 @@ def dispatch(num_modes: int):
         return DeviceSimulator(**kwargs)
 
     return make
+
+
+class BatchSimulator:
+    """Many independent propagations of one gate sequence in one device store.
+
+    Member ``q`` is the propagation of ``operators[q]`` at parameter row ``q``; all members share the
+    truncation settings, the initial state and the layer structure.  It stands for ``len(operators)``
+    reference ``MonomialPropagatorNNN`` objects (binder.h:53-274) constructed with the same keywords and
+    each given the same ``propagate`` call at its own parameters.
+    """
+
+    max_members = 2048
+
+    def __init__(self, operators: Sequence[Mapping[Sequence[int], complex]], cutoff: int, initial_state: Sequence[int],
+                 *, lower_atol: float | None = None, upper_atol: float | None = None, cutoff_type: str = "length",
+                 logical_num_modes: int, basis: str = "majorana") -> None:
+        lib = load_library()
+        if basis not in ("majorana", "pauli"):
+            raise ValueError(f"Unknown basis {basis!r}; expected 'majorana' or 'pauli'.")
+        if cutoff_type not in ("length", "support"):
+            raise ValueError(f"Unknown cutoff type {cutoff_type!r}; expected 'length' or 'support'.")
+        if isinstance(cutoff, bool) or int(cutoff) < 0:
+            raise ValueError(f"cutoff must be a non-negative integer; got {cutoff!r}.")
+        if not 1 <= len(operators) <= self.max_members:
+            raise ValueError(f"a batch holds 1 to {self.max_members} operators, got {len(operators)}.")
+        keys: list[tuple[int, ...]] = []
+        vals: list[complex] = []
+        moffs = np.zeros(len(operators) + 1, dtype=np.int64)
+        for q, op in enumerate(operators):
+            for k in op:
+                keys.append(tuple(int(i) for i in k))
+                vals.append(complex(op[k]))
+            moffs[q + 1] = len(keys)
+        offs, slots = _csr(keys)
+        re = _f64(v.real for v in vals)
+        im = _f64(v.imag for v in vals)
+        st = np.asarray([int(i) for i in initial_state] or [0], dtype=np.int32)
+        self._basis = basis
+        self._num_modes = int(logical_num_modes)
+        self._members = len(operators)
+        h = _P()
+        _check(lib.mpa_batch_create(C.byref(h), 1 if basis == "pauli" else 0, self._num_modes, int(cutoff),
+                                    0 if cutoff_type == "length" else 1, _nan_if_none(lower_atol),
+                                    _nan_if_none(upper_atol), len(operators), _ptr(moffs, C.c_int64),
+                                    _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), _ptr(re, C.c_double),
+                                    _ptr(im, C.c_double), _ptr(st, C.c_int32), len(list(initial_state)),
+                                    default_device()))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _LIB is not None:
+            _LIB.mpa_destroy(h)
+            self._h = None
+
+    @property
+    def members(self) -> int:
+        return self._members
+
+    def propagate(self, majoranas, parameter_mapping, gen_coeffs, parameters) -> None:
+        """``parameters``: one row per member (shape ``(members, n_params)``)."""
+        if not len(majoranas):
+            return
+        offs, slots, pm, gc = DeviceSimulator._layers(majoranas, parameter_mapping, gen_coeffs)
+        th = np.ascontiguousarray(np.asarray(parameters, dtype=np.float64))
+        if th.ndim != 2 or th.shape[0] != self._members:
+            raise ValueError(f"parameters must have one row per member ({self._members}), got shape {th.shape}.")
+        need = (max(int(p) for p in parameter_mapping) + 1) if len(parameter_mapping) else 0
+        if th.shape[1] < need:
+            raise ValueError(f"Expected at least {need} parameter value(s) per member, got {th.shape[1]}.")
+        flat = th.reshape(-1) if th.size else np.zeros(1, dtype=np.float64)
+        _check(load_library().mpa_batch_propagate(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
+                                                  len(majoranas), _ptr(pm, C.c_int64), _ptr(gc, C.c_double),
+                                                  _ptr(flat, C.c_double), th.shape[1]))
+
+    def expectation_values(self) -> np.ndarray:
+        out = np.zeros(self._members, dtype=np.float64)
+        _check(load_library().mpa_batch_expectation_values(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def sizes(self) -> np.ndarray:
+        out = np.zeros(self._members, dtype=np.int64)
+        _check(load_library().mpa_batch_sizes(self._h, _ptr(out, C.c_int64)))
+        return out
+
+    def size(self) -> int:
+        v = C.c_int64()
+        _check(load_library().mpa_size(self._h, C.byref(v)))
+        return v.value
+
+    def rows(self):
+        """Raw store ``(words[N, nw], coeffs[N], member[N])`` in insertion order."""
+        lib = load_library()
+        nw = C.c_int32()
+        _check(lib.mpa_words_per_row(self._h, C.byref(nw)))
+        n = self.size()
+        w = np.zeros((max(n, 1), nw.value), dtype=np.uint64)
+        c = np.zeros(max(n, 1), dtype=np.float64)
+        m = np.zeros(max(n, 1), dtype=np.int32)
+        _check(lib.mpa_read_rows(self._h, _ptr(w, C.c_uint64), _ptr(c, C.c_double)))
+        _check(lib.mpa_batch_row_members(self._h, _ptr(m, C.c_int32)))
+        return w[:n], c[:n], m[:n]
+
+    def gate_stats(self):
+        """Per processed gate, summed over members: (live terms before, anticommuting, emitted, inserted)."""
+        lib = load_library()
+        ln = C.c_int64(0)
+        _check(lib.mpa_gate_stats(self._h, C.byref(ln), None, None, None, None))
+        arrs = [np.zeros(max(ln.value, 1), dtype=np.int64) for _ in range(4)]
+        _check(lib.mpa_gate_stats(self._h, C.byref(ln), *[_ptr(a, C.c_int64) for a in arrs]))
+        return tuple(a[: ln.value] for a in arrs)
+
+    def last_propagate_ms(self) -> float:
+        v = C.c_double()
+        _check(load_library().mpa_last_propagate_ms(self._h, C.byref(v)))
+        return v.value
+

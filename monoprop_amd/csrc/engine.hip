@@ -179,6 +179,12 @@ struct Trunc {
     int32_t only_rot;  // -1: none
 };
 
+// Rotation constants of one (gate, batch member) of a batched propagation: the member's own angle
+// alpha = theta[param_map] * g gives k = cos 2a, s = sin 2a, inv_k = 1 / cos 2a, abs_s = |sin 2a|.
+struct Trig {
+    double k, s, inv_k, abs_s;
+};
+
 // ------------------------------------------------------------------------------------------------
 // kernels
 
@@ -641,17 +647,28 @@ __global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__
 // Fused contract-immediately pipeline (the default propagate: no orbital cap, cos != 0).
 //
 // A global open-addressing index over every stored row answers "is M^G stored?" for the emitting
-// terms only.  Slot = NW + 1 words: tag (fingerprint << 32) | (row + 1), 0 = empty, then the key
-// itself, so a probe reads one slot and needs no second access to the store.  The tag is claimed
-// first and the key written after; lookups skip rows appended during the current gate (they are
-// nobody's partner), so a key still being written is never compared.
+// terms only.  Slot = NW + 1 words: tag (member << 48) | (fingerprint << 32) | (row + 1), 0 = empty,
+// then the key itself, so a probe reads one slot and needs no second access to the store.  `member`
+// is the batch member owning the row (0 for a lone propagator): it enters the slot position and the
+// tag, so equal monomials of different members are different keys.  The tag is claimed first and the
+// key written after; lookups skip rows appended during the current gate (they are nobody's partner),
+// so a key still being written is never compared.
+
+template <int NW>
+__device__ __forceinline__ uint64_t gidx_hash(const uint64_t *key, uint32_t member) {
+    return mpa::hash_key<NW>(key) ^ mpa::mix64(static_cast<uint64_t>(member) * 0x9e3779b97f4a7c15ull);
+}
+
+__device__ __forceinline__ uint64_t gidx_tag_hi(uint64_t h, uint32_t member) {
+    return (static_cast<uint64_t>(member) << 48) | (h & 0x0000ffff00000000ull);
+}
 
 template <int NW>
 __device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_t gmask, const uint64_t *key,
-                                            uint32_t row) {
+                                            uint32_t row, uint32_t member) {
     constexpr int SW = NW + 1;
-    const uint64_t h = mpa::hash_key<NW>(key);
-    const uint64_t val = (h & 0xffffffff00000000ull) | (static_cast<uint64_t>(row) + 1);
+    const uint64_t h = gidx_hash<NW>(key, member);
+    const uint64_t val = gidx_tag_hi(h, member) | (static_cast<uint64_t>(row) + 1);
     uint64_t pos = h & gmask;
     for (uint64_t probe = 0; probe <= gmask; ++probe) {
         uint64_t *slot = gidx + pos * SW;
@@ -666,13 +683,13 @@ __device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_
     return false;
 }
 
-// Row of `key` among rows [0, limit), or -1.
+// Row of (`key`, `member`) among rows [0, limit), or -1.
 template <int NW>
 __device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gmask,
-                                                   const uint64_t *key, uint64_t limit) {
+                                                   const uint64_t *key, uint64_t limit, uint32_t member) {
     constexpr int SW = NW + 1;
-    const uint64_t h = mpa::hash_key<NW>(key);
-    const uint64_t fp = h & 0xffffffff00000000ull;
+    const uint64_t h = gidx_hash<NW>(key, member);
+    const uint64_t fp = gidx_tag_hi(h, member);
     uint64_t pos = h & gmask;
     for (uint64_t probe = 0; probe <= gmask; ++probe) {
         const uint64_t *slot = gidx + pos * SW;
@@ -692,16 +709,18 @@ __device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ 
     return -1;
 }
 
-// Index rows [start, start + count).
+// Index rows [start, start + count); `member` of row r is member[r] (null: every row is member 0).
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_gidx_build(const uint64_t *__restrict__ rows, uint64_t start,
-                                                       uint64_t count, uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                       uint64_t count, const uint16_t *__restrict__ member,
+                                                       uint64_t *__restrict__ gidx, uint64_t gmask,
                                                        uint64_t *__restrict__ counters) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= count) return;
     uint64_t m[NW];
     load_row<NW>(rows, static_cast<uint32_t>(start + i), m);
-    if (!gidx_insert<NW>(gidx, gmask, m, static_cast<uint32_t>(start + i)))
+    const uint32_t q = member ? member[start + i] : 0u;
+    if (!gidx_insert<NW>(gidx, gmask, m, static_cast<uint32_t>(start + i), q))
         atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
 }
 
@@ -864,14 +883,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 //   D  lane j lists word j's emitters into the tile's region [1024 tile, ...): leaders ascending, then
 //      followers ascending (pivot clear / set); the tile's counts, also summed per group of tiles;
 //      leader emit masks per word
-template <int NW, int PAULI>
+//   BATCH  rows belong to batch members (member[row]); each term takes its member's k and |s|
+//          (gtrig[member]) instead of the gate's
+template <int NW, int PAULI, int BATCH>
 __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
                                                uint64_t colstride, const int32_t *__restrict__ fold, int nfold,
                                                int piv0, int piv1, const uint64_t *__restrict__ rows,
                                                double *__restrict__ coef, double *__restrict__ vsrc,
                                                uint32_t *__restrict__ t_item, uint64_t *__restrict__ elead,
                                                uint64_t *__restrict__ tcnt, uint64_t N, uint64_t tile,
-                                               uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx) {
+                                               uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx,
+                                               const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig) {
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t w0 = tile * kWTileWords;
     const uint32_t lane = lane_id();
@@ -923,13 +945,16 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     uint32_t cand = 0, resc = 0;  // candidates; those the upper_atol rescue emits regardless
     {
         double v[kWTileWords];
+        uint32_t q[BATCH ? kWTileWords : 1];
         uint32_t act = 0;
 #pragma unroll
         for (int j = 0; j < kWTileWords; ++j) {
             const uint64_t aj = __shfl(a, j, 64);
             v[j] = 0.0;
+            if (BATCH) q[j] = 0;
             if ((aj >> lane) & 1u) {
                 v[j] = coef[row0 + static_cast<uint64_t>(j) * 64 + lane];
+                if (BATCH) q[j] = member[row0 + static_cast<uint64_t>(j) * 64 + lane];
                 act |= 1u << j;
             }
         }
@@ -937,11 +962,13 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
         for (int j = 0; j < kWTileWords; ++j) {
             if (!((act >> j) & 1u)) continue;
             const uint64_t r = row0 + static_cast<uint64_t>(j) * 64 + lane;
-            coef[r] = v[j] * gd.k;
-            if (!(tr.check_lower && gd.abs_s * fabs(v[j]) <= tr.lower)) {
+            const double k = BATCH ? gtrig[q[BATCH ? j : 0]].k : gd.k;
+            const double abs_s = BATCH ? gtrig[q[BATCH ? j : 0]].abs_s : gd.abs_s;
+            coef[r] = v[j] * k;
+            if (!(tr.check_lower && abs_s * fabs(v[j]) <= tr.lower)) {
                 cand |= 1u << j;
                 vsrc[r] = v[j];
-                if (tr.check_upper && gd.abs_s * fabs(v[j]) >= tr.upper) resc |= 1u << j;
+                if (tr.check_upper && abs_s * fabs(v[j]) >= tr.upper) resc |= 1u << j;
             }
         }
     }
@@ -1003,7 +1030,7 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
 
 // One 1024-term tile per wavefront, four per block; the block stores its tiles' summed counts
 // (bsum[block]) and adds them into their group word with one atomic.
-template <int NW, int PAULI>
+template <int NW, int PAULI, int BATCH>
 __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
                                                   const uint64_t *__restrict__ cols, uint64_t colstride,
                                                   const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
@@ -1011,7 +1038,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
                                                   double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
                                                   uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
                                                   uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
-                                                  int group_log2, uint64_t *__restrict__ counters) {
+                                                  int group_log2, uint64_t *__restrict__ counters,
+                                                  const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
@@ -1025,8 +1053,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
     const uint64_t tile = tile0 + wave;
     uint64_t grp = 0;
     if (tile * kWTileWords < nwords)
-        grp = sweep_tile<NW, PAULI>(gdesc[gi], tr, cols, colstride, fold, nfold, piv0, piv1, rows, coef, vsrc, t_item,
-                                    elead, tcnt, N, tile, sem[wave], cidx[wave]);
+        grp = sweep_tile<NW, PAULI, BATCH>(gdesc[gi], tr, cols, colstride, fold, nfold, piv0, piv1, rows, coef, vsrc,
+                                           t_item, elead, tcnt, N, tile, sem[wave], cidx[wave], member, gtrig);
     if (lane_id() == 0) s_grp[wave] = grp;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1053,7 +1081,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
 // The chunk holding the last emitter publishes N + I for the next gate and stops the device before a
 // gate whose worst case could outgrow the store or the index.  Every block also clears its share of
 // the other gate parity's group counts.
-template <int NW, int PAULI>
+// With BATCH, an emitter's member (member[S]) selects its rotation constants (gtrig[member]) and keys its
+// index lookup; the partner it creates joins the same member.
+template <int NW, int PAULI, int BATCH>
 __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
@@ -1065,7 +1095,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
                                                     uint64_t cap_rows, uint64_t gidx_slots, uint32_t max_groups,
-                                                    int group_log2) {
+                                                    int group_log2, uint16_t *__restrict__ member,
+                                                    const Trig *__restrict__ gtrig) {
     constexpr int NC = max_cols<NW, PAULI>();
     constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
     __shared__ uint64_t lcol[kColWords * NC];
@@ -1195,27 +1226,31 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
         // ---- resolution
         uint64_t t[NW];
         double ins = 0.0;
-        uint32_t miss = 0, hit = 0;
+        uint32_t miss = 0, hit = 0, q = 0;
         if (valid) {
             const double vS = vsrc[S];
+            if (BATCH) q = member[S];
             uint64_t m[NW];
             load_row<NW>(rows, S, m);
 #pragma unroll
             for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-            const int64_t T = gidx_find_below<NW>(gidx, gmask, t, N);
+            const int64_t T = gidx_find_below<NW>(gidx, gmask, t, N, q);
             const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
             if (!skip) {
+                const double gk = BATCH ? gtrig[q].k : gd.k;
+                const double gs = BATCH ? gtrig[q].s : gd.s;
+                const double ginv = BATCH ? gtrig[q].inv_k : gd.inv_k;
                 const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
-                const double wS = vS * gd.k;
+                const double wS = vS * gk;
                 if (T >= 0) {
                     const double wT = coef[T];
-                    const double vTu = wT * gd.inv_k;
-                    coef[S] = wS + gd.s * static_cast<double>(-ph) * vTu;
-                    coef[T] = wT + gd.s * static_cast<double>(ph) * vS;
+                    const double vTu = wT * ginv;
+                    coef[S] = wS + gs * static_cast<double>(-ph) * vTu;
+                    coef[T] = wT + gs * static_cast<double>(ph) * vS;
                     hit = 1;
                 } else {
-                    coef[S] = wS + gd.s * static_cast<double>(-ph) * 0.0;
-                    ins = gd.k * 0.0 + gd.s * static_cast<double>(ph) * vS;
+                    coef[S] = wS + gs * static_cast<double>(-ph) * 0.0;
+                    ins = gk * 0.0 + gs * static_cast<double>(ph) * vS;
                     miss = 1;
                 }
             }
@@ -1252,8 +1287,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
 #pragma unroll
             for (int k = 0; k < NW; ++k) pd[k] = t[k];
             coef[dest] = ins;
+            if (BATCH) member[dest] = static_cast<uint16_t>(q);
             gather_column_bits<NW, PAULI>(lcol, ncols, n_modes, t, (dest >> 6) - wlo, uint64_t{1} << (dest & 63));
-            if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest)))
+            if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest), q))
                 atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
         }
         __syncthreads();
@@ -1386,6 +1422,87 @@ __global__ __launch_bounds__(kBlock) void k_init_lambda(const uint64_t *__restri
 }
 
 // Per-layer sum of the adjoint partials, in block order: one block per layer.
+// Batched <O>: per member, the sum over its fully paired rows of phase * c.  Each wavefront keeps one
+// accumulator per member in LDS and walks a fixed set of 64-row windows; inside a window the lanes of
+// one member are summed by a fixed butterfly and added by a single lane, so the result does not depend
+// on scheduling.  partial[block * nb + member] = the block's four wave sums in wave order.
+constexpr uint32_t kMaxBatch = 2048;  // members per batch: kWaves * kMaxBatch doubles of LDS per block
+
+template <int NW, int PAULI>
+__global__ __launch_bounds__(kBlock) void k_expect_batch(const uint64_t *__restrict__ rows,
+                                                         const double *__restrict__ coef,
+                                                         const uint16_t *__restrict__ member, uint64_t N,
+                                                         uint32_t nb, const uint64_t *__restrict__ occ_dev,
+                                                         double *__restrict__ partial) {
+    extern __shared__ double acc[];  // [kWaves][nb]
+    for (uint32_t i = threadIdx.x; i < kWaves * nb; i += kBlock) acc[i] = 0.0;
+    __syncthreads();
+    uint64_t occ[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    double *mine = acc + static_cast<uint64_t>(wave) * nb;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock + wave * 64; base < N; base += stride) {
+        const uint64_t i = base + lane;
+        bool have = false;
+        double v = 0.0;
+        uint32_t q = 0;
+        if (i < N) {
+            uint64_t m[NW];
+            load_row<NW>(rows, static_cast<uint32_t>(i), m);
+            if (mpa::is_paired<NW, PAULI>(m)) {
+                v = mpa::state_phase<NW, PAULI>(m, occ) * coef[i];
+                q = member[i];
+                have = true;
+            }
+        }
+        uint64_t pending = __ballot(have);
+        while (pending) {
+            const int lead = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+            const uint32_t ql = static_cast<uint32_t>(__shfl(static_cast<int>(q), lead, 64));
+            const bool in = have && q == ql;
+            const uint64_t group = __ballot(in);
+            double x = in ? v : 0.0;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+            if (static_cast<int>(lane) == lead) mine[ql] += x;
+            pending &= ~group;
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nb; q += kBlock) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) sum += acc[static_cast<uint64_t>(w) * nb + q];
+        partial[static_cast<uint64_t>(blockIdx.x) * nb + q] = sum;
+    }
+}
+
+// out[q] = sum over blocks b (in order) of partial[b * nb + q].
+__global__ __launch_bounds__(kBlock) void k_reduce_members(const double *__restrict__ partial, uint32_t nblocks,
+                                                           uint32_t nb, double *__restrict__ out) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= nb) return;
+    double sum = 0.0;
+    for (uint32_t b = 0; b < nblocks; ++b) sum += partial[static_cast<uint64_t>(b) * nb + q];
+    out[q] = sum;
+}
+
+// Stored rows per batch member (counts[nb], zeroed by the caller).
+__global__ __launch_bounds__(kBlock) void k_count_members(const uint16_t *__restrict__ member, uint64_t N, uint32_t nb,
+                                                          unsigned long long *__restrict__ counts) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) hist[i] = 0;
+    __syncthreads();
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock)
+        atomicAdd(&hist[member[i]], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
+        if (hist[i]) atomicAdd(&counts[i], static_cast<unsigned long long>(hist[i]));
+}
+
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
                                                             const uint64_t *__restrict__ offs,
                                                             double *__restrict__ out) {
@@ -1565,6 +1682,13 @@ struct mpa_engine {
     uint64_t epoch = 0;
     int64_t *h_counters = nullptr;  // pinned
 
+    // Batched propagation: `nbatch` independent members (0: a lone propagator) share the store;
+    // bmember[row] is the member of a row, btrig[gate * nbatch + member] its rotation constants, bcore
+    // the members' identity coefficients.
+    uint32_t nbatch = 0;
+    std::vector<double> bcore;
+    DevArray bmember{&stream}, btrig{&stream}, bpartial{&stream}, bout{&stream};
+
     std::vector<double> init_coef;  // initial coefficients of rows [0, n_init)
     uint64_t n_init = 0;
     std::vector<int64_t> stat_n, stat_a, stat_e, stat_i, stat_l, stat_p;
@@ -1688,6 +1812,14 @@ struct mpa_engine {
         std::swap(coef.bytes, c2.bytes);
         std::swap(cols.p, k2.p);
         std::swap(cols.bytes, k2.bytes);
+        if (nbatch) {
+            DevArray b2{&stream};
+            b2.reserve(nc * 2);
+            if (N > 0) HIP_TRY(hipMemcpyAsync(b2.p, bmember.p, N * 2, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            std::swap(bmember.p, b2.p);
+            std::swap(bmember.bytes, b2.bytes);
+        }
         if (building) {
             DevArray s2{&stream};
             s2.reserve(nc * 8);
@@ -2006,7 +2138,7 @@ struct mpa_engine {
 
     // Arrays of the fused pipeline sized to the store (a gate runs only while N <= cap / 2), and a global
     // index with >= 2 slots per row of capacity covering every stored row.
-    template <int NW>
+    template <int NW, int BATCH>
     void ensure_fused_arrays() {
         if (fused_cap != cap) {
             cap_items = cap / 2;
@@ -2041,8 +2173,9 @@ struct mpa_engine {
             HIP_TRY(hipMemsetAsync(gidx.p, 0, gidx_slots * (nw + 1) * 8, stream));
             if (N) {
                 hipLaunchKernelGGL((k_gidx_build<NW>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)),
-                                   dim3(kBlock), 0, stream, rows.as<uint64_t>(), 0, N, gidx.as<uint64_t>(),
-                                   gidx_slots - 1, counters.as<uint64_t>());
+                                   dim3(kBlock), 0, stream, rows.as<uint64_t>(), 0, N,
+                                   BATCH ? bmember.as<uint16_t>() : nullptr, gidx.as<uint64_t>(), gidx_slots - 1,
+                                   counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
             }
             gidx_rows = N;
@@ -2085,7 +2218,7 @@ struct mpa_engine {
     // Contract-immediately propagation without host synchronisation between gates: two persistent
     // launches per gate, enqueued in chunks; the device stops before a gate whose worst case could
     // outgrow the store, the host grows it and resumes there.
-    template <int NW, int PAULI>
+    template <int NW, int PAULI, int BATCH>
     void propagate_fused(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds) {
         const size_t ng = gds.size();
         gdesc.reserve(std::max<size_t>(ng, 1) * sizeof(GateDesc));
@@ -2106,7 +2239,7 @@ struct mpa_engine {
         size_t gi = 0;
         while (gi < ng) {
             if (2 * N > cap) ensure_capacity(std::max<uint64_t>(3 * N, 64ull * kBlock));
-            ensure_fused_arrays<NW>();
+            ensure_fused_arrays<NW, BATCH>();
             const size_t end = std::min(ng, gi + kChunk);
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
             const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
@@ -2129,22 +2262,25 @@ struct mpa_engine {
                 const uint32_t ep = ++desc_epoch;
                 uint64_t *gsum_cur = t_gsum.as<uint64_t>() + (g & 1) * gsum_stride;
                 uint64_t *gsum_nxt = t_gsum.as<uint64_t>() + ((g + 1) & 1) * gsum_stride;
+                uint16_t *mem = BATCH ? bmember.as<uint16_t>() : nullptr;
+                const Trig *gtrig = BATCH ? btrig.as<Trig>() + g * nbatch : nullptr;
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), ig, tr,
-                                   cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off, gd.fold_cnt,
-                                   gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(), f_vsrc.as<double>(),
-                                   t_item.as<uint32_t>(), f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(),
-                                   t_bsum.as<uint64_t>(), gsum_cur, group_log2, counters.as<uint64_t>());
+                hipLaunchKernelGGL((k_sweep<NW, PAULI, BATCH>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
+                                   ig, tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off,
+                                   gd.fold_cnt, gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(),
+                                   f_vsrc.as<double>(), t_item.as<uint32_t>(), f_elead.as<uint64_t>(),
+                                   t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur, group_log2,
+                                   counters.as<uint64_t>(), mem, gtrig);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_resolve<NW, PAULI>), dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
+                hipLaunchKernelGGL((k_resolve<NW, PAULI, BATCH>), dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
                                    gsum_stride, f_vsrc.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
-                                   static_cast<uint32_t>(gsum_stride), group_log2);
+                                   static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
             }
@@ -2183,6 +2319,40 @@ struct mpa_engine {
             stat_p[g] = st[kStatW * g + 5];
         }
         if (prof_on) add_algorithmic_bytes_fused(fold_plus_piv);
+    }
+
+    // <O> of every batch member (its identity coefficient plus its paired rows), into out[nbatch].
+    template <int NW, int PAULI>
+    void expect_batch(double *out) {
+        bpartial.reserve(static_cast<size_t>(kExpectBlocks) * nbatch * 8);
+        bout.reserve(static_cast<size_t>(nbatch) * 8);
+        hipLaunchKernelGGL((k_expect_batch<NW, PAULI>), dim3(kExpectBlocks), dim3(kBlock),
+                           static_cast<size_t>(kWaves) * nbatch * 8, stream, rows.as<uint64_t>(), coef.as<double>(),
+                           bmember.as<uint16_t>(), N, nbatch, occ_dev.as<uint64_t>(), bpartial.as<double>());
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_members, dim3((nbatch + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                           bpartial.as<double>(), static_cast<uint32_t>(kExpectBlocks), nbatch, bout.as<double>());
+        HIP_TRY(hipGetLastError());
+        std::vector<double> h(nbatch);
+        HIP_TRY(hipMemcpyAsync(h.data(), bout.p, nbatch * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (uint32_t q = 0; q < nbatch; ++q) out[q] = bcore[q] + h[q];
+    }
+
+    // Stored rows of every batch member, into out[nbatch].
+    void member_sizes(int64_t *out) {
+        bout.reserve(static_cast<size_t>(nbatch) * 8);
+        HIP_TRY(hipMemsetAsync(bout.p, 0, static_cast<size_t>(nbatch) * 8, stream));
+        if (N) {
+            const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((N + kBlock - 1) / kBlock, kExpectBlocks));
+            hipLaunchKernelGGL(k_count_members, dim3(grid), dim3(kBlock), static_cast<size_t>(nbatch) * 4, stream,
+                               bmember.as<uint16_t>(), N, nbatch, bout.as<unsigned long long>());
+            HIP_TRY(hipGetLastError());
+        }
+        std::vector<uint64_t> h(nbatch);
+        HIP_TRY(hipMemcpyAsync(h.data(), bout.p, nbatch * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (uint32_t q = 0; q < nbatch; ++q) out[q] = static_cast<int64_t>(h[q]);
     }
 
     template <int NW, int PAULI>
@@ -2339,11 +2509,30 @@ struct RunPropagate {
     template <int NW, int PAULI>
     void operator()() {
         if (fused) {
-            e->propagate_fused<NW, PAULI>(gds, folds);
+            e->propagate_fused<NW, PAULI, 0>(gds, folds);
         } else {
             e->gidx_rows = static_cast<uint64_t>(-1);  // the unfused pipeline does not maintain the index
             e->propagate_impl<NW, PAULI>(gds, folds, graph, trl);
         }
+    }
+};
+
+struct RunBatch {
+    mpa_engine *e;
+    const std::vector<GateDesc> &gds;
+    const std::vector<int32_t> &folds;
+    template <int NW, int PAULI>
+    void operator()() {
+        e->propagate_fused<NW, PAULI, 1>(gds, folds);
+    }
+};
+
+struct RunExpectBatch {
+    mpa_engine *e;
+    double *out;
+    template <int NW, int PAULI>
+    void operator()() {
+        e->expect_batch<NW, PAULI>(out);
     }
 };
 
@@ -2439,6 +2628,107 @@ void set_trunc(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower, do
     e->tr.upper = is_none(upper) ? 0.0 : upper;
 }
 
+// A fresh engine with its stream, truncation and initial state; no terms yet.
+std::unique_ptr<mpa_engine> new_engine(int basis, int num_modes, uint32_t cutoff, int cutoff_type, double lower_atol,
+                                       double upper_atol, const int32_t *initial_state, int64_t n_state, int device) {
+    if (basis != MPA_BASIS_PAULI && basis != MPA_BASIS_MAJORANA) throw ArgError("unknown basis");
+    if (num_modes <= 0 || num_modes > 256)
+        throw RunError("logical_num_modes (" + std::to_string(num_modes) + ") must be in the range [1, 256].");
+    auto e = std::make_unique<mpa_engine>();
+    e->basis = basis;
+    e->n_modes = num_modes;
+    e->L = (num_modes + 63) / 64;
+    e->nw = basis == MPA_BASIS_PAULI ? 2 * e->L : (2 * num_modes + 63) / 64;
+    e->ncols = basis == MPA_BASIS_PAULI ? 2 * num_modes : 2 * num_modes + 1;
+    set_trunc(e.get(), cutoff, cutoff_type, lower_atol, upper_atol);
+    e->tr.only_rot = -1;
+    e->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&e->h_counters), C_COUNT * 8, hipHostMallocDefault));
+    e->occ.assign(mpa::kMaxWords, 0);
+    for (int64_t k = 0; k < n_state; ++k) {
+        const int j = initial_state[k];
+        if (j < 0 || j >= num_modes) throw ArgError("initial_state index out of range");
+        if (basis == MPA_BASIS_PAULI) e->occ[j >> 6] |= uint64_t{1} << (j & 63);
+        else e->occ[(2 * j) >> 6] |= uint64_t{1} << ((2 * j) & 63);
+    }
+    return e;
+}
+
+// Terms [t0, t1) of a CSR operator, in lexicographic slot-tuple order (std::map<VecZ, complex>
+// iteration), encoded and appended to rows / coefs; the empty tuple's coefficient goes to *core.
+// Distinct sorted slot tuples give distinct rows, so no key repeats.
+void encode_operator(const mpa_engine *e, const int64_t *term_offsets, const int32_t *term_slots,
+                     const double *coeff_re, const double *coeff_im, int64_t t0, int64_t t1,
+                     std::vector<uint64_t> &rows, std::vector<double> &coefs, double *core) {
+    std::vector<int64_t> order;
+    order.reserve(static_cast<size_t>(t1 - t0));
+    for (int64_t t = t0; t < t1; ++t) order.push_back(t);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return std::lexicographical_compare(term_slots + term_offsets[a], term_slots + term_offsets[a + 1],
+                                            term_slots + term_offsets[b], term_slots + term_offsets[b + 1]);
+    });
+    std::vector<uint64_t> m(e->nw);
+    for (int64_t t : order) {
+        const int64_t n = term_offsets[t + 1] - term_offsets[t];
+        e->slots_to_words(term_slots + term_offsets[t], n, m.data());
+        const std::complex<double> c(coeff_re[t], coeff_im[t]);
+        double enc;
+        if (e->basis == MPA_BASIS_PAULI) {  // PauliAlgebra.h:150-155
+            if (std::abs(c.imag()) > 1e-10) throw RunError("Non-real Pauli coeffs detected");
+            enc = c.real();
+        } else {  // MajoranaAlgebra.h:162-171
+            static const std::complex<double> kPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+            const size_t un = static_cast<size_t>(n);
+            const std::complex<double> d = c / kPow[(un * (un - 1) / 2) % 4];
+            if (std::abs(d.imag()) > 1e-10) throw RunError("Non-Hermitian coeffs detected");
+            enc = d.real();
+        }
+        if (n == 0) {
+            *core = enc;
+            continue;
+        }
+        rows.insert(rows.end(), m.begin(), m.end());
+        coefs.push_back(enc);
+    }
+}
+
+// Uploads the initial rows (and, for a batch, their members) and indexes their columns.
+void load_initial(mpa_engine *e, const std::vector<uint64_t> &hrows, const std::vector<double> &hcoef,
+                  const std::vector<uint16_t> *members) {
+    const uint64_t n0 = hcoef.size();
+    e->counters.reserve(C_COUNT * 8);
+    HIP_TRY(hipMemsetAsync(e->counters.p, 0, C_COUNT * 8, e->stream));
+    e->occ_dev.reserve(mpa::kMaxWords * 8);
+    HIP_TRY(hipMemcpyAsync(e->occ_dev.p, e->occ.data(), mpa::kMaxWords * 8, hipMemcpyHostToDevice, e->stream));
+    e->ensure_capacity(std::max<uint64_t>(n0, 1));
+    if (members) {
+        e->bmember.reserve(e->cap * 2);
+        if (n0) HIP_TRY(hipMemcpyAsync(e->bmember.p, members->data(), n0 * 2, hipMemcpyHostToDevice, e->stream));
+    }
+    if (n0) {
+        HIP_TRY(hipMemcpyAsync(e->rows.p, hrows.data(), hrows.size() * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->coef.p, hcoef.data(), n0 * 8, hipMemcpyHostToDevice, e->stream));
+        InitCols init{e, n0};
+        e->dispatch(init);
+        const int64_t c0[2] = {static_cast<int64_t>(n0), 0};
+        HIP_TRY(hipMemcpyAsync(e->counters.p, c0, 8, hipMemcpyHostToDevice, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->N = n0;
+    e->n_init = n0;
+    e->init_coef = hcoef;
+}
+
+// Entry points of a lone propagator refuse a batch and the batch entry points a lone propagator.
+void require_single(const mpa_engine *e) {
+    if (e->nbatch) throw ArgError("this engine holds a batch; use the mpa_batch_* entry points");
+}
+void require_batch(const mpa_engine *e) {
+    if (!e->nbatch) throw ArgError("this engine is not a batch (create it with mpa_batch_create)");
+}
+
 }  // namespace
 
 extern "C" {
@@ -2451,79 +2741,38 @@ int mpa_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int 
     return guarded([&] {
         if (!out) throw ArgError("out is null");
         *out = nullptr;
-        if (basis != MPA_BASIS_PAULI && basis != MPA_BASIS_MAJORANA) throw ArgError("unknown basis");
-        if (num_modes <= 0 || num_modes > 256)
-            throw RunError("logical_num_modes (" + std::to_string(num_modes) + ") must be in the range [1, 256].");
-        auto e = std::make_unique<mpa_engine>();
-        e->basis = basis;
-        e->n_modes = num_modes;
-        e->L = (num_modes + 63) / 64;
-        e->nw = basis == MPA_BASIS_PAULI ? 2 * e->L : (2 * num_modes + 63) / 64;
-        e->ncols = basis == MPA_BASIS_PAULI ? 2 * num_modes : 2 * num_modes + 1;
-        set_trunc(e.get(), cutoff, cutoff_type, lower_atol, upper_atol);
-        e->tr.only_rot = -1;
-        e->device = device;
-        HIP_TRY(hipSetDevice(device));
-        HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&e->h_counters), C_COUNT * 8, hipHostMallocDefault));
-        e->occ.assign(mpa::kMaxWords, 0);
-        for (int64_t k = 0; k < n_state; ++k) {
-            const int j = initial_state[k];
-            if (j < 0 || j >= num_modes) throw ArgError("initial_state index out of range");
-            if (basis == MPA_BASIS_PAULI) e->occ[j >> 6] |= uint64_t{1} << (j & 63);
-            else e->occ[(2 * j) >> 6] |= uint64_t{1} << ((2 * j) & 63);
-        }
-        // initial terms in lexicographic slot-tuple order (std::map<VecZ, complex> iteration)
-        std::vector<int64_t> order(n_terms);
-        for (int64_t t = 0; t < n_terms; ++t) order[t] = t;
-        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-            return std::lexicographical_compare(term_slots + term_offsets[a], term_slots + term_offsets[a + 1],
-                                                term_slots + term_offsets[b], term_slots + term_offsets[b + 1]);
-        });
+        auto e = new_engine(basis, num_modes, cutoff, cutoff_type, lower_atol, upper_atol, initial_state, n_state, device);
         std::vector<uint64_t> hrows;
         std::vector<double> hcoef;
-        std::vector<uint64_t> m(e->nw);
-        for (int64_t t : order) {
-            const int64_t n = term_offsets[t + 1] - term_offsets[t];
-            e->slots_to_words(term_slots + term_offsets[t], n, m.data());
-            const std::complex<double> c(coeff_re[t], coeff_im[t]);
-            double enc;
-            if (basis == MPA_BASIS_PAULI) {  // PauliAlgebra.h:150-155
-                if (std::abs(c.imag()) > 1e-10) throw RunError("Non-real Pauli coeffs detected");
-                enc = c.real();
-            } else {  // MajoranaAlgebra.h:162-171
-                static const std::complex<double> kPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
-                const size_t un = static_cast<size_t>(n);
-                const std::complex<double> d = c / kPow[(un * (un - 1) / 2) % 4];
-                if (std::abs(d.imag()) > 1e-10) throw RunError("Non-Hermitian coeffs detected");
-                enc = d.real();
-            }
-            if (n == 0) {
-                e->core = enc;
-                continue;
-            }
-            hrows.insert(hrows.end(), m.begin(), m.end());
-            hcoef.push_back(enc);
+        encode_operator(e.get(), term_offsets, term_slots, coeff_re, coeff_im, 0, n_terms, hrows, hcoef, &e->core);
+        load_initial(e.get(), hrows, hcoef, nullptr);
+        *out = e.release();
+    });
+}
+
+int mpa_batch_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type, double lower_atol,
+                     double upper_atol, int32_t n_members, const int64_t *member_offsets, const int64_t *term_offsets,
+                     const int32_t *term_slots, const double *coeff_re, const double *coeff_im,
+                     const int32_t *initial_state, int64_t n_state, int device) {
+    return guarded([&] {
+        if (!out) throw ArgError("out is null");
+        *out = nullptr;
+        if (n_members < 1 || static_cast<uint32_t>(n_members) > kMaxBatch)
+            throw ArgError("a batch holds 1 to " + std::to_string(kMaxBatch) + " members, got " +
+                           std::to_string(n_members));
+        auto e = new_engine(basis, num_modes, cutoff, cutoff_type, lower_atol, upper_atol, initial_state, n_state, device);
+        e->nbatch = static_cast<uint32_t>(n_members);
+        e->bcore.assign(static_cast<size_t>(n_members), 0.0);
+        std::vector<uint64_t> hrows;
+        std::vector<double> hcoef;
+        std::vector<uint16_t> hmem;
+        for (int32_t q = 0; q < n_members; ++q) {
+            if (member_offsets[q + 1] < member_offsets[q]) throw ArgError("member_offsets must be non-decreasing");
+            encode_operator(e.get(), term_offsets, term_slots, coeff_re, coeff_im, member_offsets[q],
+                            member_offsets[q + 1], hrows, hcoef, &e->bcore[static_cast<size_t>(q)]);
+            hmem.resize(hcoef.size(), static_cast<uint16_t>(q));
         }
-        const uint64_t n0 = hcoef.size();
-        // duplicate keys cannot occur: distinct sorted slot tuples give distinct rows
-        e->counters.reserve(C_COUNT * 8);
-        HIP_TRY(hipMemsetAsync(e->counters.p, 0, C_COUNT * 8, e->stream));
-        e->occ_dev.reserve(mpa::kMaxWords * 8);
-        HIP_TRY(hipMemcpyAsync(e->occ_dev.p, e->occ.data(), mpa::kMaxWords * 8, hipMemcpyHostToDevice, e->stream));
-        e->ensure_capacity(std::max<uint64_t>(n0, 1));
-        if (n0) {
-            HIP_TRY(hipMemcpyAsync(e->rows.p, hrows.data(), hrows.size() * 8, hipMemcpyHostToDevice, e->stream));
-            HIP_TRY(hipMemcpyAsync(e->coef.p, hcoef.data(), n0 * 8, hipMemcpyHostToDevice, e->stream));
-            InitCols init{e.get(), n0};
-            e->dispatch(init);
-            const int64_t c0[2] = {static_cast<int64_t>(n0), 0};
-            HIP_TRY(hipMemcpyAsync(e->counters.p, c0, 8, hipMemcpyHostToDevice, e->stream));
-            HIP_TRY(hipStreamSynchronize(e->stream));
-        }
-        e->N = n0;
-        e->n_init = n0;
-        e->init_coef = hcoef;
+        load_initial(e.get(), hrows, hcoef, &hmem);
         *out = e.release();
     });
 }
@@ -2540,6 +2789,7 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
                   const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
                   int32_t only_rotate_len_k) {
     return guarded([&] {
+        require_single(e);
         if (only_rotate_len_k == 0 || only_rotate_len_k > 2 * e->n_modes)
             throw ArgError("only_rotate_len_k is out of range; must be 0 < k <= 2*num_modes");
         if (!e->layers.empty())
@@ -2559,9 +2809,80 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
     });
 }
 
+int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
+                        const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params) {
+    return guarded([&] {
+        require_batch(e);
+        if (n_layers == 0) return;
+        std::vector<GateDesc> gds;
+        std::vector<int32_t> folds;
+        make_gates(e, gen_offsets, gen_slots, n_layers, param_map, gen_coeffs, params, n_params, true, gds, folds);
+        // member q's constants from its own parameter row, exactly as a lone propagator derives them
+        const uint32_t nb = e->nbatch;
+        std::vector<Trig> trig(gds.size() * nb);
+        for (size_t g = 0; g < gds.size(); ++g) {
+            const GateDesc &gd = gds[g];
+            for (uint32_t q = 0; q < nb; ++q) {
+                Trig &t = trig[g * nb + q];
+                if (gd.identity) {
+                    t = Trig{1.0, 0.0, 1.0, 0.0};
+                    continue;
+                }
+                const double alpha = 1.0 * params[static_cast<int64_t>(q) * n_params + gd.param_index] * gd.gen_coeff;
+                t.k = std::cos(2 * alpha);
+                t.s = std::sin(2 * alpha);
+                t.abs_s = std::abs(std::sin(2 * alpha));
+                const double cos_build = std::cos(2.0 * alpha);
+                if (cos_build == 0.0)
+                    throw RunError("batch member " + std::to_string(q) + " has cos(2 alpha) == 0 at a gate; the batched "
+                                   "pipeline needs the fused cos sweep, propagate that member alone");
+                t.inv_k = 1.0 / cos_build;
+            }
+        }
+        e->btrig.reserve(std::max<size_t>(trig.size(), 1) * sizeof(Trig));
+        HIP_TRY(hipMemcpyAsync(e->btrig.p, trig.data(), trig.size() * sizeof(Trig), hipMemcpyHostToDevice, e->stream));
+        RunBatch run{e, gds, folds};
+        e->dispatch(run);
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    });
+}
+
+int mpa_batch_members(mpa_engine *e, int32_t *out) {
+    return guarded([&] {
+        require_batch(e);
+        *out = static_cast<int32_t>(e->nbatch);
+    });
+}
+
+int mpa_batch_expectation_values(mpa_engine *e, double *out) {
+    return guarded([&] {
+        require_batch(e);
+        RunExpectBatch ev{e, out};
+        e->dispatch(ev);
+    });
+}
+
+int mpa_batch_sizes(mpa_engine *e, int64_t *out) {
+    return guarded([&] {
+        require_batch(e);
+        e->member_sizes(out);
+    });
+}
+
+int mpa_batch_row_members(mpa_engine *e, int32_t *out) {
+    return guarded([&] {
+        require_batch(e);
+        std::vector<uint16_t> h(e->N);
+        if (e->N) HIP_TRY(hipMemcpyAsync(h.data(), e->bmember.p, e->N * 2, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (uint64_t i = 0; i < e->N; ++i) out[i] = h[i];
+    });
+}
+
 int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
                     const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params) {
     return guarded([&] {
+        require_single(e);
         if (n_layers == 0) return;
         std::vector<GateDesc> gds;
         std::vector<int32_t> folds;
@@ -2604,6 +2925,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
 
 int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out) {
     return guarded([&] {
+        require_single(e);
         const double *c = evaluated_coeffs(e, params, n_params, false);
         RunExpect ev{e, c, out};
         e->dispatch(ev);
@@ -2613,6 +2935,7 @@ int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params,
 int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int64_t n_params, double *value,
                                        double *grad) {
     return guarded([&] {
+        require_single(e);
         const double *c = evaluated_coeffs(e, params, n_params, true);
         RunExpect ev{e, c, value};
         e->dispatch(ev);
@@ -2625,6 +2948,7 @@ int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int6
 
 int mpa_contract(mpa_engine *e, const double *params, int64_t n_params, int inplace, double *out) {
     return guarded([&] {
+        require_single(e);
         const double *c = evaluated_coeffs(e, params, n_params, false);
         if (out && e->N) {
             HIP_TRY(hipMemcpyAsync(out, c, e->N * 8, hipMemcpyDeviceToHost, e->stream));
@@ -2673,6 +2997,7 @@ int mpa_read_rows(mpa_engine *e, uint64_t *words, double *coeffs) {
 int mpa_export_terms(mpa_engine *e, const double *params, int64_t n_params, double atol, int64_t *n_terms,
                      int64_t *n_slots, int64_t *offsets, int32_t *slots, double *re, double *im) {
     return guarded([&] {
+        require_single(e);
         const double *cdev = evaluated_coeffs(e, params, n_params, false);
         std::vector<uint64_t> w(e->N * e->nw);
         std::vector<double> c(e->N);

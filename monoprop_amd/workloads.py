@@ -7,7 +7,10 @@
 * spinful Fermi-Hubbard on a periodic L x L lattice: mode 2*site+spin, Majoranas 2*mode and 2*mode+1;
   per step one hopping gate per bond per spin (generator i/2 (m_{2p} m_{2q+1} - m_{2p+1} m_{2q}) scaled
   to unit weight-2 terms) and one weight-4 on-site generator per site;
-* brickwork ring: n qubits, per layer ZZ on even bonds, ZZ on odd bonds, X and Y on every qubit.
+* brickwork ring: n qubits, per layer ZZ on even bonds, ZZ on odd bonds, X and Y on every qubit;
+* parameter sweep over the kicked-Ising lattice: (theta_J, theta_h) sets uniform in [-pi/2, pi/2) from
+  ``default_rng(2)``, eight observables (two each of Z_i, nearest-neighbour Z_i Z_j, plaquette ZZZZ
+  and a horizontal X string of length 5) at sites from ``default_rng(3)``.
 """
 
 from __future__ import annotations
@@ -123,3 +126,40 @@ def brickwork_ring(n: int = 64, layers: int = 12, seed: int = 0) -> Circuit:
 
 def ring_zz_mean(n: int) -> PauliOperator:
     return PauliOperator({Pauli("ZZ", (min(i, (i + 1) % n), max(i, (i + 1) % n))): 1.0 / n for i in range(n)}, n)
+
+
+def kicked_ising_rows(circuit: Circuit, sets) -> np.ndarray:
+    """Per (theta_J, theta_h) set, the circuit's per-gate parameter row: theta_J on ZZ gates, theta_h on
+    X gates (one row per set, ``circuit.n_parameters`` columns)."""
+    is_zz = np.array([len(next(iter(g.generator.terms)).qubits) == 2 for g in circuit.gates])
+    sets = np.asarray(sets, dtype=np.float64).reshape(-1, 2)
+    return np.where(is_zz[None, :], sets[:, :1], sets[:, 1:])
+
+
+def sweep_parameter_sets(n_sets: int = 256, seed: int = 2) -> np.ndarray:
+    """``(n_sets, 2)`` array of (theta_J, theta_h), uniform in [-pi/2, pi/2)."""
+    return np.random.default_rng(seed).uniform(-np.pi / 2, np.pi / 2, size=(n_sets, 2))
+
+
+def sweep_observables(L: int = 10, seed: int = 3) -> list[PauliOperator]:
+    """Eight observables of uneven weight on the L x L lattice (row-major sites): two single Z, two
+    nearest-neighbour ZZ, two plaquettes ZZZZ, two horizontal X strings of length 5."""
+    rng = np.random.default_rng(seed)
+    n = L * L
+    out = []
+    for _ in range(2):
+        out.append(PauliOperator({Pauli("Z", (int(rng.integers(n)),)): 1.0}, n))
+    bonds = lattice_bonds(L)
+    for _ in range(2):
+        i, j = bonds[int(rng.integers(len(bonds)))]
+        out.append(PauliOperator({Pauli("ZZ", (i, j)): 1.0}, n))
+    for _ in range(2):
+        r, c = int(rng.integers(L - 1)), int(rng.integers(L - 1))
+        s = r * L + c
+        out.append(PauliOperator({Pauli("ZZZZ", (s, s + 1, s + L, s + L + 1)): 1.0}, n))
+    for _ in range(2):
+        r, c = int(rng.integers(L)), int(rng.integers(L - 4))
+        s = r * L + c
+        out.append(PauliOperator({Pauli("XXXXX", tuple(range(s, s + 5))): 1.0}, n))
+    return out
+
