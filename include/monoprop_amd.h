@@ -140,10 +140,32 @@ int mpa_batch_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff
 
 /* params: n_members rows of n_params values (row-major); alpha of member q at layer i is
  * params[q * n_params + param_map[i]] * gen_coeffs[i].  Runs the fused pipeline for every member
- * (-2 if some member has cos 2a == 0 at a gate: propagate that member alone). */
+ * (-2 if some member has cos 2a == 0 at a gate: propagate that member alone).
+ * max_capacity bounds the store: propagation stops before the first layer that would grow it past
+ * max_capacity rows (0: no bound; < 0: the largest power of two whose growth fits in 3/4 of the
+ * device).  *layers_done receives the layers applied: the last *layers_done layers in authoring
+ * order; the first n_layers - *layers_done are still to apply. */
 int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots,
                         int64_t n_layers, const int64_t *param_map, const double *gen_coeffs,
-                        const double *params, int64_t n_params);
+                        const double *params, int64_t n_params, int64_t max_capacity,
+                        int64_t *layers_done);
+
+/* A batch engine holding given rows: words (words_per_row per row, mpa_read_rows layout),
+ * coefficients and each row's member, in store order, plus each member's identity coefficient.
+ * With mpa_read_rows / mpa_batch_row_members this moves members of a stopped batch into new
+ * engines that continue where it stopped. */
+int mpa_batch_create_rows(mpa_engine **out, int basis, int num_modes, uint32_t cutoff,
+                          int cutoff_type, double lower_atol, double upper_atol, int32_t n_members,
+                          const double *cores, int64_t n_rows, const uint64_t *words,
+                          const double *coeffs, const int32_t *members,
+                          const int32_t *initial_state, int64_t n_state, int device);
+
+/* Moves the members with move[q] != 0 (one flag per member) into a new batch engine *out; e keeps
+ * the others.  Both sides keep their rows in store order and number their members in ascending order
+ * of the old numbers; both continue from where e stopped (mpa_batch_propagate with the layers still
+ * to apply).  The partition runs on the device; each new store is sized to its rows.  At least one
+ * member must move and one stay. */
+int mpa_batch_split(mpa_engine *e, const uint8_t *move, mpa_engine **out);
 
 /* Number of members. */
 int mpa_batch_members(mpa_engine *e, int32_t *out);

@@ -53,7 +53,12 @@ _SIGNATURES = {
     "mpa_set_truncation": (C.c_int, [_P, C.c_uint32, C.c_int, C.c_double, C.c_double]),
     "mpa_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double, C.c_double,
                                    C.c_int32, _i64p, _i64p, _i32p, _f64p, _f64p, _i32p, C.c_int64, C.c_int]),
-    "mpa_batch_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
+    "mpa_batch_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int64,
+                                      _i64p]),
+    "mpa_batch_create_rows": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double,
+                                        C.c_double, C.c_int32, _f64p, C.c_int64, _u64p, _f64p, _i32p, _i32p,
+                                        C.c_int64, C.c_int]),
+    "mpa_batch_split": (C.c_int, [_P, C.POINTER(C.c_uint8), C.POINTER(_P)]),
     "mpa_batch_members": (C.c_int, [_P, _i32p]),
     "mpa_batch_expectation_values": (C.c_int, [_P, _f64p]),
     "mpa_batch_sizes": (C.c_int, [_P, _i64p]),
@@ -459,7 +464,7 @@ class BatchSimulator:
 
     def __init__(self, operators: Sequence[Mapping[Sequence[int], complex]], cutoff: int, initial_state: Sequence[int],
                  *, lower_atol: float | None = None, upper_atol: float | None = None, cutoff_type: str = "length",
-                 logical_num_modes: int, basis: str = "majorana") -> None:
+                 logical_num_modes: int, basis: str = "majorana", _rows=None) -> None:
         lib = load_library()
         if basis not in ("majorana", "pauli"):
             raise ValueError(f"Unknown basis {basis!r}; expected 'majorana' or 'pauli'.")
@@ -467,8 +472,29 @@ class BatchSimulator:
             raise ValueError(f"Unknown cutoff type {cutoff_type!r}; expected 'length' or 'support'.")
         if isinstance(cutoff, bool) or int(cutoff) < 0:
             raise ValueError(f"cutoff must be a non-negative integer; got {cutoff!r}.")
-        if not 1 <= len(operators) <= self.max_members:
-            raise ValueError(f"a batch holds 1 to {self.max_members} operators, got {len(operators)}.")
+        n_members = len(operators) if _rows is None else len(_rows[0])
+        if not 1 <= n_members <= self.max_members:
+            raise ValueError(f"a batch holds 1 to {self.max_members} operators, got {n_members}.")
+        st = np.asarray([int(i) for i in initial_state] or [0], dtype=np.int32)
+        self._basis = basis
+        self._num_modes = int(logical_num_modes)
+        self._members = n_members
+        h = _P()
+        if _rows is not None:  # (cores[members], words[N, nw], coeffs[N], members[N]) of a stopped batch
+            cores, words, coeffs, members = _rows
+            cores = np.ascontiguousarray(cores, dtype=np.float64)
+            nr = len(coeffs)
+            words = np.ascontiguousarray(words, dtype=np.uint64).reshape(-1) if nr else np.zeros(1, np.uint64)
+            coeffs = np.ascontiguousarray(coeffs, dtype=np.float64) if nr else np.zeros(1, np.float64)
+            members = np.ascontiguousarray(members, dtype=np.int32) if nr else np.zeros(1, np.int32)
+            _check(lib.mpa_batch_create_rows(C.byref(h), 1 if basis == "pauli" else 0, self._num_modes, int(cutoff),
+                                             0 if cutoff_type == "length" else 1, _nan_if_none(lower_atol),
+                                             _nan_if_none(upper_atol), n_members, _ptr(cores, C.c_double), nr,
+                                             _ptr(words, C.c_uint64), _ptr(coeffs, C.c_double),
+                                             _ptr(members, C.c_int32), _ptr(st, C.c_int32), len(list(initial_state)),
+                                             default_device()))
+            self._h = h
+            return
         keys: list[tuple[int, ...]] = []
         vals: list[complex] = []
         moffs = np.zeros(len(operators) + 1, dtype=np.int64)
@@ -480,11 +506,6 @@ class BatchSimulator:
         offs, slots = _csr(keys)
         re = _f64(v.real for v in vals)
         im = _f64(v.imag for v in vals)
-        st = np.asarray([int(i) for i in initial_state] or [0], dtype=np.int32)
-        self._basis = basis
-        self._num_modes = int(logical_num_modes)
-        self._members = len(operators)
-        h = _P()
         _check(lib.mpa_batch_create(C.byref(h), 1 if basis == "pauli" else 0, self._num_modes, int(cutoff),
                                     0 if cutoff_type == "length" else 1, _nan_if_none(lower_atol),
                                     _nan_if_none(upper_atol), len(operators), _ptr(moffs, C.c_int64),
@@ -503,10 +524,11 @@ class BatchSimulator:
     def members(self) -> int:
         return self._members
 
-    def propagate(self, majoranas, parameter_mapping, gen_coeffs, parameters) -> None:
-        """``parameters``: one row per member (shape ``(members, n_params)``)."""
+    def propagate(self, majoranas, parameter_mapping, gen_coeffs, parameters, max_capacity: int = 0) -> int:
+        """``parameters``: one row per member (shape ``(members, n_params)``).  Returns the layers applied
+        (all of them unless ``max_capacity`` stopped the store from growing: then the last ones)."""
         if not len(majoranas):
-            return
+            return 0
         offs, slots, pm, gc = DeviceSimulator._layers(majoranas, parameter_mapping, gen_coeffs)
         th = np.ascontiguousarray(np.asarray(parameters, dtype=np.float64))
         if th.ndim != 2 or th.shape[0] != self._members:
@@ -515,9 +537,27 @@ class BatchSimulator:
         if th.shape[1] < need:
             raise ValueError(f"Expected at least {need} parameter value(s) per member, got {th.shape[1]}.")
         flat = th.reshape(-1) if th.size else np.zeros(1, dtype=np.float64)
+        done = C.c_int64(0)
         _check(load_library().mpa_batch_propagate(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
                                                   len(majoranas), _ptr(pm, C.c_int64), _ptr(gc, C.c_double),
-                                                  _ptr(flat, C.c_double), th.shape[1]))
+                                                  _ptr(flat, C.c_double), th.shape[1], int(max_capacity),
+                                                  C.byref(done)))
+        return done.value
+
+    def split(self, move) -> BatchSimulator:
+        """Move the members with ``move[q]`` true into a new batch (both renumbered in ascending order,
+        rows in store order); both continue from the gate this one stopped at."""
+        mv = np.ascontiguousarray(np.asarray(move, dtype=bool), dtype=np.uint8)
+        if mv.shape != (self._members,):
+            raise ValueError(f"move must have one flag per member ({self._members}), got shape {mv.shape}")
+        h = _P()
+        _check(load_library().mpa_batch_split(self._h, _ptr(mv, C.c_uint8), C.byref(h)))
+        other = BatchSimulator.__new__(BatchSimulator)
+        other._basis, other._num_modes = self._basis, self._num_modes
+        other._members = int(mv.sum())
+        other._h = h
+        self._members -= other._members
+        return other
 
     def expectation_values(self) -> np.ndarray:
         out = np.zeros(self._members, dtype=np.float64)

@@ -61,9 +61,28 @@ class BatchPropagator:
         self._basis = basis
         self._num_modes = widths.pop()
         self._initial_state = [int(i) for i in initial_state]
-        self._sim = BatchSimulator([t for _, t, _ in encoded], cutoff, self._initial_state, lower_atol=lower_atol,
-                                   upper_atol=upper_atol, cutoff_type=cutoff_type,
-                                   logical_num_modes=self._num_modes, basis=basis)
+        self._config = dict(cutoff=cutoff, lower_atol=lower_atol, upper_atol=upper_atol, cutoff_type=cutoff_type)
+        # identity coefficients (the engine's core terms): the real part, as both encodings keep it
+        self._cores = np.asarray([complex(t.get((), 0.0)).real for _, t, _ in encoded], dtype=np.float64)
+        self._sim = BatchSimulator([t for _, t, _ in encoded], cutoff, self._initial_state,
+                                   logical_num_modes=self._num_modes, basis=basis, **self._drop_cutoff())
+
+    def _drop_cutoff(self) -> dict:
+        return {k: v for k, v in self._config.items() if k != "cutoff"}
+
+    @property
+    def _spec(self):
+        return self._basis, self._num_modes, self._initial_state, dict(self._config)
+
+    @classmethod
+    def _from_rows(cls, spec, cores, words, coeffs, members) -> BatchPropagator:
+        """A batch of configuration ``spec`` holding the given rows (members numbered 0..)."""
+        b = cls.__new__(cls)
+        b._basis, b._num_modes, b._initial_state, b._config = spec
+        b._cores = np.asarray(cores, dtype=np.float64)
+        b._sim = BatchSimulator([], b._config["cutoff"], b._initial_state, logical_num_modes=b._num_modes,
+                                basis=b._basis, _rows=(b._cores, words, coeffs, members), **b._drop_cutoff())
+        return b
 
     @property
     def members(self) -> int:
@@ -73,9 +92,7 @@ class BatchPropagator:
     def num_modes(self) -> int:
         return self._num_modes
 
-    def propagate(self, circuit: Circuit, parameter_sets) -> None:
-        """Evolve every member through ``circuit`` (Heisenberg, back to front) at its own row of
-        ``parameter_sets`` (shape ``(members, circuit.n_parameters)``)."""
+    def _layers(self, circuit: Circuit):
         if circuit.system_size != self._num_modes:
             raise ValueError(f"Circuit system_size={circuit.system_size} does not match the batch width "
                              f"{self._num_modes}.")
@@ -86,6 +103,12 @@ class BatchPropagator:
             raise ValueError("circuit.initial_state does not match the batch's initial state")
         majoranas, gen_coeffs, mapping, _ = expand_monomials(circuit.gates, circuit.resolved_mapping,
                                                              self._num_modes)
+        return majoranas, mapping, gen_coeffs
+
+    def propagate(self, circuit: Circuit, parameter_sets) -> None:
+        """Evolve every member through ``circuit`` (Heisenberg, back to front) at its own row of
+        ``parameter_sets`` (shape ``(members, circuit.n_parameters)``)."""
+        majoranas, mapping, gen_coeffs = self._layers(circuit)
         params = np.asarray(parameter_sets, dtype=np.float64)
         if params.ndim != 2 or params.shape != (self.members, circuit.n_parameters):
             raise ValueError(f"parameter_sets must have shape ({self.members}, {circuit.n_parameters}), got "
@@ -103,6 +126,18 @@ class BatchPropagator:
     def size(self) -> int:
         return self._sim.size()
 
+    def split(self, move) -> BatchPropagator:
+        """Move the members with ``move[q]`` true into a new batch on the device (store order kept,
+        members renumbered in ascending order on both sides); both continue from where this one
+        stopped."""
+        mv = np.asarray(move, dtype=bool)
+        b = BatchPropagator.__new__(BatchPropagator)
+        b._basis, b._num_modes, b._initial_state, b._config = self._spec
+        b._sim = self._sim.split(mv)
+        b._cores = self._cores[mv]
+        self._cores = self._cores[~mv]
+        return b
+
     def member_rows(self, q: int):
         """Member ``q``'s store ``(words[n, nw], coeffs[n])`` in its insertion order."""
         w, c, m = self._sim.rows()
@@ -116,21 +151,33 @@ class BatchPropagator:
         return self._sim.gate_stats()
 
 
-def _out_of_memory(err: RuntimeError) -> bool:
-    msg = str(err)
-    return any(s in msg for s in ("out of device memory", "out of memory", "32-bit row index"))
+def _halves(sizes: np.ndarray) -> np.ndarray:
+    """Two groups of members with near-equal stored terms (largest first, each to the lighter group);
+    True marks the second group.  Both groups are non-empty for two or more members."""
+    side = np.zeros(len(sizes), dtype=bool)
+    load = [0, 0]
+    for q in np.argsort(-np.asarray(sizes), kind="stable"):
+        g = 0 if load[0] <= load[1] else 1
+        side[q] = g == 1
+        load[g] += int(sizes[q]) + 1
+    return side
 
 
 def expectation_grid(circuit: Circuit, operators: Sequence[Operator], parameter_sets, *, cutoff: int,
                      lower_atol: float | None = None, upper_atol: float | None = None,
                      cutoff_type: str | None = None, max_members: int = BatchSimulator.max_members,
-                     stats: list | None = None) -> np.ndarray:
+                     max_capacity: int = -1, stats: list | None = None, progress=None) -> np.ndarray:
     """``<O_j>(theta_i)`` for every parameter set ``i`` and observable ``j``: shape
     ``(len(parameter_sets), len(operators))``.
 
-    Members are the pairs in row-major order, taken ``max_members`` at a time; a batch whose store
-    outgrows device memory is split in halves and run again.  ``stats``, when given, receives one
-    ``(members, stored terms, propagate ms)`` tuple per batch that completed.
+    Members are the pairs in row-major order, ``max_members`` per batch.  A batch of several members
+    whose store would grow past ``max_capacity`` rows (< 0: sized to the device, 0: no bound) stops
+    before that gate; its members are split into two halves of near-equal size, which continue from
+    that gate as separate batches (depth first; the waiting half stays on the device at its exact
+    size).  A single
+    member is never bounded.  ``stats``, when given, receives one ``(members, stored terms, device
+    ms, split)`` tuple per propagate call, ``split`` true when the call stopped at the bound;
+    ``progress``, when given, is called with each such tuple.
     """
     params = np.asarray(parameter_sets, dtype=np.float64)
     if params.ndim != 2:
@@ -138,27 +185,38 @@ def expectation_grid(circuit: Circuit, operators: Sequence[Operator], parameter_
     n_sets, n_ops = params.shape[0], len(operators)
     pairs = [(i, j) for i in range(n_sets) for j in range(n_ops)]
     out = np.empty((n_sets, n_ops), dtype=np.float64)
-    todo = [pairs[k:k + max_members] for k in range(0, len(pairs), max_members)]
     state = list(circuit.initial_state)
-    while todo:
-        chunk = todo.pop(0)
-        b = None
-        try:
+    # job: (pairs, layers still to apply, the stopped batch holding them or None)
+    jobs = [(pairs[k:k + max_members], None, None) for k in range(0, len(pairs), max_members)][::-1]
+    layers = None
+    while jobs:
+        chunk, left, b = jobs.pop()
+        if b is None:
             b = BatchPropagator([operators[j] for _, j in chunk], state, cutoff=cutoff, lower_atol=lower_atol,
                                 upper_atol=upper_atol, cutoff_type=cutoff_type)
-            b.propagate(circuit, params[[i for i, _ in chunk]])
-            vals = b.expectation_values()
-            if stats is not None:
-                stats.append((len(chunk), b.size(), b.last_propagate_ms()))
-        except RuntimeError as err:
-            b = None  # give the store back before the halves run
-            if len(chunk) == 1 or not _out_of_memory(err):
-                raise
-            half = len(chunk) // 2
-            todo[:0] = [chunk[:half], chunk[half:]]
+        if layers is None:
+            layers = b._layers(circuit)
+            if params.shape[1] != circuit.n_parameters:
+                raise ValueError(f"parameter_sets must have {circuit.n_parameters} columns, got {params.shape[1]}")
+        majoranas, mapping, gen_coeffs = layers
+        if left is None:
+            left = len(majoranas)
+        bound = max_capacity if len(chunk) > 1 else 0
+        done = b._sim.propagate(majoranas[:left], mapping[:left], gen_coeffs[:left], params[[i for i, _ in chunk]],
+                                bound)
+        left -= done
+        entry = (len(chunk), b.size(), b.last_propagate_ms(), left > 0)
+        if stats is not None:
+            stats.append(entry)
+        if progress is not None:
+            progress(entry)
+        if left == 0:
+            for (i, j), v in zip(chunk, b.expectation_values()):
+                out[i, j] = v
             continue
-        finally:
-            b = None
-        for (i, j), v in zip(chunk, vals):
-            out[i, j] = v
+        move = _halves(b.sizes())
+        moved = b.split(move)
+        jobs.append(([p for p, f in zip(chunk, move) if f], left, moved))
+        jobs.append(([p for p, f in zip(chunk, move) if not f], left, b))
+        b = moved = None
     return out

@@ -1422,6 +1422,48 @@ __global__ __launch_bounds__(kBlock) void k_init_lambda(const uint64_t *__restri
 }
 
 // Per-layer sum of the adjoint partials, in block order: one block per layer.
+// Stable partition of a batch store by member: side[member] = 1 moves the row out.  Per 256-row block,
+// the number of moving rows.
+__global__ __launch_bounds__(kBlock) void k_part_count(const uint16_t *__restrict__ member, uint64_t N,
+                                                       const uint8_t *__restrict__ side, uint32_t *__restrict__ blk) {
+    __shared__ uint32_t lds[kWaves];
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const uint32_t f = i < N ? side[member[i]] : 0u;
+    const uint32_t c = block_sum(f, lds);
+    if (threadIdx.x == 0) blk[blockIdx.x] = c;
+}
+
+// blk: moving rows before each block (exclusive scan of k_part_count).  Row i goes to position
+// (moving rows before i) of the moving side or (staying rows before i) of the staying side, with its
+// member renumbered to renum[member].
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_part_scatter(const uint64_t *__restrict__ rows, const double *__restrict__ coef,
+                                                         const uint16_t *__restrict__ member, uint64_t N,
+                                                         const uint8_t *__restrict__ side,
+                                                         const uint16_t *__restrict__ renum,
+                                                         const uint32_t *__restrict__ blk,
+                                                         uint64_t *__restrict__ rows_mv, double *__restrict__ coef_mv,
+                                                         uint16_t *__restrict__ mem_mv, uint64_t *__restrict__ rows_st,
+                                                         double *__restrict__ coef_st, uint16_t *__restrict__ mem_st) {
+    __shared__ uint32_t lds[kWaves];
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    uint32_t f = 0, q = 0;
+    if (i < N) {
+        q = member[i];
+        f = side[q];
+    }
+    uint32_t tot;
+    const uint32_t rank = block_excl_scan(f, lds, &tot);
+    if (i >= N) return;
+    const uint64_t moved_before = static_cast<uint64_t>(blk[blockIdx.x]) + rank;
+    const uint64_t dest = f ? moved_before : i - moved_before;
+    uint64_t *rd = (f ? rows_mv : rows_st) + dest * NW;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) rd[k] = rows[i * NW + k];
+    (f ? coef_mv : coef_st)[dest] = coef[i];
+    (f ? mem_mv : mem_st)[dest] = renum[q];
+}
+
 // Batched <O>: per member, the sum over its fully paired rows of phase * c.  Each wavefront keeps one
 // accumulator per member in LDS and walks a fixed set of 64-row windows; inside a window the lanes of
 // one member are summed by a fixed butterfly and added by a single lane, so the result does not depend
@@ -1578,6 +1620,13 @@ class BlockCache {
         std::lock_guard<std::mutex> lk(mu_);
         free_.push_back({p, bytes, dev, s, ev});
     }
+    size_t cached_bytes(int dev) {
+        std::lock_guard<std::mutex> lk(mu_);
+        size_t t = 0;
+        for (const Blk &b : free_)
+            if (b.dev == dev) t += b.bytes;
+        return t;
+    }
     void release_all() {
         std::lock_guard<std::mutex> lk(mu_);
         for (Blk &b : free_) {
@@ -1619,6 +1668,12 @@ struct DevArray {
     template <typename T>
     T *as() const {
         return static_cast<T *>(p);
+    }
+    // Back to the cache.
+    void release() {
+        if (p) BlockCache::get().give(p, bytes, stream());
+        p = nullptr;
+        bytes = 0;
     }
     // Grow to at least `need` bytes (contents not preserved).
     void reserve(size_t need) {
@@ -1687,6 +1742,8 @@ struct mpa_engine {
     // the members' identity coefficients.
     uint32_t nbatch = 0;
     std::vector<double> bcore;
+    uint64_t cap_limit = 0;  // fused pipeline: stop before a gate that would grow the store past this (0: none)
+    size_t fused_done = 0;   // gates the last fused propagate applied
     DevArray bmember{&stream}, btrig{&stream}, bpartial{&stream}, bout{&stream};
 
     std::vector<double> init_coef;  // initial coefficients of rows [0, n_init)
@@ -1790,9 +1847,43 @@ struct mpa_engine {
 
     // Capacities are powers of two (at least 16384 rows), so every per-row array of a store, and of the
     // next propagator, recurs at the same sizes and the memory pool hands back cached blocks.
-    void grow_store(uint64_t need) {
+    // A fresh store of `c` rows of capacity (a multiple of 64; contents undefined, column index zero).
+    void alloc_store(uint64_t c) {
+        rows.reserve(c * nw * 8);
+        coef.reserve(c * 8);
+        cols.reserve(static_cast<size_t>(ncols) * (c / 64) * 8);
+        cap = c;
+        HIP_TRY(hipMemsetAsync(cols.p, 0, static_cast<size_t>(ncols) * (c / 64) * 8, stream));
+        if (nbatch) bmember.reserve(c * 2);
+    }
+
+    // Gives back the arrays sized by the capacity other than the store itself (the fused pipeline's,
+    // the row index, the per-gate pipeline's); they are sized again by the next propagate.
+    void release_work_arrays() {
+        for (DevArray *a : {&gidx, &f_elead, &f_vsrc, &t_item, &t_cnt, &t_bsum, &t_gsum, &desc_m, &AL, &lead, &foll,
+                            &blk_a, &blk_b})
+            a->release();
+        fused_cap = 0;
+        cap_items = 0;
+        gidx_slots = 0;
+        gidx_rows = static_cast<uint64_t>(-1);
+    }
+
+    // Capacity grow_store(need) picks: a power of two, at least 64 * kBlock, need and twice the current.
+    uint64_t grown_capacity(uint64_t need) const {
         uint64_t nc = 64ull * kBlock;
         while (nc < need || nc < 2 * cap) nc <<= 1;
+        return nc;
+    }
+
+    // Device bytes per row of capacity: store, column index, fused-pipeline arrays, row index at two
+    // slots per row, per-gate pipeline item list.
+    double bytes_per_row() const {
+        return 8.0 * nw + 8 + ncols / 8.0 + 8 + (nbatch ? 2 : 0) + 2 + 16.0 * (nw + 1) + 4 + 0.25;
+    }
+
+    void grow_store(uint64_t need) {
+        const uint64_t nc = grown_capacity(need);
         DevArray r2{&stream}, c2{&stream}, k2{&stream};
         r2.reserve(nc * nw * 8);
         c2.reserve(nc * 8);
@@ -2238,7 +2329,11 @@ struct mpa_engine {
         HIP_TRY(hipEventRecord(ev[0], stream));
         size_t gi = 0;
         while (gi < ng) {
-            if (2 * N > cap) ensure_capacity(std::max<uint64_t>(3 * N, 64ull * kBlock));
+            if (2 * N > cap) {
+                const uint64_t need = std::max<uint64_t>(3 * N, 64ull * kBlock);
+                if (cap_limit && grown_capacity(need) > cap_limit) break;  // stop before gate gi
+                ensure_capacity(need);
+            }
             ensure_fused_arrays<NW, BATCH>();
             const size_t end = std::min(ng, gi + kChunk);
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
@@ -2299,6 +2394,7 @@ struct mpa_engine {
             reset_fused_counters(next);
             gi = next;
         }
+        fused_done = gi;
         HIP_TRY(hipEventRecord(ev[1], stream));
         HIP_TRY(hipStreamSynchronize(stream));
         prof_resolve();
@@ -2656,6 +2752,54 @@ std::unique_ptr<mpa_engine> new_engine(int basis, int num_modes, uint32_t cutoff
     return e;
 }
 
+// An engine with the configuration (basis, width, truncation, initial state, device) of `e`, its own
+// stream and counters, and no terms.
+std::unique_ptr<mpa_engine> clone_config(const mpa_engine *e) {
+    auto o = std::make_unique<mpa_engine>();
+    o->basis = e->basis;
+    o->n_modes = e->n_modes;
+    o->L = e->L;
+    o->nw = e->nw;
+    o->ncols = e->ncols;
+    o->tr = e->tr;
+    o->occ = e->occ;
+    o->device = e->device;
+    o->fused_enabled = e->fused_enabled;
+    HIP_TRY(hipSetDevice(o->device));
+    HIP_TRY(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&o->h_counters), C_COUNT * 8, hipHostMallocDefault));
+    o->counters.reserve(C_COUNT * 8);
+    HIP_TRY(hipMemsetAsync(o->counters.p, 0, C_COUNT * 8, o->stream));
+    o->occ_dev.reserve(mpa::kMaxWords * 8);
+    HIP_TRY(hipMemcpyAsync(o->occ_dev.p, o->occ.data(), mpa::kMaxWords * 8, hipMemcpyHostToDevice, o->stream));
+    HIP_TRY(hipStreamSynchronize(o->stream));
+    return o;
+}
+
+// Capacity of a store that holds n rows exactly (rounded up to the allocation granule).
+uint64_t exact_capacity(uint64_t n) {
+    const uint64_t g = 64ull * kBlock;
+    return std::max<uint64_t>(g, (n + g - 1) / g * g);
+}
+
+struct RunPartition {
+    mpa_engine *e;
+    const uint8_t *side;
+    const uint16_t *renum;
+    const uint32_t *blk;
+    uint64_t *rows_mv, *rows_st;
+    double *coef_mv, *coef_st;
+    uint16_t *mem_mv, *mem_st;
+    template <int NW, int PAULI>
+    void operator()() {
+        const uint64_t nblk = (e->N + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL((k_part_scatter<NW>), dim3(static_cast<uint32_t>(nblk)), dim3(kBlock), 0, e->stream,
+                           e->rows.as<uint64_t>(), e->coef.as<double>(), e->bmember.as<uint16_t>(), e->N, side, renum,
+                           blk, rows_mv, coef_mv, mem_mv, rows_st, coef_st, mem_st);
+        HIP_TRY(hipGetLastError());
+    }
+};
+
 // Terms [t0, t1) of a CSR operator, in lexicographic slot-tuple order (std::map<VecZ, complex>
 // iteration), encoded and appended to rows / coefs; the empty tuple's coefficient goes to *core.
 // Distinct sorted slot tuples give distinct rows, so no key repeats.
@@ -2695,9 +2839,7 @@ void encode_operator(const mpa_engine *e, const int64_t *term_offsets, const int
 }
 
 // Uploads the initial rows (and, for a batch, their members) and indexes their columns.
-void load_initial(mpa_engine *e, const std::vector<uint64_t> &hrows, const std::vector<double> &hcoef,
-                  const std::vector<uint16_t> *members) {
-    const uint64_t n0 = hcoef.size();
+void load_initial(mpa_engine *e, uint64_t n0, const uint64_t *hrows, const double *hcoef, const uint16_t *members) {
     e->counters.reserve(C_COUNT * 8);
     HIP_TRY(hipMemsetAsync(e->counters.p, 0, C_COUNT * 8, e->stream));
     e->occ_dev.reserve(mpa::kMaxWords * 8);
@@ -2705,11 +2847,11 @@ void load_initial(mpa_engine *e, const std::vector<uint64_t> &hrows, const std::
     e->ensure_capacity(std::max<uint64_t>(n0, 1));
     if (members) {
         e->bmember.reserve(e->cap * 2);
-        if (n0) HIP_TRY(hipMemcpyAsync(e->bmember.p, members->data(), n0 * 2, hipMemcpyHostToDevice, e->stream));
+        if (n0) HIP_TRY(hipMemcpyAsync(e->bmember.p, members, n0 * 2, hipMemcpyHostToDevice, e->stream));
     }
     if (n0) {
-        HIP_TRY(hipMemcpyAsync(e->rows.p, hrows.data(), hrows.size() * 8, hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->coef.p, hcoef.data(), n0 * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->rows.p, hrows, n0 * e->nw * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->coef.p, hcoef, n0 * 8, hipMemcpyHostToDevice, e->stream));
         InitCols init{e, n0};
         e->dispatch(init);
         const int64_t c0[2] = {static_cast<int64_t>(n0), 0};
@@ -2718,7 +2860,7 @@ void load_initial(mpa_engine *e, const std::vector<uint64_t> &hrows, const std::
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->N = n0;
     e->n_init = n0;
-    e->init_coef = hcoef;
+    if (!members) e->init_coef.assign(hcoef, hcoef + n0);  // the graph path restarts from these
 }
 
 // Entry points of a lone propagator refuse a batch and the batch entry points a lone propagator.
@@ -2745,7 +2887,7 @@ int mpa_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int 
         std::vector<uint64_t> hrows;
         std::vector<double> hcoef;
         encode_operator(e.get(), term_offsets, term_slots, coeff_re, coeff_im, 0, n_terms, hrows, hcoef, &e->core);
-        load_initial(e.get(), hrows, hcoef, nullptr);
+        load_initial(e.get(), hcoef.size(), hrows.data(), hcoef.data(), nullptr);
         *out = e.release();
     });
 }
@@ -2772,7 +2914,7 @@ int mpa_batch_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff
                             member_offsets[q + 1], hrows, hcoef, &e->bcore[static_cast<size_t>(q)]);
             hmem.resize(hcoef.size(), static_cast<uint16_t>(q));
         }
-        load_initial(e.get(), hrows, hcoef, &hmem);
+        load_initial(e.get(), hcoef.size(), hrows.data(), hcoef.data(), hmem.data());
         *out = e.release();
     });
 }
@@ -2810,10 +2952,25 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
 }
 
 int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
-                        const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params) {
+                        const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
+                        int64_t max_capacity, int64_t *layers_done) {
     return guarded([&] {
         require_batch(e);
+        if (layers_done) *layers_done = 0;
         if (n_layers == 0) return;
+        if (max_capacity < 0) {  // automatic: a growth (old + new store) stays within 3/4 of what is available
+            size_t free_b = 0, total_b = 0;
+            HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+            const double avail = static_cast<double>(free_b) +
+                                 static_cast<double>(BlockCache::get().cached_bytes(e->device)) +
+                                 e->bytes_per_row() * static_cast<double>(e->cap);
+            const double rows = 0.75 * avail / (1.5 * e->bytes_per_row());
+            uint64_t lim = 64ull * kBlock;
+            while (static_cast<double>(lim) * 2 <= rows) lim <<= 1;
+            e->cap_limit = lim;
+        } else {
+            e->cap_limit = static_cast<uint64_t>(max_capacity);
+        }
         std::vector<GateDesc> gds;
         std::vector<int32_t> folds;
         make_gates(e, gen_offsets, gen_slots, n_layers, param_map, gen_coeffs, params, n_params, true, gds, folds);
@@ -2842,8 +2999,129 @@ int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t
         e->btrig.reserve(std::max<size_t>(trig.size(), 1) * sizeof(Trig));
         HIP_TRY(hipMemcpyAsync(e->btrig.p, trig.data(), trig.size() * sizeof(Trig), hipMemcpyHostToDevice, e->stream));
         RunBatch run{e, gds, folds};
-        e->dispatch(run);
+        try {
+            e->dispatch(run);
+        } catch (...) {
+            e->cap_limit = 0;
+            throw;
+        }
+        e->cap_limit = 0;
         HIP_TRY(hipStreamSynchronize(e->stream));
+        if (layers_done) *layers_done = static_cast<int64_t>(e->fused_done);
+    });
+}
+
+int mpa_batch_create_rows(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type,
+                          double lower_atol, double upper_atol, int32_t n_members, const double *cores, int64_t n_rows,
+                          const uint64_t *words, const double *coeffs, const int32_t *members,
+                          const int32_t *initial_state, int64_t n_state, int device) {
+    return guarded([&] {
+        if (!out) throw ArgError("out is null");
+        *out = nullptr;
+        if (n_members < 1 || static_cast<uint32_t>(n_members) > kMaxBatch)
+            throw ArgError("a batch holds 1 to " + std::to_string(kMaxBatch) + " members, got " +
+                           std::to_string(n_members));
+        if (n_rows < 0 || n_rows >= (int64_t{1} << 31)) throw ArgError("row count out of range");
+        auto e = new_engine(basis, num_modes, cutoff, cutoff_type, lower_atol, upper_atol, initial_state, n_state, device);
+        e->nbatch = static_cast<uint32_t>(n_members);
+        e->bcore.assign(cores, cores + n_members);
+        const size_t nr = static_cast<size_t>(n_rows);
+        std::vector<uint16_t> hmem(nr);
+        for (size_t i = 0; i < nr; ++i) {
+            if (members[i] < 0 || members[i] >= n_members) throw ArgError("row member out of range");
+            hmem[i] = static_cast<uint16_t>(members[i]);
+        }
+        load_initial(e.get(), static_cast<uint64_t>(nr), words, coeffs, hmem.data());
+        *out = e.release();
+    });
+}
+
+int mpa_batch_split(mpa_engine *e, const uint8_t *move, mpa_engine **out) {
+    return guarded([&] {
+        require_batch(e);
+        if (!out) throw ArgError("out is null");
+        *out = nullptr;
+        const uint32_t nb = e->nbatch;
+        std::vector<uint8_t> side(nb);
+        uint32_t n_mv = 0;
+        for (uint32_t q = 0; q < nb; ++q) {
+            side[q] = move[q] ? 1 : 0;
+            n_mv += side[q];
+        }
+        if (n_mv == 0 || n_mv == nb) throw ArgError("a split moves at least one member and keeps at least one");
+        std::vector<int64_t> sizes(nb);
+        e->member_sizes(sizes.data());
+        uint64_t rows_mv = 0;
+        std::vector<uint16_t> renum(nb);
+        std::vector<double> core_mv, core_st;
+        for (uint32_t q = 0; q < nb; ++q) {
+            if (side[q]) {
+                rows_mv += static_cast<uint64_t>(sizes[q]);
+                renum[q] = static_cast<uint16_t>(core_mv.size());
+                core_mv.push_back(e->bcore[q]);
+            } else {
+                renum[q] = static_cast<uint16_t>(core_st.size());
+                core_st.push_back(e->bcore[q]);
+            }
+        }
+        const uint64_t rows_st = e->N - rows_mv;
+        auto o = clone_config(e);
+        o->nbatch = n_mv;
+        o->bcore = core_mv;
+        o->alloc_store(exact_capacity(rows_mv));
+        HIP_TRY(hipStreamSynchronize(o->stream));
+        // the staying members' new store, on e's stream
+        const uint64_t cst = exact_capacity(rows_st);
+        DevArray r2{&e->stream}, c2{&e->stream}, m2{&e->stream}, k2{&e->stream}, dside{&e->stream},
+            drenum{&e->stream}, dblk{&e->stream};
+        r2.reserve(cst * e->nw * 8);
+        c2.reserve(cst * 8);
+        m2.reserve(cst * 2);
+        k2.reserve(static_cast<size_t>(e->ncols) * (cst / 64) * 8);
+        HIP_TRY(hipMemsetAsync(k2.p, 0, static_cast<size_t>(e->ncols) * (cst / 64) * 8, e->stream));
+        const uint64_t nblk = (e->N + kBlock - 1) / kBlock;
+        dside.reserve(nb);
+        drenum.reserve(static_cast<size_t>(nb) * 2);
+        dblk.reserve(std::max<uint64_t>(nblk, 1) * 4);
+        HIP_TRY(hipMemcpyAsync(dside.p, side.data(), nb, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(drenum.p, renum.data(), static_cast<size_t>(nb) * 2, hipMemcpyHostToDevice, e->stream));
+        if (e->N) {
+            hipLaunchKernelGGL(k_part_count, dim3(static_cast<uint32_t>(nblk)), dim3(kBlock), 0, e->stream,
+                               e->bmember.as<uint16_t>(), e->N, dside.as<uint8_t>(), dblk.as<uint32_t>());
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, e->stream, dblk.as<uint32_t>(), nullptr,
+                               static_cast<uint32_t>(nblk), e->counters.as<uint64_t>(), -1, -1, 0, 0);
+            HIP_TRY(hipGetLastError());
+            RunPartition part{e, dside.as<uint8_t>(), drenum.as<uint16_t>(), dblk.as<uint32_t>(),
+                              o->rows.as<uint64_t>(), r2.as<uint64_t>(), o->coef.as<double>(), c2.as<double>(),
+                              o->bmember.as<uint16_t>(), m2.as<uint16_t>()};
+            e->dispatch(part);
+        }
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        std::swap(e->rows.p, r2.p);
+        std::swap(e->rows.bytes, r2.bytes);
+        std::swap(e->coef.p, c2.p);
+        std::swap(e->coef.bytes, c2.bytes);
+        std::swap(e->bmember.p, m2.p);
+        std::swap(e->bmember.bytes, m2.bytes);
+        std::swap(e->cols.p, k2.p);
+        std::swap(e->cols.bytes, k2.bytes);
+        e->cap = cst;
+        e->N = rows_st;
+        e->nbatch = nb - n_mv;
+        e->bcore = core_st;
+        e->release_work_arrays();
+        o->N = rows_mv;
+        for (mpa_engine *x : {e, o.get()}) {
+            if (x->N) {
+                InitCols init{x, x->N};
+                x->dispatch(init);
+            }
+            const int64_t c0[1] = {static_cast<int64_t>(x->N)};
+            HIP_TRY(hipMemcpyAsync(x->counters.p, c0, 8, hipMemcpyHostToDevice, x->stream));
+            HIP_TRY(hipStreamSynchronize(x->stream));
+        }
+        *out = o.release();
     });
 }
 

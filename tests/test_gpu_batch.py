@@ -161,3 +161,69 @@ def test_batch_refuses_single_entry_points():
     assert b"batch" in lib.mpa_last_error()
     with pytest.raises(ValueError):
         BatchPropagator([PauliOperator({Pauli("Z", (0,)): 1.0}, 8), MajoranaOperator({(0, 1): 1j}, 8)], [], cutoff=4)
+
+
+def _c1_members(n_sets=3):
+    n = 20
+    circ = workloads.kicked_ising_chain(n, 5)
+    rng = np.random.default_rng(17)
+    sets = rng.uniform(-np.pi / 2, np.pi / 2, size=(n_sets, 2))
+    obs = [PauliOperator({Pauli("Z", (10,)): 1.0}, n), PauliOperator({Pauli("ZZ", (9, 10)): 1.0}, n)]
+    return circ, sets, obs
+
+
+def test_stop_and_resume_is_bit_exact(oracle_mod):
+    """A batch stopped by its capacity bound and rebuilt from its rows continues exactly where it stopped:
+    every member's store equals the oracle's."""
+    print("case: batch stop + resume c1", flush=True)
+    circ, sets, obs = _c1_members()
+    ops = [obs[j] for i in range(len(sets)) for j in range(len(obs))]
+    params = _chain_params(circ, [sets[i] for i in range(len(sets)) for _ in obs])
+    b = BatchPropagator(ops, [], cutoff=8, lower_atol=1e-8)
+    majoranas, mapping, gen_coeffs = b._layers(circ)
+    done = b._sim.propagate(majoranas, mapping, gen_coeffs, params, 1 << 15)
+    assert 0 < done < len(majoranas)
+    w, c, m = b._sim.rows()
+    b2 = BatchPropagator._from_rows(b._spec, b._cores, w, c, m)
+    del b
+    left = len(majoranas) - done
+    assert b2._sim.propagate(majoranas[:left], mapping[:left], gen_coeffs[:left], params, 0) == left
+    _check_members(oracle_mod, b2, "pauli", ops, circ, params, cutoff=8, cutoff_type="support", lower=1e-8)
+
+
+def test_expectation_grid_splits_match_oracle(oracle_mod):
+    """A capacity bound of 2^15 rows makes the sweep stop, split the members into halves and resume them;
+    every value still matches the oracle's."""
+    print("case: expectation grid with splits c1", flush=True)
+    circ, sets, obs = _c1_members()
+    params = _chain_params(circ, sets)
+    stats = []
+    grid = expectation_grid(circ, obs, params, cutoff=8, lower_atol=1e-8, max_capacity=1 << 15, stats=stats)
+    assert any(s[3] for s in stats), stats
+    for i in range(len(sets)):
+        for j in range(len(obs)):
+            o = _oracle(oracle_mod, "pauli", obs[j], circ, params[i], cutoff=8, cutoff_type="support", lower=1e-8)
+            assert abs(grid[i, j] - o.expectation()) <= 1e-12, (i, j)
+
+
+def test_device_split_then_resume_is_bit_exact(oracle_mod):
+    """A batch stopped by its bound, split on the device into two batches (members 1, 2, 5 move), and both
+    continued: every member's store equals the oracle's, members renumbered in ascending order."""
+    print("case: batch device split c1", flush=True)
+    circ, sets, obs = _c1_members()
+    ops = [obs[j] for i in range(len(sets)) for j in range(len(obs))]
+    params = _chain_params(circ, [sets[i] for i in range(len(sets)) for _ in obs])
+    b = BatchPropagator(ops, [], cutoff=8, lower_atol=1e-8)
+    majoranas, mapping, gen_coeffs = b._layers(circ)
+    done = b._sim.propagate(majoranas, mapping, gen_coeffs, params, 1 << 15)
+    assert 0 < done < len(majoranas)
+    before = b.sizes()
+    move = np.zeros(len(ops), dtype=bool)
+    move[[1, 2, 5]] = True
+    other = b.split(move)
+    assert np.array_equal(other.sizes(), before[move]) and np.array_equal(b.sizes(), before[~move])
+    left = len(majoranas) - done
+    for part, sel in ((b, ~move), (other, move)):
+        assert part._sim.propagate(majoranas[:left], mapping[:left], gen_coeffs[:left], params[sel], 0) == left
+        _check_members(oracle_mod, part, "pauli", [o for o, f in zip(ops, sel) if f], circ, params[sel], cutoff=8,
+                       cutoff_type="support", lower=1e-8)
