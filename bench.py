@@ -11,11 +11,12 @@ circuits (their generator lists are built once, before timing).  The roofline fi
 kernel's algorithmic bytes over its HIP-event-timed device duration, from one extra step of the same
 workload run with the per-launch profiler on (kept out of the timed steps).  `gradient` reports the
 other half of the metric, the full-angle gradient's time against one forward replay, on the 64-qubit
-ring brickwork (2304 angles), measured after the timed steps.  The CPU baseline is the oracle
+ring brickwork (2304 angles), measured after the timed steps; `batch` times a parameter sweep (sets x
+eight observables, 10 Trotter steps) batched against one propagation per pair.  The CPU baseline is the oracle
 (a serial restatement of the reference algorithm) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--trotter-steps 20] [--no-cpu-baseline]
-       [--no-gradient]
+       [--no-gradient] [--batch-sets 4]
 """
 
 from __future__ import annotations
@@ -46,6 +47,9 @@ def parse_args():
     ap.add_argument("--lattice", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--batch-sets", type=int, default=4, help="parameter sets of the batched sweep (0: skip)")
+    ap.add_argument("--batch-no-solo", action="store_true", help="skip the one-propagation-per-pair comparison")
+    ap.add_argument("--only-batch", action="store_true", help="run and print only the batched sweep")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the known-size copy (counter scale)")
     ap.add_argument("--traffic-from", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "FETCH", "WRITE"),
                     help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
@@ -157,6 +161,58 @@ def gradient_measurement(reps: int = 3):
     }
 
 
+def batch_measurement(n_sets: int, solo: bool = True):
+    """Parameter sweep over the 10x10 kicked-Ising lattice, 10 Trotter steps, cutoff 8, lower_atol 1e-5:
+    the first `n_sets` (theta_J, theta_h) sets of default_rng(2) x the eight seeded observables, once
+    through the batched sweep driver and once as one propagation per pair."""
+    from monoprop_amd import workloads
+    from monoprop_amd._engine import DeviceSimulator
+    from monoprop_amd.batch import expectation_grid
+
+    n = 100
+    circ = workloads.kicked_ising_lattice(10, 10)
+    obs = workloads.sweep_observables()
+    rows = workloads.kicked_ising_rows(circ, workloads.sweep_parameter_sets()[:n_sets])
+    stats = []
+    t_start = time.perf_counter()
+
+    def progress(e):  # one line per propagate call, so a long sweep keeps reporting
+        print(f"[{time.perf_counter() - t_start:7.1f} s] batch of {e[0]} members: {e[1]} terms, {e[2]:.0f} ms"
+              f"{', split' if e[3] else ''}", file=sys.stderr, flush=True)
+
+    grid = expectation_grid(circ, obs, rows, cutoff=8, lower_atol=1e-5, stats=stats, progress=progress)
+    t_batch = time.perf_counter() - t_start
+    tg = sum(s[4] for s in stats)
+    out = {
+        "workload": f"10x10 kicked-Ising, 10 Trotter steps, {n_sets} (theta_J, theta_h) sets x 8 observables, "
+                    "cutoff 8, lower_atol 1e-5",
+        "propagations": int(grid.size),
+        "batched_s": t_batch,
+        "batched_term_gates_per_s": tg / t_batch,
+        "batched_device_s": sum(s[2] for s in stats) / 1e3,
+        "propagate_calls": len(stats),
+        "splits": sum(1 for s in stats if s[3]),
+        "term_gate_applications": tg,
+    }
+    if not solo:
+        return out
+    monos, gcs, pm, _ = layers_of(circ)
+    ref, t_solo, tg_solo = np.empty_like(grid), 0.0, 0
+    for i in range(n_sets):
+        for j, o in enumerate(obs):
+            t0 = time.perf_counter()
+            d = DeviceSimulator(o.get_local_operator().terms, 8, [], lower_atol=1e-5, cutoff_type="support",
+                                logical_num_modes=n, basis="pauli")
+            d.propagate(monos, pm, gcs, list(rows[i]))
+            ref[i, j] = d.expectation_value([])
+            t_solo += time.perf_counter() - t0
+            tg_solo += int(d.gate_stats()[0].sum())
+            del d
+    out.update(solo_s=t_solo, solo_term_gates_per_s=tg_solo / t_solo,
+               max_abs_diff_vs_solo=float(np.max(np.abs(grid - ref))))
+    return out
+
+
 COPY_BYTES = 256 << 20
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_kicked_ising_10x10.json"
 
@@ -228,6 +284,9 @@ def main():
     from monoprop_amd import _engine
 
     _engine.load_library()
+    if args.only_batch:
+        print(json.dumps({"batch": batch_measurement(args.batch_sets, solo=not args.batch_no_solo)}))
+        return
     inputs = build_inputs(args.lattice, args.trotter_steps)
     torch.cuda.synchronize()
     for _ in range(args.warmup):
@@ -306,6 +365,8 @@ def main():
     }
     if not args.no_gradient:
         line["gradient"] = gradient_measurement()
+    if args.batch_sets > 0:
+        line["batch"] = batch_measurement(args.batch_sets, solo=not args.batch_no_solo)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.lattice, args.cpu_sample_steps)
     print(json.dumps(line))

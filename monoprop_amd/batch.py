@@ -176,8 +176,8 @@ def expectation_grid(circuit: Circuit, operators: Sequence[Operator], parameter_
     that gate as separate batches (depth first; the waiting half stays on the device at its exact
     size).  A single
     member is never bounded.  ``stats``, when given, receives one ``(members, stored terms, device
-    ms, split)`` tuple per propagate call, ``split`` true when the call stopped at the bound;
-    ``progress``, when given, is called with each such tuple.
+    ms, split, term-gate applications)`` tuple per propagate call, ``split`` true when the call
+    stopped at the bound; ``progress``, when given, is called with each such tuple.
     """
     params = np.asarray(parameter_sets, dtype=np.float64)
     if params.ndim != 2:
@@ -205,7 +205,7 @@ def expectation_grid(circuit: Circuit, operators: Sequence[Operator], parameter_
         done = b._sim.propagate(majoranas[:left], mapping[:left], gen_coeffs[:left], params[[i for i, _ in chunk]],
                                 bound)
         left -= done
-        entry = (len(chunk), b.size(), b.last_propagate_ms(), left > 0)
+        entry = (len(chunk), b.size(), b.last_propagate_ms(), left > 0, int(b.gate_stats()[0].sum()))
         if stats is not None:
             stats.append(entry)
         if progress is not None:

@@ -86,6 +86,10 @@ class _FakeBatch:
     def last_propagate_ms(self):
         return 0.0
 
+    def gate_stats(self):
+        z = np.zeros(N_LAYERS, dtype=np.int64)
+        return z, z, z, z
+
 
 @pytest.fixture
 def fake(monkeypatch):
