@@ -864,6 +864,13 @@ constexpr uint32_t kMaxGrid = 1024;                       // resolve blocks that
 constexpr uint32_t kCntBits = 16;
 constexpr uint64_t kCntMask = (uint64_t{1} << kCntBits) - 1;
 
+// Lane l's x as a wave-uniform (scalar) value; l must be wave-uniform.
+__device__ __forceinline__ uint64_t lane_value(uint64_t x, int l) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x >> 32)), l));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Wave-level LDS hand-off between lanes of one wavefront.
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -876,20 +883,21 @@ __device__ __forceinline__ void wave_lds_sync() {
 //   A  G's fold and pivot columns of the tile's 16 words, all loads in flight (lane = slot * 16 + word),
 //      XOR-reduced across lanes -> anticommuting mask a_j, pivot mask p_j
 //   B  c <- c k on every anticommuting term (the reference's in-place sweep); lane b owns bit b of every
-//      word, so at most 16 loads per lane, all in flight.  Terms past the lower_atol gate keep their
-//      pre-gate value in vsrc[row] and go to the tile's candidate list (LDS)
+//      word, so at most 16 loads per lane, all in flight; the pre-gate values stay in the lane's
+//      registers.  Terms past the lower_atol gate go to the tile's candidate list (LDS)
 //   E  candidates dense over the lanes: partner key m ^ G against the structural cutoff (or the
 //      upper_atol rescue) -> emit bit (LDS)
-//   D  lane j lists word j's emitters into the tile's region [1024 tile, ...): leaders ascending, then
-//      followers ascending (pivot clear / set); the tile's counts, also summed per group of tiles;
-//      leader emit masks per word
+//   D  the tile's region [1024 tile, ...) of t_item (row) and t_val (pre-gate value) lists the emitters,
+//      leaders ascending then followers ascending (pivot clear / set): lane b writes those of bit b of
+//      every word, with the values it kept from B, at positions from per-word popcounts.  The tile's
+//      counts, also summed per group of tiles; leader emit masks per word
 //   BATCH  rows belong to batch members (member[row]); each term takes its member's k and |s|
 //          (gtrig[member]) instead of the gate's
 template <int NW, int PAULI, int BATCH>
 __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
                                                uint64_t colstride, const int32_t *__restrict__ fold, int nfold,
                                                int piv0, int piv1, const uint64_t *__restrict__ rows,
-                                               double *__restrict__ coef, double *__restrict__ vsrc,
+                                               double *__restrict__ coef, double *__restrict__ t_val,
                                                uint32_t *__restrict__ t_item, uint64_t *__restrict__ elead,
                                                uint64_t *__restrict__ tcnt, uint64_t N, uint64_t tile,
                                                uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx,
@@ -943,13 +951,13 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     if (lane < kWTileWords) sem[lane] = 0;
     // ---- B: lane owns bit `lane` of each word
     uint32_t cand = 0, resc = 0;  // candidates; those the upper_atol rescue emits regardless
+    double v[kWTileWords];        // pre-gate values, kept for D
     {
-        double v[kWTileWords];
         uint32_t q[BATCH ? kWTileWords : 1];
         uint32_t act = 0;
 #pragma unroll
         for (int j = 0; j < kWTileWords; ++j) {
-            const uint64_t aj = __shfl(a, j, 64);
+            const uint64_t aj = lane_value(a, j);
             v[j] = 0.0;
             if (BATCH) q[j] = 0;
             if ((aj >> lane) & 1u) {
@@ -967,7 +975,6 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
             coef[r] = v[j] * k;
             if (!(tr.check_lower && abs_s * fabs(v[j]) <= tr.lower)) {
                 cand |= 1u << j;
-                vsrc[r] = v[j];
                 if (tr.check_upper && abs_s * fabs(v[j]) >= tr.upper) resc |= 1u << j;
             }
         }
@@ -1006,24 +1013,38 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
         }
     }
     wave_lds_sync();
-    // ---- D: lane j < 16 owns word j
-    const uint64_t em = lane < kWTileWords ? sem[lane] : 0;
-    const uint64_t sel = em & ~p, sef = em & p;
-    const uint32_t cl = static_cast<uint32_t>(__popcll(sel)), cf = static_cast<uint32_t>(__popcll(sef));
-    const uint32_t il = wave_incl_scan(cl), iff = wave_incl_scan(cf);
-    const uint32_t nl = __shfl(il, 63, 64), nf = __shfl(iff, 63, 64);
+    // ---- D: per word j the emit mask sem[j] split by the pivot mask p_j (lanes j, j + 16, ... hold it)
+    uint32_t nl = 0, nf = 0;
+#pragma unroll
+    for (int j = 0; j < kWTileWords; ++j) {
+        const uint64_t em = sem[j], pj = lane_value(p, j);
+        nl += static_cast<uint32_t>(__popcll(em & ~pj));
+        nf += static_cast<uint32_t>(__popcll(em & pj));
+    }
     const uint32_t na = wave_sum(lane < kWTileWords ? static_cast<uint32_t>(__popcll(a)) : 0u);
-    if (lane < kWTileWords && w0 + lane < nwords) elead[w0 + lane] = sel;
+    if (lane < kWTileWords && w0 + lane < nwords) elead[w0 + lane] = sem[lane] & ~p;
     if (lane == 0)
         tcnt[tile] = static_cast<uint64_t>(nl) | (static_cast<uint64_t>(nf) << kCntBits) |
                      (static_cast<uint64_t>(na) << (2 * kCntBits));
-    uint32_t *region = t_item + tile * kWTileRows;
-    uint32_t k = il - cl;
-    for (uint64_t bits = sel; bits; bits &= bits - 1)
-        region[k++] = static_cast<uint32_t>(row0 + lane * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
-    k = nl + iff - cf;
-    for (uint64_t bits = sef; bits; bits &= bits - 1)
-        region[k++] = static_cast<uint32_t>(row0 + lane * 64 + __ffsll(static_cast<unsigned long long>(bits)) - 1);
+    if (nl + nf) {
+        uint32_t *region = t_item + tile * kWTileRows;
+        double *vregion = t_val + tile * kWTileRows;
+        const uint64_t below = (uint64_t{1} << lane) - 1;
+        uint32_t lpre = 0, fpre = nl;  // list positions before word j: leaders, followers
+#pragma unroll
+        for (int j = 0; j < kWTileWords; ++j) {
+            const uint64_t em = sem[j], pj = lane_value(p, j);
+            const uint64_t sel = em & ~pj, sef = em & pj;
+            if ((em >> lane) & 1u) {
+                const uint32_t pos = ((pj >> lane) & 1u) ? fpre + static_cast<uint32_t>(__popcll(sef & below))
+                                                         : lpre + static_cast<uint32_t>(__popcll(sel & below));
+                region[pos] = static_cast<uint32_t>(row0 + static_cast<uint64_t>(j) * 64 + lane);
+                vregion[pos] = v[j];
+            }
+            lpre += static_cast<uint32_t>(__popcll(sel));
+            fpre += static_cast<uint32_t>(__popcll(sef));
+        }
+    }
     return static_cast<uint64_t>(nl) | (static_cast<uint64_t>(nf) << kGrpBits) |
            (static_cast<uint64_t>(na) << (2 * kGrpBits));
 }
@@ -1035,7 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
                                                   const uint64_t *__restrict__ cols, uint64_t colstride,
                                                   const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
-                                                  double *__restrict__ vsrc, uint32_t *__restrict__ t_item,
+                                                  double *__restrict__ t_val, uint32_t *__restrict__ t_item,
                                                   uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
                                                   uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
                                                   int group_log2, uint64_t *__restrict__ counters,
@@ -1053,7 +1074,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
     const uint64_t tile = tile0 + wave;
     uint64_t grp = 0;
     if (tile * kWTileWords < nwords)
-        grp = sweep_tile<NW, PAULI, BATCH>(gdesc[gi], tr, cols, colstride, fold, nfold, piv0, piv1, rows, coef, vsrc,
+        grp = sweep_tile<NW, PAULI, BATCH>(gdesc[gi], tr, cols, colstride, fold, nfold, piv0, piv1, rows, coef, t_val,
                                            t_item, elead, tcnt, N, tile, sem[wave], cidx[wave], member, gtrig);
     if (lane_id() == 0) s_grp[wave] = grp;
     __syncthreads();
@@ -1090,7 +1111,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     const uint32_t *__restrict__ t_item,
                                                     const uint64_t *__restrict__ tcnt, const uint64_t *__restrict__ bsum,
                                                     const uint64_t *__restrict__ gsum, uint64_t *__restrict__ gsum_next,
-                                                    uint64_t n_groups_next, const double *__restrict__ vsrc,
+                                                    uint64_t n_groups_next, const double *__restrict__ t_val,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
                                                     uint64_t gmask, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
@@ -1167,6 +1188,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
         // group of f: binary search over the group prefixes (LDS); then the group's sweep-block sums
         // and the block's tile counts (two rounds of independent loads, however sparse the emitters)
         uint32_t S = 0;
+        uint64_t sidx = 0;
+        double vS = 0.0;
         if (valid) {
             uint32_t lo = 0, hi = ng;
             while (hi - lo > 1) {
@@ -1221,14 +1244,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                     rem -= own;
                 }
             }
-            S = t_item[tile * kWTileRows + (leader ? 0u : nl_t) + rem];
+            sidx = tile * kWTileRows + (leader ? 0u : nl_t) + rem;
+            S = t_item[sidx];
+            vS = t_val[sidx];
         }
         // ---- resolution
         uint64_t t[NW];
         double ins = 0.0;
         uint32_t miss = 0, hit = 0, q = 0;
         if (valid) {
-            const double vS = vsrc[S];
             if (BATCH) q = member[S];
             uint64_t m[NW];
             load_row<NW>(rows, S, m);
@@ -1720,9 +1744,10 @@ struct mpa_engine {
     uint64_t gcos_total = 0, grot_total = 0;  // records stored
     bool building = false;                    // seedc tracks capacity while a graph is built
 
-    // Fused pipeline state: global row index, the emitter list with pre-gate values (cap / 2 entries:
-    // leaders from the front, followers from the back), leader emit masks, look-back descriptors.
-    DevArray gidx{&stream}, f_elead{&stream}, f_vsrc{&stream}, t_item{&stream}, t_cnt{&stream}, t_bsum{&stream},
+    // Fused pipeline state: global row index; per 1024-term sweep tile a region of emitter rows (t_item)
+    // and their pre-gate values (t_val); tile, sweep-block and group counts; leader emit masks per word;
+    // look-back descriptors.
+    DevArray gidx{&stream}, f_elead{&stream}, t_val{&stream}, t_item{&stream}, t_cnt{&stream}, t_bsum{&stream},
         t_gsum{&stream}, desc_m{&stream};
     uint64_t gsum_stride = 0;  // per gate parity
     int group_log2 = kMinGroupLog2;
@@ -1860,7 +1885,7 @@ struct mpa_engine {
     // Gives back the arrays sized by the capacity other than the store itself (the fused pipeline's,
     // the row index, the per-gate pipeline's); they are sized again by the next propagate.
     void release_work_arrays() {
-        for (DevArray *a : {&gidx, &f_elead, &f_vsrc, &t_item, &t_cnt, &t_bsum, &t_gsum, &desc_m, &AL, &lead, &foll,
+        for (DevArray *a : {&gidx, &f_elead, &t_val, &t_item, &t_cnt, &t_bsum, &t_gsum, &desc_m, &AL, &lead, &foll,
                             &blk_a, &blk_b})
             a->release();
         fused_cap = 0;
@@ -1879,7 +1904,7 @@ struct mpa_engine {
     // Device bytes per row of capacity: store, column index, fused-pipeline arrays, row index at two
     // slots per row, per-gate pipeline item list.
     double bytes_per_row() const {
-        return 8.0 * nw + 8 + ncols / 8.0 + 8 + (nbatch ? 2 : 0) + 2 + 16.0 * (nw + 1) + 4 + 0.25;
+        return 8.0 * nw + 8 + ncols / 8.0 + (nbatch ? 2 : 0) + 6 + 16.0 * (nw + 1) + 4 + 0.25;
     }
 
     void grow_store(uint64_t need) {
@@ -2246,7 +2271,7 @@ struct mpa_engine {
                                "use the per-gate pipeline (mpa_set_pipeline(e, 0))");
             t_gsum.reserve(2 * gsum_stride * 8);
             HIP_TRY(hipMemsetAsync(t_gsum.p, 0, t_gsum.bytes, stream));  // the sweep accumulates into it
-            f_vsrc.reserve(cap * 8);
+            t_val.reserve(tiles * kWTileRows * 8);
             f_elead.reserve((cap / 64) * 8);
             desc_m.reserve((cap_items / kResChunk + 2) * 8);
             clear_descriptors();
@@ -2328,6 +2353,7 @@ struct mpa_engine {
         HIP_TRY(hipEventCreate(&ev[1]));
         HIP_TRY(hipEventRecord(ev[0], stream));
         size_t gi = 0;
+        double rate = 1.0;  // per-gate growth of the live count over the last chunk
         while (gi < ng) {
             if (2 * N > cap) {
                 const uint64_t need = std::max<uint64_t>(3 * N, 64ull * kBlock);
@@ -2335,7 +2361,15 @@ struct mpa_engine {
                 ensure_capacity(need);
             }
             ensure_fused_arrays<NW, BATCH>();
-            const size_t end = std::min(ng, gi + kChunk);
+            // enqueue about as many gates as the store takes at the last chunk's growth rate, so that a
+            // device stop leaves few empty launches behind
+            size_t span = kChunk;
+            if (rate > 1.0 && N > 0) {
+                const double room = std::log(static_cast<double>(cap) / (2.0 * static_cast<double>(N)));
+                span = static_cast<size_t>(std::clamp(room / std::log(rate) + 8.0, 16.0, static_cast<double>(kChunk)));
+            }
+            const size_t end = std::min(ng, gi + span);
+            const uint64_t n_start = N;
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
             const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
             const uint32_t rgrid =
@@ -2363,7 +2397,7 @@ struct mpa_engine {
                 hipLaunchKernelGGL((k_sweep<NW, PAULI, BATCH>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
                                    ig, tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off,
                                    gd.fold_cnt, gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(),
-                                   f_vsrc.as<double>(), t_item.as<uint32_t>(), f_elead.as<uint64_t>(),
+                                   t_val.as<double>(), t_item.as<uint32_t>(), f_elead.as<uint64_t>(),
                                    t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur, group_log2,
                                    counters.as<uint64_t>(), mem, gtrig);
                 HIP_TRY(hipGetLastError());
@@ -2373,7 +2407,7 @@ struct mpa_engine {
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
-                                   gsum_stride, f_vsrc.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
+                                   gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
                                    static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig);
                 HIP_TRY(hipGetLastError());
@@ -2389,6 +2423,10 @@ struct mpa_engine {
             const uint64_t stop = static_cast<uint64_t>(h_counters[C_STOP]);
             const size_t next = stop ? static_cast<size_t>(stop - 1) : end;
             N = static_cast<uint64_t>(h_counters[C_NB + (next & 1)]);
+            if (next > gi && n_start > 0 && N > n_start)
+                rate = std::pow(static_cast<double>(N) / static_cast<double>(n_start), 1.0 / static_cast<double>(next - gi));
+            else
+                rate = 1.0;
             gidx_rows = N;
             if (N >= (uint64_t{1} << 31)) throw RunError("term count exceeds the 32-bit row index");
             reset_fused_counters(next);
