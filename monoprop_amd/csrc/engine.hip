@@ -859,7 +859,8 @@ constexpr uint32_t kMaxGroups = 4096;
 constexpr uint32_t kGrpBits = 21;
 constexpr uint64_t kGrpMask = (uint64_t{1} << kGrpBits) - 1;
 constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
-constexpr uint32_t kMaxGrid = 1024;                       // resolve blocks that take chunks
+constexpr uint32_t kMaxGrid = 1280;                       // resolve blocks that take chunks (5 per CU)
+constexpr int kBsumBatch = 8;                             // resolve: sweep-block sums read per round
 // per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
 constexpr uint32_t kCntBits = 16;
 constexpr uint64_t kCntMask = (uint64_t{1} << kCntBits) - 1;
@@ -1098,7 +1099,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
 //              writes the new term's coefficient k*0 + (s ph) v
 //   append     a chunk's misses land at N + (misses of earlier chunks) by look-back; new terms enter
 //              the row index, and their column bits are gathered per word in LDS before one OR per
-//              word
+//              word.  A block appends a chunk one iteration late: it resolves its next chunk first
+//              (misses staged in LDS), so the look-back rarely waits on chunks still resolving
 // The chunk holding the last emitter publishes N + I for the next gate and stops the device before a
 // gate whose worst case could outgrow the store or the index.  Every block also clears its share of
 // the other gate parity's group counts.
@@ -1124,6 +1126,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
     extern __shared__ uint32_t dyn_lds[];  // group prefixes: leaders [0, max_groups], followers after
     __shared__ uint32_t lds[kWaves];
     __shared__ uint32_t s_ticket, s_base;
+    __shared__ uint32_t st_row[2][kResChunk];  // a chunk's misses: emitter row, new coefficient
+    __shared__ double st_ins[2][kResChunk];
     uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
     if (counters[C_STOP]) return;
@@ -1170,163 +1174,185 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
         if (nchunks == 0) counters[C_NB + nxt] = N;
     }
     if (blockIdx.x >= nchunks) return;  // only blocks with work take tickets
+    // One chunk in flight behind the current one: chunk c is resolved and its misses staged (LDS) while
+    // the previous chunk of this block, whose look-back has had chunk c's time to settle, is appended.
+    uint32_t pend = 0xffffffffu, pend_miss = 0, buf = 0;
     for (;;) {
-        __syncthreads();  // LDS of the previous chunk fully consumed (and the group prefixes written)
+        __syncthreads();  // LDS of the previous iteration fully consumed (and the group prefixes written)
         if (threadIdx.x == 0)
             s_ticket = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
         __syncthreads();
         const uint32_t chunk = s_ticket;
-        if (chunk >= nchunks) break;
-        // ---- positions: this thread's emitter f of side L (leaders) or F (followers)
-        const bool leader = chunk < cL;
-        const uint32_t *gp = leader ? gpl : gpf;
-        const uint32_t total = leader ? nEL : nEF;
-        const uint32_t first = (leader ? chunk : chunk - cL) * kResChunk;
-        const uint32_t cnt = std::min<uint32_t>(kResChunk, total - first);
-        const uint32_t f = first + threadIdx.x;
-        const bool valid = threadIdx.x < cnt;
-        // group of f: binary search over the group prefixes (LDS); then the group's sweep-block sums
-        // and the block's tile counts (two rounds of independent loads, however sparse the emitters)
-        uint32_t S = 0;
-        uint64_t sidx = 0;
-        double vS = 0.0;
-        if (valid) {
-            uint32_t lo = 0, hi = ng;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (gp[mid] <= f) lo = mid;
-                else hi = mid;
-            }
-            uint32_t rem = f - gp[lo];
-            const uint32_t blocks_per_group = 1u << (group_log2 - 2);
-            const uint64_t b0 = static_cast<uint64_t>(lo) << (group_log2 - 2);
-            uint64_t blk = b0;
-            for (uint32_t base = 0; base < blocks_per_group; base += 16) {
-                uint32_t cb[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const uint64_t bb = b0 + base + k;
-                    const uint64_t v = bb < nblocks ? bsum[bb] : 0;
-                    cb[k] = static_cast<uint32_t>(leader ? (v & kGrpMask) : ((v >> kGrpBits) & kGrpMask));
+        const bool have = chunk < nchunks;
+        if (!have && pend == 0xffffffffu) break;
+        uint32_t n_miss = 0;
+        if (have) {
+            // ---- positions: this thread's emitter f of side L (leaders) or F (followers)
+            const bool leader = chunk < cL;
+            const uint32_t *gp = leader ? gpl : gpf;
+            const uint32_t total = leader ? nEL : nEF;
+            const uint32_t first = (leader ? chunk : chunk - cL) * kResChunk;
+            const uint32_t cnt = std::min<uint32_t>(kResChunk, total - first);
+            const uint32_t f = first + threadIdx.x;
+            const bool valid = threadIdx.x < cnt;
+            // group of f: binary search over the group prefixes (LDS); then the group's sweep-block sums
+            // and the block's tile counts (two rounds of independent loads, however sparse the emitters)
+            uint32_t S = 0;
+            uint64_t sidx = 0;
+            double vS = 0.0;
+            if (valid) {
+                uint32_t lo = 0, hi = ng;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (gp[mid] <= f) lo = mid;
+                    else hi = mid;
                 }
+                uint32_t rem = f - gp[lo];
+                const uint32_t blocks_per_group = 1u << (group_log2 - 2);
+                const uint64_t b0 = static_cast<uint64_t>(lo) << (group_log2 - 2);
+                uint64_t blk = b0;
+                for (uint32_t base = 0; base < blocks_per_group; base += kBsumBatch) {
+                    uint32_t cb[kBsumBatch];
+#pragma unroll
+                    for (int k = 0; k < kBsumBatch; ++k) {
+                        const uint64_t bb = b0 + base + k;
+                        const uint64_t v = bb < nblocks ? bsum[bb] : 0;
+                        cb[k] = static_cast<uint32_t>(leader ? (v & kGrpMask) : ((v >> kGrpBits) & kGrpMask));
+                    }
+                    bool found = false;
+#pragma unroll
+                    for (int k = 0; k < kBsumBatch; ++k) {
+                        if (found) continue;
+                        if (rem < cb[k]) {
+                            blk = b0 + base + k;
+                            found = true;
+                        } else {
+                            rem -= cb[k];
+                        }
+                    }
+                    if (found) break;
+                }
+                uint64_t tc[kWaves];
+#pragma unroll
+                for (int k = 0; k < kWaves; ++k) {
+                    const uint64_t tt = blk * kWaves + k;
+                    tc[k] = tt < ntiles ? tcnt[tt] : 0;
+                }
+                uint64_t tile = blk * kWaves;
+                uint32_t nl_t = 0;
                 bool found = false;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
+                for (int k = 0; k < kWaves; ++k) {
                     if (found) continue;
-                    if (rem < cb[k]) {
-                        blk = b0 + base + k;
+                    const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
+                    const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
+                    if (rem < own) {
+                        tile = blk * kWaves + k;
+                        nl_t = nl1;
                         found = true;
                     } else {
-                        rem -= cb[k];
+                        rem -= own;
                     }
                 }
-                if (found) break;
+                sidx = tile * kWTileRows + (leader ? 0u : nl_t) + rem;
+                S = t_item[sidx];
+                vS = t_val[sidx];
             }
-            uint64_t tc[kWaves];
+            // ---- resolution
+            double ins = 0.0;
+            uint32_t miss = 0, hit = 0;
+            if (valid) {
+                const uint32_t q = BATCH ? member[S] : 0u;
+                uint64_t m[NW], t[NW];
+                load_row<NW>(rows, S, m);
 #pragma unroll
-            for (int k = 0; k < kWaves; ++k) {
-                const uint64_t tt = blk * kWaves + k;
-                tc[k] = tt < ntiles ? tcnt[tt] : 0;
-            }
-            uint64_t tile = blk * kWaves;
-            uint32_t nl_t = 0;
-            bool found = false;
-#pragma unroll
-            for (int k = 0; k < kWaves; ++k) {
-                if (found) continue;
-                const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
-                const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
-                if (rem < own) {
-                    tile = blk * kWaves + k;
-                    nl_t = nl1;
-                    found = true;
-                } else {
-                    rem -= own;
+                for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
+                const int64_t T = gidx_find_below<NW>(gidx, gmask, t, N, q);
+                const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
+                if (!skip) {
+                    const double gk = BATCH ? gtrig[q].k : gd.k;
+                    const double gs = BATCH ? gtrig[q].s : gd.s;
+                    const double ginv = BATCH ? gtrig[q].inv_k : gd.inv_k;
+                    const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
+                    const double wS = vS * gk;
+                    if (T >= 0) {
+                        const double wT = coef[T];
+                        const double vTu = wT * ginv;
+                        coef[S] = wS + gs * static_cast<double>(-ph) * vTu;
+                        coef[T] = wT + gs * static_cast<double>(ph) * vS;
+                        hit = 1;
+                    } else {
+                        coef[S] = wS + gs * static_cast<double>(-ph) * 0.0;
+                        ins = gk * 0.0 + gs * static_cast<double>(ph) * vS;
+                        miss = 1;
+                    }
                 }
             }
-            sidx = tile * kWTileRows + (leader ? 0u : nl_t) + rem;
-            S = t_item[sidx];
-            vS = t_val[sidx];
-        }
-        // ---- resolution
-        uint64_t t[NW];
-        double ins = 0.0;
-        uint32_t miss = 0, hit = 0, q = 0;
-        if (valid) {
-            if (BATCH) q = member[S];
-            uint64_t m[NW];
-            load_row<NW>(rows, S, m);
-#pragma unroll
-            for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-            const int64_t T = gidx_find_below<NW>(gidx, gmask, t, N, q);
-            const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
-            if (!skip) {
-                const double gk = BATCH ? gtrig[q].k : gd.k;
-                const double gs = BATCH ? gtrig[q].s : gd.s;
-                const double ginv = BATCH ? gtrig[q].inv_k : gd.inv_k;
-                const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
-                const double wS = vS * gk;
-                if (T >= 0) {
-                    const double wT = coef[T];
-                    const double vTu = wT * ginv;
-                    coef[S] = wS + gs * static_cast<double>(-ph) * vTu;
-                    coef[T] = wT + gs * static_cast<double>(ph) * vS;
-                    hit = 1;
-                } else {
-                    coef[S] = wS + gs * static_cast<double>(-ph) * 0.0;
-                    ins = gk * 0.0 + gs * static_cast<double>(ph) * vS;
-                    miss = 1;
-                }
-            }
-        }
-        for (int k = threadIdx.x; k < kColWords * NC; k += kBlock) lcol[k] = 0;
-        uint32_t n_miss;
-        const uint32_t rank = block_excl_scan(miss, lds, &n_miss);
-        const uint32_t n_hit = block_sum(hit, lds);
-        if (threadIdx.x < 64) {
-            uint32_t excl = 0;
-            if (chunk == 0) {
-                if (threadIdx.x == 0) desc_publish(desc_m, ep, kDescIncl, n_miss);
-            } else {
-                if (threadIdx.x == 0) desc_publish(desc_m + chunk, ep, kDescAgg, n_miss);
-                excl = look_back(desc_m, chunk, ep, counters);
-                if (threadIdx.x == 0) desc_publish(desc_m + chunk, ep, kDescIncl, excl + n_miss);
+            const uint32_t rank = block_excl_scan(miss, lds, &n_miss);
+            const uint32_t n_hit = block_sum(hit, lds);
+            if (miss) {  // staged in the chunk's rank order
+                st_row[buf][rank] = S;
+                st_ins[buf][rank] = ins;
             }
             if (threadIdx.x == 0) {
-                s_base = excl;
+                desc_publish(desc_m + chunk, ep, chunk == 0 ? kDescIncl : kDescAgg, n_miss);
                 if (n_hit) atomicAdd(reinterpret_cast<unsigned long long *>(st + 5), static_cast<unsigned long long>(n_hit));
-                if (chunk == nchunks - 1) {
-                    const uint64_t I = excl + n_miss, n = N + I;
-                    st[3] = static_cast<int64_t>(I);
-                    counters[C_NB + nxt] = n;
-                    if (2 * n > cap_rows || 2 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+            }
+        }
+        if (pend != 0xffffffffu) {
+            // ---- append the previous chunk: its misses land at N + (misses of earlier chunks) + rank
+            const uint32_t pb = buf ^ 1u;
+            if (threadIdx.x < 64) {
+                uint32_t excl = 0;
+                if (pend != 0) {
+                    excl = look_back(desc_m, pend, ep, counters);
+                    if (threadIdx.x == 0) desc_publish(desc_m + pend, ep, kDescIncl, excl + pend_miss);
+                }
+                if (threadIdx.x == 0) {
+                    s_base = excl;
+                    if (pend == nchunks - 1) {
+                        const uint64_t I = excl + pend_miss, n = N + I;
+                        st[3] = static_cast<int64_t>(I);
+                        counters[C_NB + nxt] = n;
+                        if (2 * n > cap_rows || 2 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+                    }
+                }
+            }
+            for (int k = threadIdx.x; k < kColWords * NC; k += kBlock) lcol[k] = 0;
+            __syncthreads();
+            const uint64_t first_row = N + s_base, wlo = first_row >> 6;
+            if (threadIdx.x < pend_miss) {
+                const uint32_t S = st_row[pb][threadIdx.x];
+                const uint32_t q = BATCH ? member[S] : 0u;
+                uint64_t t[NW];
+                load_row<NW>(rows, S, t);
+#pragma unroll
+                for (int k = 0; k < NW; ++k) t[k] ^= gd.g[k];
+                const uint64_t dest = first_row + threadIdx.x;
+                uint64_t *pd = rows + dest * NW;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) pd[k] = t[k];
+                coef[dest] = st_ins[pb][threadIdx.x];
+                if (BATCH) member[dest] = static_cast<uint16_t>(q);
+                gather_column_bits<NW, PAULI>(lcol, ncols, n_modes, t, (dest >> 6) - wlo, uint64_t{1} << (dest & 63));
+                if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest), q))
+                    atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
+            }
+            __syncthreads();
+            if (pend_miss) {
+                const int nwt = static_cast<int>(((first_row + pend_miss - 1) >> 6) - wlo + 1);
+                for (int k = threadIdx.x; k < nwt * ncols; k += kBlock) {
+                    const uint64_t bits = lcol[k];
+                    if (bits)
+                        atomicOr(reinterpret_cast<unsigned long long *>(cols + static_cast<uint64_t>(k % ncols) * colstride +
+                                                                        wlo + k / ncols),
+                                 bits);
                 }
             }
         }
-        __syncthreads();
-        const uint64_t first_row = N + s_base, wlo = first_row >> 6;
-        if (miss) {
-            const uint64_t dest = first_row + rank;
-            uint64_t *pd = rows + dest * NW;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) pd[k] = t[k];
-            coef[dest] = ins;
-            if (BATCH) member[dest] = static_cast<uint16_t>(q);
-            gather_column_bits<NW, PAULI>(lcol, ncols, n_modes, t, (dest >> 6) - wlo, uint64_t{1} << (dest & 63));
-            if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest), q))
-                atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
-        }
-        __syncthreads();
-        if (n_miss) {
-            const int nwt = static_cast<int>(((first_row + n_miss - 1) >> 6) - wlo + 1);
-            for (int k = threadIdx.x; k < nwt * ncols; k += kBlock) {
-                const uint64_t bits = lcol[k];
-                if (bits)
-                    atomicOr(reinterpret_cast<unsigned long long *>(cols + static_cast<uint64_t>(k % ncols) * colstride +
-                                                                    wlo + k / ncols),
-                             bits);
-            }
-        }
+        pend = have ? chunk : 0xffffffffu;
+        pend_miss = n_miss;
+        buf ^= 1u;
     }
 }
 
