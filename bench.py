@@ -3,7 +3,9 @@
 Workload (one "step"): the 10x10 open-lattice kicked-Ising circuit (180 ZZ bonds, row-major), 20
 Trotter steps of ZZ rotations theta_J = -pi/4 then X rotations theta_h, propagated once for each
 theta_h in {0.1, 0.4, 0.8}; observable (1/100) sum_i Z_i, support cutoff 10, lower_atol 1e-5,
-FP64 coefficients, two u64 limbs per plane.  Inputs are synthetic circuits (no data files).
+FP64 coefficients, two u64 limbs per plane.  Inputs are synthetic circuits (no data files).  The three
+propagations run as members of one batched store (each member's terms and coefficients are those of
+its own propagation); `unbatched` times the same steps one propagation at a time.
 
 Metric: term-gate applications per second, T / t with T = sum over gates of the live terms before
 the gate (counted on the device); t covers engine creation, propagation and <O> for the three
@@ -16,7 +18,7 @@ eight observables, 10 Trotter steps) batched against one propagation per pair.  
 (a serial restatement of the reference algorithm) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--trotter-steps 20] [--no-cpu-baseline]
-       [--no-gradient] [--batch-sets 4]
+       [--no-gradient] [--batch-sets 4] [--unbatched]
 """
 
 from __future__ import annotations
@@ -55,6 +57,8 @@ def parse_args():
                     help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
     ap.add_argument("--traffic-out", default=str(TRAFFIC_FILE))
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--unbatched", action="store_true",
+                    help="time one propagation per theta_h (default: the three as members of one store)")
     return ap.parse_args()
 
 
@@ -80,6 +84,24 @@ def build_inputs(L, trotter):
         monos, gcs, pm, _ = layers_of(circ)
         out.append((monos, gcs, pm, list(circ.parameters)))
     return out
+
+
+def run_gpu_step_batched(L, inputs, profile):
+    """One step as one store: the three theta_h propagations are members of one batch (same gates,
+    their own angles), so each gate is one pair of launches for all three.  Returns (term-gate
+    applications, <O> list, [simulator])."""
+    from monoprop_amd._engine import BatchSimulator
+
+    n = L * L
+    monos, gcs, pm, _ = inputs[0]
+    params = np.asarray([p for _, _, _, p in inputs], dtype=np.float64)
+    sim = BatchSimulator([observable_terms(n)] * len(inputs), 10, [], lower_atol=1e-5, cutoff_type="support",
+                         logical_num_modes=n, basis="pauli")
+    if profile:
+        sim.set_profiling(True)
+    sim.propagate(monos, pm, gcs, params)
+    total_t = int(sim.gate_stats()[0].sum())
+    return total_t, [float(v) for v in sim.expectation_values()], [sim]
 
 
 def run_gpu_step(L, inputs, profile):
@@ -288,24 +310,28 @@ def main():
         print(json.dumps({"batch": batch_measurement(args.batch_sets, solo=not args.batch_no_solo)}))
         return
     inputs = build_inputs(args.lattice, args.trotter_steps)
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        run_gpu_step(args.lattice, inputs, profile=False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    total_t = 0
-    evs = None
-    for _ in range(args.steps):
-        t, evs, sims = run_gpu_step(args.lattice, inputs, profile=False)
-        total_t += t
-        del sims
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    step = run_gpu_step if args.unbatched else run_gpu_step_batched
+
+    def timed(step_fn):
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            step_fn(args.lattice, inputs, profile=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        total, evs = 0, None
+        for _ in range(args.steps):
+            t, evs, sims = step_fn(args.lattice, inputs, profile=False)
+            total += t
+            del sims
+        torch.cuda.synchronize()
+        return total, evs, time.perf_counter() - t0
+
+    total_t, evs, elapsed = timed(step)
 
     # per-kernel device time (HIP events around every launch) and algorithmic bytes, from one more step
     # of the same workload run with the profiler on (it is kept out of the timed steps above)
     prof: dict[str, dict[str, float]] = {}
-    _, _, sims = run_gpu_step(args.lattice, inputs, profile=True)
+    _, _, sims = step(args.lattice, inputs, profile=True)
     prof_wall_ms = 0.0
     for sim in sims:
         prof_wall_ms += sim.last_propagate_ms()
@@ -343,6 +369,7 @@ def main():
             "gates_per_propagation": (2 * args.lattice * (args.lattice - 1) + args.lattice ** 2) * args.trotter_steps,
             "term_gate_applications_per_step": total_t // args.steps,
             "expectation": evs,
+            "batched": not args.unbatched,
         },
         "roofline": {
             "bound": "hbm",
@@ -363,6 +390,10 @@ def main():
         "all_kernels_GBps": kern_bytes / (kern_ms * 1e6) if kern_ms > 0 else 0.0,
         "profiled_step": {"propagate_ms": round(prof_wall_ms, 2), "kernel_ms": round(kern_ms, 2)},
     }
+    if not args.unbatched:  # the same steps, one propagation per theta_h, for comparison
+        t_u, evs_u, el_u = timed(run_gpu_step)
+        line["unbatched"] = {"value": t_u / el_u, "ms_per_step": el_u * 1e3 / args.steps,
+                             "max_abs_diff": max(abs(a - b) for a, b in zip(evs, evs_u))}
     if not args.no_gradient:
         line["gradient"] = gradient_measurement()
     if args.batch_sets > 0:

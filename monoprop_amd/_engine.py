@@ -601,3 +601,11 @@ class BatchSimulator:
         _check(load_library().mpa_last_propagate_ms(self._h, C.byref(v)))
         return v.value
 
+    def set_profiling(self, on: bool) -> None:
+        """Bracket every kernel launch with HIP events (resets the accumulators)."""
+        _check(load_library().mpa_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_profile(self) -> dict[str, dict[str, float]]:
+        """Per kernel: summed device ms, algorithmic bytes and launches since profiling was enabled."""
+        return DeviceSimulator.kernel_profile(self)
+
