@@ -59,6 +59,7 @@ def parse_args():
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--unbatched", action="store_true",
                     help="time one propagation per theta_h (default: the three as members of one store)")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the unbatched steps for comparison")
     return ap.parse_args()
 
 
@@ -390,7 +391,7 @@ def main():
         "all_kernels_GBps": kern_bytes / (kern_ms * 1e6) if kern_ms > 0 else 0.0,
         "profiled_step": {"propagate_ms": round(prof_wall_ms, 2), "kernel_ms": round(kern_ms, 2)},
     }
-    if not args.unbatched:  # the same steps, one propagation per theta_h, for comparison
+    if not args.unbatched and not args.no_compare:  # the same steps, one propagation per theta_h, for comparison
         t_u, evs_u, el_u = timed(run_gpu_step)
         line["unbatched"] = {"value": t_u / el_u, "ms_per_step": el_u * 1e3 / args.steps,
                              "max_abs_diff": max(abs(a - b) for a, b in zip(evs, evs_u))}
