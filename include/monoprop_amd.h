@@ -10,7 +10,8 @@
  * tuples (Pauli: X_q -> 2q, Y_q -> 2q+1, Z_q -> {2q, 2q+1}; Majorana: the index itself), with
  * complex coefficients for operators (re, im arrays) and real structural coefficients for
  * generators.  `lower_atol` / `upper_atol` take NaN for "None".  Layer arrays are in authoring
- * order; the engine applies them back to front (Heisenberg picture).
+ * order; the engine applies them back to front (Heisenberg picture) or front to back (Schrodinger
+ * picture, an engine made by mpa_create_schrodinger).
  *
  * Each entry point names the reference interface it replaces.
  */
@@ -39,6 +40,22 @@ int mpa_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int 
                double lower_atol, double upper_atol, const int64_t *term_offsets,
                const int32_t *term_slots, const double *coeff_re, const double *coeff_im,
                int64_t n_terms, const int32_t *initial_state, int64_t n_state, int device);
+
+/* Replaces the same constructor with schrodinger_cutoff set (MonomialPropagator.inl:86-89, 162-181;
+ * binder.h:60-100): the Schrodinger picture.  schrodinger_cutoff is in slots (the reference's
+ * PauliPropagator passes 2 x its qubit value, pauli_propagator.py:71-74).  The engine starts from the
+ * fully paired state terms of at most ceil(schrodinger_cutoff / 2) modes at their <b|M|b>; the given
+ * operator is the observable, matched against the evolved state's terms when <O> is evaluated.  Every
+ * other entry point works on such an engine except the mpa_batch_* ones. */
+int mpa_create_schrodinger(mpa_engine **out, int basis, int num_modes, uint32_t cutoff,
+                           int cutoff_type, double lower_atol, double upper_atol,
+                           int64_t schrodinger_cutoff, const int64_t *term_offsets,
+                           const int32_t *term_slots, const double *coeff_re,
+                           const double *coeff_im, int64_t n_terms, const int32_t *initial_state,
+                           int64_t n_state, int device);
+
+/* Replaces MonomialPropagator::schrodinger (binder.h:191-193): 1 for the Schrodinger picture. */
+int mpa_is_schrodinger(mpa_engine *e, int32_t *out);
 
 /* Releases all device memory of the engine. */
 int mpa_destroy(mpa_engine *e);

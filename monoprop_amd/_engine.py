@@ -33,6 +33,10 @@ _SIGNATURES = {
     "mpa_last_error": (C.c_char_p, []),
     "mpa_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double, C.c_double,
                              _i64p, _i32p, _f64p, _f64p, C.c_int64, _i32p, C.c_int64, C.c_int]),
+    "mpa_create_schrodinger": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double,
+                                         C.c_double, C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_int64, _i32p,
+                                         C.c_int64, C.c_int]),
+    "mpa_is_schrodinger": (C.c_int, [_P, _i32p]),
     "mpa_destroy": (C.c_int, [_P]),
     "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
@@ -151,8 +155,8 @@ class DeviceSimulator:
         lib = load_library()
         if comm is not None:
             raise ValueError("MPI communicators are not supported: the engine holds the whole operator on one GPU.")
-        if schrodinger_cutoff is not None:
-            raise ValueError("The Schrodinger picture is not supported by the HIP engine (Heisenberg only).")
+        if schrodinger_cutoff is not None and (isinstance(schrodinger_cutoff, bool) or int(schrodinger_cutoff) < 0):
+            raise ValueError(f"schrodinger_cutoff must be a non-negative integer; got {schrodinger_cutoff!r}.")
         if basis_change is not None:
             raise ValueError("basis_change is not supported by the HIP engine.")
         if basis not in ("majorana", "pauli"):
@@ -175,11 +179,15 @@ class DeviceSimulator:
         self._lower = lower_atol
         self._upper = upper_atol
         h = _P()
-        _check(lib.mpa_create(C.byref(h), 1 if basis == "pauli" else 0, n, int(cutoff),
-                              0 if cutoff_type == "length" else 1, _nan_if_none(lower_atol),
-                              _nan_if_none(upper_atol), _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
-                              _ptr(re, C.c_double), _ptr(im, C.c_double), len(keys), _ptr(st, C.c_int32),
-                              len(list(initial_state)), default_device()))
+        head = (C.byref(h), 1 if basis == "pauli" else 0, n, int(cutoff), 0 if cutoff_type == "length" else 1,
+                _nan_if_none(lower_atol), _nan_if_none(upper_atol))
+        tail = (_ptr(offs, C.c_int64), _ptr(slots, C.c_int32), _ptr(re, C.c_double), _ptr(im, C.c_double), len(keys),
+                _ptr(st, C.c_int32), len(list(initial_state)), default_device())
+        if schrodinger_cutoff is None:
+            _check(lib.mpa_create(*head, *tail))
+        else:
+            _check(lib.mpa_create_schrodinger(*head, int(schrodinger_cutoff), *tail))
+        self._schrodinger = schrodinger_cutoff is not None
         self._h = h
         self._parameter_mapping: list[int] = []
         self._n_gates = 0
@@ -239,8 +247,14 @@ class DeviceSimulator:
         gates = list(gate_indices) if gate_indices is not None else list(range(len(majoranas)))
         offset = self._n_gates
         self._n_gates = offset + (max(gates) + 1 if gates else 0)
-        # parameter_mapping in optimizer order (reverse of the processing order of the layers)
-        self._parameter_mapping = [int(p) for p in parameter_mapping] + list(self._parameter_mapping)
+        # parameter_mapping in optimizer order, the reverse of the reference's layer order
+        # (MonomialPropagator.inl:855-870): a Heisenberg call's layers go after the stored ones, a
+        # Schrodinger call's before them (MPGraph.h:41-43)
+        new = [int(p) for p in parameter_mapping]
+        if self._schrodinger:
+            self._parameter_mapping = list(self._parameter_mapping) + new
+        else:
+            self._parameter_mapping = new + list(self._parameter_mapping)
 
     # --- evaluation ------------------------------------------------------------------------------
     def expectation_value(self, parameters) -> float:
@@ -420,7 +434,7 @@ class DeviceSimulator:
 
     @property
     def schrodinger(self) -> bool:
-        return False
+        return self._schrodinger
 
     @property
     def basis(self) -> str:

@@ -50,6 +50,7 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kScanBlock = 1024;
 constexpr int kExpectBlocks = 1024;
+constexpr uint64_t kMaxSchrodingerRows = uint64_t{1} << 28;  // initial Schrodinger state terms
 
 // counters[] on the device
 enum : int {
@@ -295,6 +296,18 @@ __device__ __forceinline__ void load_row(const uint64_t *__restrict__ rows, uint
     for (int w = 0; w < NW; ++w) m[w] = p[w];
 }
 
+// Value a new row takes in the Schrodinger picture before the gate rotates it: its diagonal phase
+// against the initial product state if fully paired, else 0 (MPOperator.h:158-181 dense_state()
+// scoring, read back as the insert's v_tgt by FusedApply.h:38-44).  The Heisenberg value is 0.
+template <int NW, int PAULI>
+__device__ __forceinline__ double fresh_score(const uint64_t *t, const uint64_t *__restrict__ occ_dev) {
+    if (!mpa::is_paired<NW, PAULI>(t)) return 0.0;
+    uint64_t occ[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
+    return mpa::state_phase<NW, PAULI>(t, occ);
+}
+
 // Emission decision of one anticommuting term (Scan.h:146-157, 315-347; CutoffContext.h:36-41).
 // bit0 emit, bit1 phase is -1, bit2 term is in the cos set.
 template <int NW, int PAULI>
@@ -390,7 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
                                                       uint64_t epoch, const uint8_t *__restrict__ rec,
                                                       const double *__restrict__ lv, uint8_t *__restrict__ match,
                                                       uint8_t *__restrict__ miss, double *__restrict__ ins,
-                                                      uint64_t *__restrict__ counters, GraphRec grec) {
+                                                      uint64_t *__restrict__ counters, GraphRec grec,
+                                                      const uint64_t *__restrict__ occ_dev) {
     constexpr int SW = NW + 1;
     const GateDesc &gd = gdesc[gi];
     const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
@@ -455,9 +469,10 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
             coef[r] = cF;
         } else {
             const double w = in_cos ? v * gd.k : v;
-            if (emit) {
-                coef[r] = w + gd.s * static_cast<double>(-ph) * 0.0;
-                new_coef = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
+            if (emit) {  // the new partner starts at vT0 (occ_dev set: Schrodinger picture)
+                const double vT0 = occ_dev ? fresh_score<NW, PAULI>(t, occ_dev) : 0.0;
+                coef[r] = w + gd.s * static_cast<double>(-ph) * vT0;
+                new_coef = gd.k * vT0 + gd.s * static_cast<double>(ph) * v;
                 is_miss = static_cast<uint8_t>(1u | (ph < 0 ? 2u : 0u));  // rotation recorded by k_insert
             } else {
                 coef[r] = w;
@@ -484,7 +499,10 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
     }
 }
 
+template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__restrict__ gdesc, int gi,
+                                                              const uint64_t *__restrict__ rows,
+                                                              const uint64_t *__restrict__ occ_dev,
                                                               double *__restrict__ coef,
                                                               const uint32_t *__restrict__ AL, uint32_t nL,
                                                               const uint8_t *__restrict__ rec,
@@ -507,8 +525,16 @@ __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__
             const double w = in_cos ? v * gd.k : v;
             row = AL[b];
             if (e) {
-                coef[row] = w + gd.s * static_cast<double>(-ph) * 0.0;
-                ins[b] = gd.k * 0.0 + gd.s * static_cast<double>(ph) * v;
+                double vT0 = 0.0;
+                if (occ_dev) {
+                    uint64_t t[NW];
+                    load_row<NW>(rows, row, t);
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) t[k] ^= gd.g[k];
+                    vT0 = fresh_score<NW, PAULI>(t, occ_dev);
+                }
+                coef[row] = w + gd.s * static_cast<double>(-ph) * vT0;
+                ins[b] = gd.k * vT0 + gd.s * static_cast<double>(ph) * v;
                 miss[b] = static_cast<uint8_t>(1u | (ph < 0 ? 2u : 0u));
             } else {
                 coef[row] = w;
@@ -534,14 +560,16 @@ __global__ __launch_bounds__(kBlock) void k_count_miss(const uint8_t *__restrict
 }
 
 // Append the partners to create at N + rank (rank over leaders then followers).  `zero_fill`, when
-// set, receives 0 at the new rows (the operator's own coefficients while a graph is being built).
-template <int NW>
+// set, receives the new rows' starting value (the operator's own coefficients while a graph is being
+// built): 0, or with occ_dev set (Schrodinger picture) the rows' state score.
+template <int NW, int PAULI>
 __global__ __launch_bounds__(kBlock) void k_insert(const GateDesc *__restrict__ gdesc, int gi,
                                                    uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                    double *__restrict__ zero_fill, const uint32_t *__restrict__ AL,
                                                    uint32_t nA, const uint8_t *__restrict__ miss,
                                                    const double *__restrict__ ins, const uint32_t *__restrict__ blk,
-                                                   uint64_t *__restrict__ counters, GraphRec grec) {
+                                                   uint64_t *__restrict__ counters, GraphRec grec,
+                                                   const uint64_t *__restrict__ occ_dev) {
     __shared__ uint32_t lds[kWaves];
     const GateDesc &gd = gdesc[gi];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -554,12 +582,16 @@ __global__ __launch_bounds__(kBlock) void k_insert(const GateDesc *__restrict__ 
     if (v) {
         dest = counters[C_N] + blk[blockIdx.x] + rank;
         src = AL[i];
-        const uint64_t *ps = rows + static_cast<uint64_t>(src) * NW;
+        uint64_t t[NW];
+        load_row<NW>(rows, src, t);
         uint64_t *pd = rows + dest * NW;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) pd[w] = ps[w] ^ gd.g[w];
+        for (int w = 0; w < NW; ++w) {
+            t[w] ^= gd.g[w];
+            pd[w] = t[w];
+        }
         coef[dest] = ins[i];
-        if (zero_fill != nullptr) zero_fill[dest] = 0.0;
+        if (zero_fill != nullptr) zero_fill[dest] = occ_dev ? fresh_score<NW, PAULI>(t, occ_dev) : 0.0;
     }
     if (grec.cos != nullptr) {
         const uint64_t p = wave_append(v != 0, counters + C_GROT);
@@ -1106,7 +1138,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
 // the other gate parity's group counts.
 // With BATCH, an emitter's member (member[S]) selects its rotation constants (gtrig[member]) and keys its
 // index lookup; the partner it creates joins the same member.
-template <int NW, int PAULI, int BATCH>
+template <int NW, int PAULI, int BATCH, int SCHR>
 __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
@@ -1119,7 +1151,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
                                                     uint64_t cap_rows, uint64_t gidx_slots, uint32_t max_groups,
                                                     int group_log2, uint16_t *__restrict__ member,
-                                                    const Trig *__restrict__ gtrig) {
+                                                    const Trig *__restrict__ gtrig,
+                                                    const uint64_t *__restrict__ occ_dev) {
     constexpr int NC = max_cols<NW, PAULI>();
     constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
     __shared__ uint64_t lcol[kColWords * NC];
@@ -1281,9 +1314,10 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                         coef[S] = wS + gs * static_cast<double>(-ph) * vTu;
                         coef[T] = wT + gs * static_cast<double>(ph) * vS;
                         hit = 1;
-                    } else {
-                        coef[S] = wS + gs * static_cast<double>(-ph) * 0.0;
-                        ins = gk * 0.0 + gs * static_cast<double>(ph) * vS;
+                    } else {  // the new partner starts at vT0: 0, or its state score (SCHR)
+                        const double vT0 = SCHR ? fresh_score<NW, PAULI>(t, occ_dev) : 0.0;
+                        coef[S] = wS + gs * static_cast<double>(-ph) * vT0;
+                        ins = gk * vT0 + gs * static_cast<double>(ph) * vS;
                         miss = 1;
                     }
                 }
@@ -1469,6 +1503,54 @@ __global__ __launch_bounds__(kBlock) void k_init_lambda(const uint64_t *__restri
     for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
     load_row<NW>(rows, static_cast<uint32_t>(i), m);
     lam[i] = mpa::is_paired<NW, PAULI>(m) ? mpa::state_phase<NW, PAULI>(m, occ) : 0.0;
+}
+
+// Rows that hold the observable's terms (Schrodinger picture, MPOperator.h:119-141).  `tab` is an
+// open-addressing table of the terms, NW key words then the term's index + 1 (0: empty slot), at load
+// <= 1/2.  A row holding term j writes its index and value c[row] at hit_row[j] / hit_val[j] and, when
+// `scatter` is set, the term's coefficient obs_c[j] at scatter[row].
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_obs_match(const uint64_t *__restrict__ rows, const double *__restrict__ c,
+                                                      uint64_t N, const uint64_t *__restrict__ tab, uint64_t tmask,
+                                                      int64_t *__restrict__ hit_row, double *__restrict__ hit_val,
+                                                      const double *__restrict__ obs_c, double *__restrict__ scatter) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= N) return;
+    uint64_t m[NW];
+    load_row<NW>(rows, static_cast<uint32_t>(i), m);
+    uint64_t h = mpa::hash_key<NW>(m) & tmask;
+    for (uint64_t probe = 0; probe <= tmask; ++probe) {
+        const uint64_t *slot = tab + h * (NW + 1);
+        const uint64_t id = slot[NW];
+        if (!id) return;
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) eq = eq && (slot[w] == m[w]);
+        if (eq) {
+            hit_row[id - 1] = static_cast<int64_t>(i);
+            hit_val[id - 1] = c[i];
+            if (scatter) scatter[i] = obs_c[id - 1];
+            return;
+        }
+        h = (h + 1) & tmask;
+    }
+}
+
+// Fixed-order block partials of sum_i a[i] b[i] (the reference's state . op, MPFunctions.cpp:86-94).
+__global__ __launch_bounds__(kBlock) void k_dot(const double *__restrict__ a, const double *__restrict__ b, uint64_t N,
+                                                double *__restrict__ partial) {
+    __shared__ double red[kBlock];
+    double acc = 0.0;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock)
+        acc += a[i] * b[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(s)) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 // Per-layer sum of the adjoint partials, in block order: one block per layer.
@@ -1797,6 +1879,15 @@ struct mpa_engine {
     size_t fused_done = 0;   // gates the last fused propagate applied
     DevArray bmember{&stream}, btrig{&stream}, bpartial{&stream}, bout{&stream};
 
+    // Schrodinger picture (schrodinger_cutoff set): the store holds the evolved state; the observable's
+    // terms (core excluded) are matched against the rows when <O> is evaluated (obs_tab: their
+    // open-addressing table, built on first use).
+    bool schr = false;
+    std::vector<uint64_t> obs_rows;
+    std::vector<double> obs_coef;
+    uint64_t obs_tmask = 0;
+    DevArray obs_tab{&stream}, obs_c{&stream}, obs_hit{&stream}, obs_val{&stream};
+
     std::vector<double> init_coef;  // initial coefficients of rows [0, n_init)
     uint64_t n_init = 0;
     std::vector<int64_t> stat_n, stat_a, stat_e, stat_i, stat_l, stat_p;
@@ -2099,6 +2190,7 @@ struct mpa_engine {
             ensure_capacity(n_ub);
             ensure_items(nL, nA);
             double *cv = graph ? seedc.as<double>() : coef.as<double>();
+            const uint64_t *occ_s = schr ? occ_dev.as<uint64_t>() : nullptr;  // Schrodinger: fresh rows scored
             GraphRec grec{nullptr, nullptr};
             if (graph && nA > 0) {
                 gcos_ub += nA;
@@ -2130,14 +2222,15 @@ struct mpa_engine {
                                        AL.as<uint32_t>(), static_cast<uint32_t>(nL), static_cast<uint32_t>(nF),
                                        table.as<uint64_t>(), tmask, epoch, rec.as<uint8_t>(), lv.as<double>(),
                                        match.as<uint8_t>(), miss.as<uint8_t>(), ins.as<double>(),
-                                       counters.as<uint64_t>(), grec);
+                                       counters.as<uint64_t>(), grec, occ_s);
                     HIP_TRY(hipGetLastError());
                     prof_end(K_FOLLOWERS, pa);
                 }
                 if (lb) {
                     pa = prof_begin();
-                    hipLaunchKernelGGL(k_leaders_unmatched, dim3(lb), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                       igi, cv, AL.as<uint32_t>(), static_cast<uint32_t>(nL),
+                    hipLaunchKernelGGL((k_leaders_unmatched<NW, PAULI>), dim3(lb), dim3(kBlock), 0, stream,
+                                       gdesc.as<GateDesc>(), igi, rows.as<uint64_t>(), occ_s, cv, AL.as<uint32_t>(),
+                                       static_cast<uint32_t>(nL),
                                        rec.as<uint8_t>(), lv.as<double>(), match.as<uint8_t>(), miss.as<uint8_t>(),
                                        ins.as<double>(), counters.as<uint64_t>(), grec);
                     HIP_TRY(hipGetLastError());
@@ -2154,10 +2247,10 @@ struct mpa_engine {
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SCANSUM, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_insert<NW>), dim3(ab), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), igi,
+                hipLaunchKernelGGL((k_insert<NW, PAULI>), dim3(ab), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(), igi,
                                    rows.as<uint64_t>(), cv, graph ? coef.as<double>() : static_cast<double *>(nullptr),
                                    AL.as<uint32_t>(), static_cast<uint32_t>(nA), miss.as<uint8_t>(), ins.as<double>(),
-                                   blk_a.as<uint32_t>(), counters.as<uint64_t>(), grec);
+                                   blk_a.as<uint32_t>(), counters.as<uint64_t>(), grec, occ_s);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_INSERT, pa);
                 pa = prof_begin();
@@ -2208,18 +2301,33 @@ struct mpa_engine {
 
     // Replay the stored graph at `params` into `rep` (evolve_operator, Evolution.cpp:529-540); with
     // `snapshots` the pre-layer values of every record are kept for the adjoint sweep.
-    void replay(const double *params, bool snapshots) {
+    //   kOperator   (Heisenberg): the operator from the coefficients the graph started at, layers in
+    //               processing order, +alpha.
+    //   kState      (Schrodinger contract_partially, MonomialPropagator.inl:1069-1083): the state,
+    //               oldest layer first, -alpha.
+    //   kObservable (Schrodinger functionals, MonomialPropagator.inl:918-1000): the observable's
+    //               coefficients, which the caller has put in `rep`, newest layer first (the order the
+    //               reference keeps Schrodinger layers in, MPGraph.h:41-43), +alpha.
+    enum ReplayKind { kOperator, kState, kObservable };
+    void replay(const double *params, bool snapshots, ReplayKind kind = kOperator) {
         rep.reserve(std::max<uint64_t>(N, 1) * 8);
-        if (base_n) HIP_TRY(hipMemcpyAsync(rep.p, base.p, base_n * 8, hipMemcpyDeviceToDevice, stream));
-        if (N > base_n) HIP_TRY(hipMemsetAsync(rep.as<double>() + base_n, 0, (N - base_n) * 8, stream));
+        if (kind == kOperator) {
+            if (base_n) HIP_TRY(hipMemcpyAsync(rep.p, base.p, base_n * 8, hipMemcpyDeviceToDevice, stream));
+            if (N > base_n) HIP_TRY(hipMemsetAsync(rep.as<double>() + base_n, 0, (N - base_n) * 8, stream));
+        } else if (kind == kState && N) {
+            HIP_TRY(hipMemcpyAsync(rep.p, coef.p, N * 8, hipMemcpyDeviceToDevice, stream));
+        }
         if (snapshots) {
             snap_c.reserve(std::max<uint64_t>(gcos_total, 1) * 8);
             snap_r.reserve(std::max<uint64_t>(grot_total, 1) * 16);
         }
-        for (const LayerInfo &ly : layers) {
+        const double phase = kind == kState ? -1.0 : 1.0;
+        const size_t nl = layers.size();
+        for (size_t i = 0; i < nl; ++i) {
+            const LayerInfo &ly = layers[kind == kObservable ? nl - 1 - i : i];
             const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
             if (nc + nr == 0) continue;
-            const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
+            const double alpha = phase * params[ly.param_index] * ly.gen_coeff;
             const uint32_t blocks = static_cast<uint32_t>((nc + nr + kBlock - 1) / kBlock);
             hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, stream, g_cos.as<uint32_t>(),
                                g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr, std::cos(2 * alpha),
@@ -2229,11 +2337,15 @@ struct mpa_engine {
         }
     }
 
-    // d<O>/d params: adjoint sweep over the layers in reverse (after replay(params, true)).
+    // d<O>/d params: adjoint sweep over the layers in the reverse of the replay's order (after
+    // replay(params, true, kOperator) in the Heisenberg picture, kObservable in the Schrodinger
+    // picture, whose adjoint starts from the state instead of the diagonal phases).
     template <int NW, int PAULI>
     void gradient(const double *params, int64_t n_params, double *grad) {
         lam.reserve(std::max<uint64_t>(N, 1) * 8);
-        if (N) {
+        if (N && schr) {
+            HIP_TRY(hipMemcpyAsync(lam.p, coef.p, N * 8, hipMemcpyDeviceToDevice, stream));
+        } else if (N) {
             hipLaunchKernelGGL((k_init_lambda<NW, PAULI>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, stream, rows.as<uint64_t>(), N, occ_dev.as<uint64_t>(),
                                lam.as<double>());
@@ -2248,7 +2360,8 @@ struct mpa_engine {
         gpartial.reserve(std::max<uint64_t>(offs[nl], 1) * 8);
         goffs.reserve((nl + 1) * 8);
         gdl.reserve(std::max<size_t>(nl, 1) * 8);
-        for (size_t l = nl; l-- > 0;) {
+        for (size_t i = 0; i < nl; ++i) {
+            const size_t l = schr ? i : nl - 1 - i;
             const LayerInfo &ly = layers[l];
             const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
             if (nc + nr == 0) continue;
@@ -2270,6 +2383,85 @@ struct mpa_engine {
         HIP_TRY(hipStreamSynchronize(stream));
         for (int64_t p = 0; p < n_params; ++p) grad[p] = 0.0;
         for (size_t l = nl; l-- > 0;) grad[layers[l].param_index] += layers[l].gen_coeff * dl[l];
+    }
+
+    // Open-addressing table of the observable's terms (Schrodinger picture), uploaded once.
+    template <int NW>
+    void ensure_obs_table() {
+        if (obs_tmask) return;
+        const size_t n = obs_coef.size();
+        uint64_t slots = 16;
+        while (slots < 2 * std::max<size_t>(n, 1)) slots <<= 1;
+        std::vector<uint64_t> tab(slots * (NW + 1), 0);
+        for (size_t j = 0; j < n; ++j) {
+            const uint64_t *m = obs_rows.data() + j * NW;
+            uint64_t h = mpa::hash_key<NW>(m) & (slots - 1);
+            while (tab[h * (NW + 1) + NW]) h = (h + 1) & (slots - 1);
+            std::copy(m, m + NW, tab.begin() + static_cast<ptrdiff_t>(h * (NW + 1)));
+            tab[h * (NW + 1) + NW] = j + 1;
+        }
+        obs_tab.reserve(tab.size() * 8);
+        obs_c.reserve(std::max<size_t>(n, 1) * 8);
+        obs_hit.reserve(std::max<size_t>(n, 1) * 8);
+        obs_val.reserve(std::max<size_t>(n, 1) * 8);
+        HIP_TRY(hipMemcpyAsync(obs_tab.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, stream));
+        if (n) HIP_TRY(hipMemcpyAsync(obs_c.p, obs_coef.data(), n * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        obs_tmask = slots - 1;
+    }
+
+    // Rows holding the observable's terms and the values `c` has there (hit_row -1: no row holds the
+    // term); with `scatter`, the observable's coefficient vector over the rows is written there.
+    template <int NW>
+    void match_observable(const double *c, std::vector<int64_t> &hit_row, std::vector<double> &hit_val,
+                          double *scatter) {
+        ensure_obs_table<NW>();
+        const size_t n = obs_coef.size();
+        if (scatter && N) HIP_TRY(hipMemsetAsync(scatter, 0, N * 8, stream));
+        if (n) HIP_TRY(hipMemsetAsync(obs_hit.p, 0xff, n * 8, stream));
+        if (n && N) {
+            hipLaunchKernelGGL((k_obs_match<NW>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)), dim3(kBlock),
+                               0, stream, rows.as<uint64_t>(), c, N, obs_tab.as<uint64_t>(), obs_tmask,
+                               obs_hit.as<int64_t>(), obs_val.as<double>(), obs_c.as<double>(), scatter);
+            HIP_TRY(hipGetLastError());
+        }
+        hit_row.assign(n, -1);
+        hit_val.assign(n, 0.0);
+        if (n) {
+            HIP_TRY(hipMemcpyAsync(hit_row.data(), obs_hit.p, n * 8, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(hit_val.data(), obs_val.p, n * 8, hipMemcpyDeviceToHost, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+
+    // Schrodinger <O> with no graph: core + state . op over the rows in ascending order
+    // (MPFunctions.cpp:86-94, 196-205); op is zero away from the observable's rows, so the sum runs
+    // over those rows only and is the reference's to the bit.
+    template <int NW>
+    double observable_expect(const double *c) {
+        std::vector<int64_t> hr;
+        std::vector<double> hv;
+        match_observable<NW>(c, hr, hv, nullptr);
+        std::vector<std::pair<int64_t, size_t>> at;
+        for (size_t j = 0; j < hr.size(); ++j)
+            if (hr[j] >= 0) at.emplace_back(hr[j], j);
+        std::sort(at.begin(), at.end());
+        double r = 0.0;
+        for (const auto &x : at) r += hv[x.second] * obs_coef[x.second];
+        return core + r;
+    }
+
+    // core + sum_i a[i] b[i] in fixed-order block partials.
+    double dot_core(const double *a, const double *b) {
+        partial.reserve(kExpectBlocks * 8);
+        hipLaunchKernelGGL(k_dot, dim3(kExpectBlocks), dim3(kBlock), 0, stream, a, b, N, partial.as<double>());
+        HIP_TRY(hipGetLastError());
+        std::vector<double> h(kExpectBlocks);
+        HIP_TRY(hipMemcpyAsync(h.data(), partial.p, kExpectBlocks * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        double r = 0.0;
+        for (double x : h) r += x;
+        return core + r;
     }
 
     // Fresh descriptor memory reads as epoch 0, which no gate uses.
@@ -2429,13 +2621,15 @@ struct mpa_engine {
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_resolve<NW, PAULI, BATCH>), dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
+                auto *kres = (!BATCH && schr) ? k_resolve<NW, PAULI, 0, 1> : k_resolve<NW, PAULI, BATCH, 0>;
+                hipLaunchKernelGGL(kres, dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
                                    gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
-                                   static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig);
+                                   static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig,
+                                   schr ? occ_dev.as<uint64_t>() : static_cast<uint64_t *>(nullptr));
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
             }
@@ -2706,6 +2900,33 @@ struct RunExpect {
     }
 };
 
+// Schrodinger <O>: without a graph the state against the observable's rows; with one, the observable
+// replayed newest layer first and dotted with the state (the reference's functional).
+struct RunSchrExpect {
+    mpa_engine *e;
+    const double *params;
+    int64_t n_params;
+    bool snapshots;
+    double *out;
+    template <int NW, int PAULI>
+    void operator()() {
+        if (e->layers.empty()) {
+            if (n_params != 0) throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
+            *out = e->observable_expect<NW>(e->coef.as<double>());
+            return;
+        }
+        if (n_params != e->graph_nparams)
+            throw ArgError("Expected " + std::to_string(e->graph_nparams) + " parameter value(s), got " +
+                           std::to_string(n_params) + ".");
+        e->rep.reserve(std::max<uint64_t>(e->N, 1) * 8);
+        std::vector<int64_t> hr;
+        std::vector<double> hv;
+        e->match_observable<NW>(e->coef.as<double>(), hr, hv, e->rep.as<double>());
+        e->replay(params, snapshots, mpa_engine::kObservable);
+        *out = e->dot_core(e->coef.as<double>(), e->rep.as<double>());
+    }
+};
+
 struct RunGradient {
     mpa_engine *e;
     const double *params;
@@ -2718,7 +2939,7 @@ struct RunGradient {
 };
 
 // Layer arrays (authoring order) -> per-gate descriptors in processing order (Heisenberg: back to
-// front, MonomialPropagator.inl:720-733).  alpha = (1.0 * theta[map]) * g (MPFunctions.cpp:57-69);
+// front; Schrodinger: front to back, MonomialPropagator.inl:720-733).  alpha = (1.0 * theta[map]) * g (MPFunctions.cpp:57-69);
 // without params (structural graph build) alpha = 0.  `fused`: the contract-immediately cos sweep.
 void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
                 const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
@@ -2728,7 +2949,7 @@ void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *
     gds.reserve(n_layers);
     std::vector<uint64_t> g(mpa::kMaxWords);
     for (int64_t i = 0; i < n_layers; ++i) {
-        const int64_t idx = n_layers - 1 - i;
+        const int64_t idx = e->schr ? i : n_layers - 1 - i;
         if (param_map[idx] < 0 || (params != nullptr && param_map[idx] >= n_params))
             throw ArgError("parameter index out of range for the given parameters");
         GateDesc gd{};
@@ -2738,9 +2959,12 @@ void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *
         gd.param_index = param_map[idx];
         gd.gen_coeff = gen_coeffs[idx];
         if (params != nullptr) {
+            // build angle alpha (cutoffs, cos sweep); the rotation applies it negated in the
+            // Schrodinger picture (MonomialPropagator.h:411-416)
             const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
-            gd.k = std::cos(2 * alpha);
-            gd.s = std::sin(2 * alpha);
+            const double apply = e->schr ? -alpha : alpha;
+            gd.k = std::cos(2 * apply);
+            gd.s = std::sin(2 * apply);
             gd.abs_s = std::abs(std::sin(2 * alpha));
             const double cos_build = std::cos(2.0 * alpha);
             gd.fused = (fused_allowed && cos_build != 0.0) ? 1 : 0;
@@ -2760,8 +2984,9 @@ void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *
     }
 }
 
-// The coefficient vector the current picture evaluates: the replay at params when a graph is stored
-// (n_params must cover it), else the contracted operator (n_params must be 0).
+// The coefficient vector of the current picture at params: the replay when a graph is stored (n_params
+// must cover it; the Schrodinger picture replays the state), else the contracted coefficients
+// (n_params must be 0).
 const double *evaluated_coeffs(mpa_engine *e, const double *params, int64_t n_params, bool snapshots) {
     if (e->layers.empty()) {
         if (n_params != 0) throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
@@ -2770,7 +2995,7 @@ const double *evaluated_coeffs(mpa_engine *e, const double *params, int64_t n_pa
     if (n_params != e->graph_nparams)
         throw ArgError("Expected " + std::to_string(e->graph_nparams) + " parameter value(s), got " +
                        std::to_string(n_params) + ".");
-    e->replay(params, snapshots);
+    e->replay(params, snapshots, e->schr ? mpa_engine::kState : mpa_engine::kOperator);
     return e->rep.as<double>();
 }
 
@@ -2902,6 +3127,55 @@ void encode_operator(const mpa_engine *e, const int64_t *term_offsets, const int
     }
 }
 
+// The Schrodinger picture's initial rows (MonomialPropagator.inl:162-181, MajoranaAlgebra.h:125-160):
+// every fully paired monomial of at most `pairs` modes, by pair count; within a count the mode sets in
+// the order std::prev_permutation walks a selector that starts as `pairs` ones then zeros (the k-subsets
+// of selector positions in lexicographic order), selector position i being mode n-1-i.  A row starts at
+// its diagonal phase <b|M|b> (MPOperator.h:235-254).
+void paired_state_rows(const mpa_engine *e, int pairs, std::vector<uint64_t> &rows, std::vector<double> &coefs) {
+    const int n = e->n_modes;
+    double count = 0.0, binom = 1.0;
+    for (int k = 0; k <= pairs; ++k) {
+        count += binom;
+        binom = binom * (n - k) / (k + 1);
+    }
+    if (count > static_cast<double>(kMaxSchrodingerRows))
+        throw ArgError("schrodinger_cutoff gives " + std::to_string(static_cast<long double>(count)) +
+                       " initial state terms, more than the " + std::to_string(kMaxSchrodingerRows) +
+                       " this engine starts from");
+    rows.reserve(static_cast<size_t>(count) * e->nw);
+    coefs.reserve(static_cast<size_t>(count));
+    std::vector<int> pos;
+    std::vector<uint64_t> m(e->nw);
+    for (int k = 0; k <= pairs; ++k) {
+        pos.resize(k);
+        for (int i = 0; i < k; ++i) pos[i] = i;
+        for (;;) {
+            std::fill(m.begin(), m.end(), 0);
+            int occupied = 0;
+            for (int i : pos) {
+                const int j = n - 1 - i;
+                if (e->basis == MPA_BASIS_PAULI) {
+                    m[e->L + (j >> 6)] |= uint64_t{1} << (j & 63);  // Z_j
+                    occupied += static_cast<int>((e->occ[j >> 6] >> (j & 63)) & 1u);
+                } else {
+                    m[(2 * j) >> 6] |= uint64_t{3} << ((2 * j) & 63);  // slots 2j, 2j+1
+                    occupied += static_cast<int>((e->occ[(2 * j) >> 6] >> ((2 * j) & 63)) & 1u);
+                }
+            }
+            // Pauli: (-1)^{|Z support & occupied|}; Majorana: (-1)^{occupied + pairs} (MajoranaAlgebra.h:52-60)
+            const int odd = e->basis == MPA_BASIS_PAULI ? (occupied & 1) : ((occupied + k) & 1);
+            rows.insert(rows.end(), m.begin(), m.end());
+            coefs.push_back(odd ? -1.0 : 1.0);
+            int i = k - 1;
+            while (i >= 0 && pos[i] == n - k + i) --i;
+            if (i < 0) break;
+            ++pos[i];
+            for (int t = i + 1; t < k; ++t) pos[t] = pos[t - 1] + 1;
+        }
+    }
+}
+
 // Uploads the initial rows (and, for a batch, their members) and indexes their columns.
 void load_initial(mpa_engine *e, uint64_t n0, const uint64_t *hrows, const double *hcoef, const uint16_t *members) {
     e->counters.reserve(C_COUNT * 8);
@@ -2954,6 +3228,35 @@ int mpa_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int 
         load_initial(e.get(), hcoef.size(), hrows.data(), hcoef.data(), nullptr);
         *out = e.release();
     });
+}
+
+int mpa_create_schrodinger(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type,
+                           double lower_atol, double upper_atol, int64_t schrodinger_cutoff,
+                           const int64_t *term_offsets, const int32_t *term_slots, const double *coeff_re,
+                           const double *coeff_im, int64_t n_terms, const int32_t *initial_state, int64_t n_state,
+                           int device) {
+    return guarded([&] {
+        if (!out) throw ArgError("out is null");
+        *out = nullptr;
+        if (schrodinger_cutoff < 0) throw ArgError("schrodinger_cutoff must be non-negative");
+        auto e = new_engine(basis, num_modes, cutoff, cutoff_type, lower_atol, upper_atol, initial_state, n_state, device);
+        e->schr = true;
+        encode_operator(e.get(), term_offsets, term_slots, coeff_re, coeff_im, 0, n_terms, e->obs_rows, e->obs_coef,
+                        &e->core);
+        // ceil(sc / 2) pairs of the clamped slot cutoff, at most one per mode (MonomialPropagator.inl:162-164)
+        const int64_t sc = std::min<int64_t>(schrodinger_cutoff, 2 * static_cast<int64_t>(num_modes));
+        const int pairs = static_cast<int>(std::min<int64_t>(sc / 2 + sc % 2, num_modes));
+        std::vector<uint64_t> hrows;
+        std::vector<double> hcoef;
+        paired_state_rows(e.get(), pairs, hrows, hcoef);
+        load_initial(e.get(), hcoef.size(), hrows.data(), hcoef.data(), nullptr);
+        *out = e.release();
+    });
+}
+
+int mpa_is_schrodinger(mpa_engine *e, int32_t *out) {
+    *out = e->schr ? 1 : 0;
+    return 0;
 }
 
 int mpa_batch_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff, int cutoff_type, double lower_atol,
@@ -3240,7 +3543,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
             if (n_params < e->graph_nparams)
                 throw ArgError("Coefficient-informed build_graph() needs at least " +
                                std::to_string(e->graph_nparams) + " parameter value(s) to replay the existing graph.");
-            e->replay(params, false);
+            e->replay(params, false, e->schr ? mpa_engine::kState : mpa_engine::kOperator);
             if (e->N) HIP_TRY(hipMemcpyAsync(e->seedc.p, e->rep.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
         } else if (e->N) {
             HIP_TRY(hipMemcpyAsync(e->seedc.p, e->coef.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
@@ -3268,6 +3571,11 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
 int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out) {
     return guarded([&] {
         require_single(e);
+        if (e->schr) {
+            RunSchrExpect ev{e, params, n_params, false, out};
+            e->dispatch(ev);
+            return;
+        }
         const double *c = evaluated_coeffs(e, params, n_params, false);
         RunExpect ev{e, c, out};
         e->dispatch(ev);
@@ -3278,9 +3586,14 @@ int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int6
                                        double *grad) {
     return guarded([&] {
         require_single(e);
-        const double *c = evaluated_coeffs(e, params, n_params, true);
-        RunExpect ev{e, c, value};
-        e->dispatch(ev);
+        if (e->schr) {
+            RunSchrExpect ev{e, params, n_params, true, value};
+            e->dispatch(ev);
+        } else {
+            const double *c = evaluated_coeffs(e, params, n_params, true);
+            RunExpect ev{e, c, value};
+            e->dispatch(ev);
+        }
         if (!e->layers.empty()) {
             RunGradient rg{e, params, n_params, grad};
             e->dispatch(rg);
@@ -3366,8 +3679,8 @@ int mpa_export_terms(mpa_engine *e, const double *params, int64_t n_params, doub
             ns += static_cast<int64_t>(tmp.size());
         }
         if (slots) offsets[nt] = ns;
-        // core term (binder.h:213-215)
-        if (std::abs(e->core) >= atol) {
+        // core term, Heisenberg picture only (binder.h:212-215)
+        if (!e->schr && std::abs(e->core) >= atol) {
             if (slots) {
                 offsets[nt + 1] = ns;
                 re[nt] = e->core;

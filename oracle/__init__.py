@@ -37,6 +37,11 @@ def _lib():
         lib.orc_create.restype = P
         lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_double, C.c_double,
                                    i64p, i32p, f64p, f64p, C.c_int64, i32p, C.c_int64]
+        lib.orc_create_schrodinger.restype = P
+        lib.orc_create_schrodinger.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_double, C.c_double,
+                                               C.c_int64, i64p, i32p, f64p, f64p, C.c_int64, i32p, C.c_int64]
+        lib.orc_is_schrodinger.restype = C.c_int
+        lib.orc_is_schrodinger.argtypes = [P]
         lib.orc_destroy.argtypes = [P]
         lib.orc_size.restype = C.c_int64
         lib.orc_size.argtypes = [P]
@@ -89,7 +94,8 @@ class Oracle:
     """Serial CPU engine with the reference's propagate / graph / expectation semantics.
 
     ``basis`` is ``"pauli"`` or ``"majorana"``; ``initial_operator`` maps engine slot tuples to complex
-    coefficients exactly as the reference's compiled core receives them.
+    coefficients exactly as the reference's compiled core receives them.  ``schrodinger_cutoff`` (in
+    slots, as the compiled core takes it) selects the Schrodinger picture.
     """
 
     def __init__(
@@ -103,6 +109,7 @@ class Oracle:
         lower_atol: float | None = None,
         upper_atol: float | None = None,
         initial_state: Sequence[int] = (),
+        schrodinger_cutoff: int | None = None,
     ) -> None:
         lib = _lib()
         keys = list(initial_operator.keys())
@@ -113,12 +120,14 @@ class Oracle:
         st = np.asarray(list(initial_state) or [0], dtype=np.int32)
         self.num_modes = num_modes
         self.basis = basis
-        self._h = lib.orc_create(
-            1 if basis == "pauli" else 0, num_modes, cutoff, 0 if cutoff_type == "length" else 1,
-            _none_nan(lower_atol), _none_nan(upper_atol),
-            _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), _ptr(re, C.c_double), _ptr(im, C.c_double),
-            len(keys), _ptr(st, C.c_int32), len(list(initial_state)),
-        )
+        head = (1 if basis == "pauli" else 0, num_modes, cutoff, 0 if cutoff_type == "length" else 1,
+                _none_nan(lower_atol), _none_nan(upper_atol))
+        tail = (_ptr(offs, C.c_int64), _ptr(slots, C.c_int32), _ptr(re, C.c_double), _ptr(im, C.c_double),
+                len(keys), _ptr(st, C.c_int32), len(list(initial_state)))
+        if schrodinger_cutoff is None:
+            self._h = lib.orc_create(*head, *tail)
+        else:
+            self._h = lib.orc_create_schrodinger(*head, int(schrodinger_cutoff), *tail)
         if not self._h:
             raise RuntimeError(lib.orc_last_error().decode())
 
@@ -211,8 +220,13 @@ class Oracle:
         _lib().orc_gate_stats(self._h, *[_ptr(a, C.c_int64) for a in arrs])
         return tuple(a[:n] for a in arrs)
 
+    @property
+    def schrodinger(self) -> bool:
+        return bool(_lib().orc_is_schrodinger(self._h))
+
     def evolved_operator(self, atol: float = 1e-12) -> dict[tuple[int, ...], complex]:
-        """Reference ``evolved_operator`` dict (slot tuples -> rounded complex), core included."""
+        """Reference ``evolved_operator`` dict (slot tuples -> rounded complex); the core term is
+        included in the Heisenberg picture only (binder.h:212-214)."""
         lib = _lib()
         nt, ns = C.c_int64(), C.c_int64()
         lib.orc_export(self._h, atol, C.byref(nt), C.byref(ns), None, None, None, None)
@@ -227,6 +241,6 @@ class Oracle:
             key = tuple(int(s) for s in slots[offs[k]:offs[k + 1]])
             out[key] = complex(re[k], im[k])
         core = self.core()
-        if abs(core) >= atol:
+        if not self.schrodinger and abs(core) >= atol:
             out[()] = complex(core, 0.0)
         return out

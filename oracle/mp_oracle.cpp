@@ -84,6 +84,11 @@ struct Engine {
     std::vector<double> coeffs;  // current-picture coefficients (MPOperator::op_coeffs)
     std::vector<double> init_coeffs;  // initial operator coefficients at their rows
     std::vector<uint64_t> occ;   // occupation mask of the initial product state
+    // Schrodinger picture (MonomialPropagator.inl:86-89): the rows hold the evolved state and the
+    // observable's terms (core excluded) are looked up among them at evaluation time.
+    bool schr = false;
+    std::vector<uint64_t> obs_rows;
+    std::vector<double> obs_coef;
     std::vector<uint32_t> table; // keyless open-addressing index over rows (OperatorIndex.h:278-367)
     uint64_t table_mask = 0;
     std::vector<Layer> graph;
@@ -294,6 +299,28 @@ struct Engine {
         return ((c + pop / 2) % 2) ? -1.0 : 1.0;
     }
 
+    // Value a row takes when it enters the Schrodinger state: its diagonal phase if fully paired, else
+    // 0 (MPOperator.h:158-181 dense_state() scoring of every row it has not seen yet).
+    double score(const uint64_t *m) const { return paired(m) ? state_phase(m) : 0.0; }
+
+    // The observable's coefficients at the rows that hold its terms (MPOperator.h:119-141 get_operator():
+    // a term joins the vector once some row holds it).
+    std::vector<double> observable_vector() const {
+        std::vector<double> op(size(), 0.0);
+        for (size_t j = 0; j < obs_coef.size(); ++j) {
+            const int64_t r = find(obs_rows.data() + j * nw);
+            if (r >= 0) op[static_cast<size_t>(r)] = obs_coef[j];
+        }
+        return op;
+    }
+
+    // Schrodinger <O> = core + state . op in row order (MPFunctions.cpp:86-94, 196-205).
+    double expectation_schr(const std::vector<double> &state, const std::vector<double> &op) const {
+        double r = 0.0;
+        for (size_t i = 0; i < state.size(); ++i) r += state[i] * op[i];
+        return core + r;
+    }
+
     // --- per-gate build ------------------------------------------------------------------------------
 
     // Emitted partner queries, flat: key words, phase, source row, pre-cos source value.
@@ -317,8 +344,11 @@ struct Engine {
     // One layer of the contract-immediately path (Engine.h:522-637 + FusedApply.h:32-83) when
     // `contract` is true, or the graph-build path (GraphSink, Engine.h:69-180) when false.
     // `coeff_view` is the coefficient vector the cutoffs read (nullptr = structural build).
-    void build_layer(const uint64_t *g, bool have_param, double alpha, int only_rot, bool contract,
-                     std::vector<double> *coeff_view, Layer *out_layer, std::vector<uint32_t> *out_cos) {
+    // `alpha` is the build angle (the cutoffs and the fused cos sweep); `apply_alpha` the angle of the
+    // rotation, the build angle negated in the Schrodinger picture (MonomialPropagator.h:411-416).
+    void build_layer(const uint64_t *g, bool have_param, double alpha, double apply_alpha, int only_rot,
+                     bool contract, std::vector<double> *coeff_view, Layer *out_layer,
+                     std::vector<uint32_t> *out_cos) {
         std::vector<double> *cp = contract ? &coeffs : coeff_view;
         const bool local_coeffs = cp != nullptr;
         const bool check_atol = has_lower && local_coeffs && have_param;
@@ -435,8 +465,12 @@ struct Engine {
         for (size_t k = 0; k < deferred.size(); ++k) {
             const Queries &qs = *deferred[k].first;
             const size_t q = deferred[k].second;
-            append_row(qs.key.data() + q * nw, 0.0);
-            inserts.push_back({qs.src[q], static_cast<uint32_t>(n0 + k), qs.v_src[q], 0.0, qs.phase[q]});
+            // A fresh row starts at 0 in the Heisenberg picture and at its state score in the
+            // Schrodinger picture, where that value is also the insert's v_tgt (FusedApply.h:38-44).
+            const uint64_t *key = qs.key.data() + q * nw;
+            const double fresh = schr ? score(key) : 0.0;
+            append_row(key, fresh);
+            inserts.push_back({qs.src[q], static_cast<uint32_t>(n0 + k), qs.v_src[q], fresh, qs.phase[q]});
         }
         init_coeffs.resize(size(), 0.0);
         stat_i.push_back(static_cast<int64_t>(deferred.size()));
@@ -444,8 +478,8 @@ struct Engine {
         if (contract) {
             // FusedApply.h:46-82
             std::vector<double> &c = *cp;
-            const double cos_val = std::cos(2 * alpha);
-            const double sin_val = std::sin(2 * alpha);
+            const double cos_val = std::cos(2 * apply_alpha);
+            const double sin_val = std::sin(2 * apply_alpha);
             if (!fused) {
                 for (uint32_t i : cos_set) c[i] *= cos_val;
                 for (size_t k = n0; k < size(); ++k) c[k] *= cos_val;
@@ -461,7 +495,10 @@ struct Engine {
                     c[r.tgt] += sin_val * static_cast<double>(r.phase) * r.v_src;
             }
         } else {
-            if (coeff_view) coeff_view->resize(size(), 0.0);
+            // the seed grows by the current picture's values of the new rows
+            // (MonomialPropagator.inl:523-538 extend_coeffs_from_current_picture_if_needed_)
+            if (coeff_view)
+                for (size_t k = coeff_view->size(); k < size(); ++k) coeff_view->push_back(coeffs[k]);
             Layer &ly = *out_layer;
             ly.gen.assign(g, g + nw);
             ly.scaled_count = size();
@@ -564,9 +601,39 @@ const char *orc_last_error() { return g_last_error.c_str(); }
 // Create: initial operator in reference slot-tuple CSR form, complex coefficients.  Terms are taken in
 // lexicographic slot-tuple order, the iteration order of the reference's std::map<VecZ, complex>
 // (binder.h:60-100, MonomialPropagator.inl:147-182).
-void *orc_create(int basis, int n_modes, unsigned cutoff, int cutoff_type, double lower_atol, double upper_atol,
-                 const int64_t *offsets, const int32_t *slots, const double *re, const double *im, int64_t n_terms,
-                 const int32_t *state, int64_t n_state) {
+// The Schrodinger picture's initial rows (MonomialPropagator.inl:162-181, MajoranaAlgebra.h:125-160):
+// every fully paired monomial of at most max_pairs modes, by pair count, each count's mode sets in the
+// order std::prev_permutation walks a selector that starts as max_pairs ones then zeros; selector
+// position i is mode n-1-i.  Each row starts at its diagonal phase.
+void add_paired_rows(Engine &e, int max_pairs) {
+    const int n = e.n_modes;
+    std::vector<int> pos;
+    std::vector<uint64_t> m(e.nw);
+    for (int k = 0; k <= max_pairs; ++k) {
+        // the selector's set positions run through the k-subsets of [0, n) in lexicographic order
+        pos.resize(k);
+        for (int i = 0; i < k; ++i) pos[i] = i;
+        for (;;) {
+            std::fill(m.begin(), m.end(), 0);
+            for (int i : pos) {
+                const int j = n - 1 - i;
+                if (e.basis == 1) m[e.L + (j >> 6)] |= uint64_t{1} << (j & 63);  // Z_j
+                else m[(2 * j) >> 6] |= uint64_t{3} << ((2 * j) & 63);           // slots 2j, 2j+1
+            }
+            e.append_row(m.data(), e.score(m.data()));
+            int i = k - 1;
+            while (i >= 0 && pos[i] == n - k + i) --i;
+            if (i < 0) break;
+            ++pos[i];
+            for (int t = i + 1; t < k; ++t) pos[t] = pos[t - 1] + 1;
+        }
+    }
+}
+
+// schr_cutoff < 0: Heisenberg picture; otherwise the Schrodinger picture with that cutoff in slots.
+void *create_impl(int basis, int n_modes, unsigned cutoff, int cutoff_type, double lower_atol, double upper_atol,
+                  int64_t schr_cutoff, const int64_t *offsets, const int32_t *slots, const double *re,
+                  const double *im, int64_t n_terms, const int32_t *state, int64_t n_state) {
     try {
         if (n_modes <= 0 || n_modes > 256) {
             fail("num_modes out of range");
@@ -633,7 +700,17 @@ void *orc_create(int basis, int n_modes, unsigned cutoff, int cutoff_type, doubl
                 e->core = enc;
                 continue;
             }
+            if (schr_cutoff >= 0) {
+                e->obs_rows.insert(e->obs_rows.end(), m.begin(), m.end());
+                e->obs_coef.push_back(enc);
+                continue;
+            }
             e->append_row(m.data(), enc);
+        }
+        if (schr_cutoff >= 0) {
+            e->schr = true;
+            const int64_t sc = std::min<int64_t>(schr_cutoff, 2 * static_cast<int64_t>(n_modes));
+            add_paired_rows(*e, static_cast<int>(std::min<int64_t>(sc / 2 + sc % 2, n_modes)));
         }
         e->init_coeffs = e->coeffs;
         return e;
@@ -643,13 +720,37 @@ void *orc_create(int basis, int n_modes, unsigned cutoff, int cutoff_type, doubl
     }
 }
 
+void *orc_create(int basis, int n_modes, unsigned cutoff, int cutoff_type, double lower_atol, double upper_atol,
+                 const int64_t *offsets, const int32_t *slots, const double *re, const double *im, int64_t n_terms,
+                 const int32_t *state, int64_t n_state) {
+    return create_impl(basis, n_modes, cutoff, cutoff_type, lower_atol, upper_atol, -1, offsets, slots, re, im,
+                       n_terms, state, n_state);
+}
+
+// Schrodinger picture (MonomialPropagator.inl:73-190 with schrodinger_cutoff set): the initial operator
+// is the observable, the rows start as the paired state terms of at most ceil(schr_cutoff / 2) modes.
+void *orc_create_schrodinger(int basis, int n_modes, unsigned cutoff, int cutoff_type, double lower_atol,
+                             double upper_atol, int64_t schr_cutoff, const int64_t *offsets, const int32_t *slots,
+                             const double *re, const double *im, int64_t n_terms, const int32_t *state,
+                             int64_t n_state) {
+    if (schr_cutoff < 0) {
+        fail("schrodinger_cutoff must be non-negative");
+        return nullptr;
+    }
+    return create_impl(basis, n_modes, cutoff, cutoff_type, lower_atol, upper_atol, schr_cutoff, offsets, slots, re,
+                       im, n_terms, state, n_state);
+}
+
+int orc_is_schrodinger(void *h) { return static_cast<Engine *>(h)->schr ? 1 : 0; }
+
 void orc_destroy(void *h) { delete static_cast<Engine *>(h); }
 
 int64_t orc_size(void *h) { return static_cast<int64_t>(static_cast<Engine *>(h)->size()); }
 int orc_words(void *h) { return static_cast<Engine *>(h)->nw; }
 
-// Layers in authoring order; processed back to front (Heisenberg, MonomialPropagator.inl:720-733),
-// alpha = (1.0 * theta[map[i]]) * g[i] (MPFunctions.cpp:57-69).
+// Layers in authoring order; processed back to front in the Heisenberg picture, front to back in the
+// Schrodinger picture (MonomialPropagator.inl:720-733), alpha = (1.0 * theta[map[i]]) * g[i]
+// (MPFunctions.cpp:57-69), applied negated in the Schrodinger picture (MonomialPropagator.h:411-416).
 int orc_propagate(void *h, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
                   const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
                   int only_rot) {
@@ -661,12 +762,12 @@ int orc_propagate(void *h, const int64_t *gen_offsets, const int32_t *gen_slots,
     e->stat_i.clear();
     std::vector<uint64_t> g(e->nw);
     for (int64_t i = 0; i < n_layers; ++i) {
-        const int64_t idx = n_layers - 1 - i;
+        const int64_t idx = e->schr ? i : n_layers - 1 - i;
         if (param_map[idx] < 0 || param_map[idx] >= n_params) return fail("parameter index out of range");
         if (!slots_to_words(*e, gen_slots + gen_offsets[idx], gen_offsets[idx + 1] - gen_offsets[idx], g.data()))
             return fail("generator index out of range");
         const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
-        e->build_layer(g.data(), true, alpha, only_rot, true, nullptr, nullptr, nullptr);
+        e->build_layer(g.data(), true, alpha, e->schr ? -alpha : alpha, only_rot, true, nullptr, nullptr, nullptr);
     }
     return 0;
 }
@@ -679,8 +780,10 @@ int orc_build_graph(void *h, const int64_t *gen_offsets, const int32_t *gen_slot
     if (params) seed = e->coeffs;  // current picture (no stored graph yet in our use)
     std::vector<uint64_t> g(e->nw);
     std::vector<uint32_t> cos;
+    // Layers are kept in processing order (the reference keeps Schrodinger layers newest first,
+    // MPGraph.h:41-43; the evaluations below walk them accordingly).
     for (int64_t i = 0; i < n_layers; ++i) {
-        const int64_t idx = n_layers - 1 - i;
+        const int64_t idx = e->schr ? i : n_layers - 1 - i;
         if (!slots_to_words(*e, gen_slots + gen_offsets[idx], gen_offsets[idx + 1] - gen_offsets[idx], g.data()))
             return fail("generator index out of range");
         Layer ly;
@@ -689,10 +792,11 @@ int orc_build_graph(void *h, const int64_t *gen_offsets, const int32_t *gen_slot
         if (params) {
             if (param_map[idx] < 0 || param_map[idx] >= n_params) return fail("parameter index out of range");
             const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
-            e->build_layer(g.data(), true, alpha, -1, false, &seed, &ly, &cos);
-            e->evolve_step(seed, ly, cos, alpha);
+            const double apply = e->schr ? -alpha : alpha;
+            e->build_layer(g.data(), true, alpha, apply, -1, false, &seed, &ly, &cos);
+            e->evolve_step(seed, ly, cos, apply);
         } else {
-            e->build_layer(g.data(), false, 0.0, -1, false, nullptr, &ly, &cos);
+            e->build_layer(g.data(), false, 0.0, 0.0, -1, false, nullptr, &ly, &cos);
         }
         e->graph.push_back(std::move(ly));
     }
@@ -717,12 +821,27 @@ int64_t orc_gate_stats_len(void *h) { return static_cast<int64_t>(static_cast<En
 // Expectation with no graph: core + dot (inl:1034-1041 -> MPFunctions.cpp:196-205).
 double orc_expectation(void *h) {
     auto *e = static_cast<Engine *>(h);
+    if (e->schr) return e->expectation_schr(e->coeffs, e->observable_vector());
     return e->expectation_of(e->coeffs);
 }
 
 // Functional replay of the stored graph at params (evolve_operator, Evolution.cpp:529-540).
 int orc_graph_ev(void *h, const double *params, int64_t n_params, double *out) {
     auto *e = static_cast<Engine *>(h);
+    if (e->schr) {
+        // the observable evolves through the layers newest first at +alpha and meets the state
+        // (MonomialPropagator.inl:918-1000 make_functional_, MPFunctions.cpp:71-82, 196-205)
+        std::vector<double> op = e->observable_vector();
+        std::vector<uint32_t> cos;
+        for (size_t l = e->graph.size(); l-- > 0;) {
+            const Layer &ly = e->graph[l];
+            if (ly.param_index >= n_params) return fail("parameters too short");
+            e->layer_cos(ly, cos);
+            e->evolve_step(op, ly, cos, 1.0 * params[ly.param_index] * ly.gen_coeff);
+        }
+        *out = e->expectation_schr(e->coeffs, op);
+        return 0;
+    }
     std::vector<double> op = e->init_coeffs;
     op.resize(e->size(), 0.0);
     std::vector<uint32_t> cos;
@@ -737,15 +856,16 @@ int orc_graph_ev(void *h, const double *params, int64_t n_params, double *out) {
 }
 
 // Replayed coefficient vector at params (contract_partially(params, inplace=False)).
+// Schrodinger: the state evolves oldest layer first at -alpha (MonomialPropagator.inl:1069-1083).
 int orc_graph_contract(void *h, const double *params, int64_t n_params, double *out) {
     auto *e = static_cast<Engine *>(h);
-    std::vector<double> op = e->init_coeffs;
+    std::vector<double> op = e->schr ? e->coeffs : e->init_coeffs;
     op.resize(e->size(), 0.0);
     std::vector<uint32_t> cos;
     for (const Layer &ly : e->graph) {
         if (ly.param_index >= n_params) return fail("parameters too short");
         e->layer_cos(ly, cos);
-        e->evolve_step(op, ly, cos, 1.0 * params[ly.param_index] * ly.gen_coeff);
+        e->evolve_step(op, ly, cos, (e->schr ? -1.0 : 1.0) * params[ly.param_index] * ly.gen_coeff);
     }
     std::memcpy(out, op.data(), op.size() * sizeof(double));
     return 0;
@@ -760,12 +880,14 @@ int orc_graph_grad(void *h, const double *params, int64_t n_params, double *valu
     const size_t nl = e->graph.size();
     std::vector<std::vector<uint32_t>> cos_only(nl);
     std::vector<std::vector<double>> snap_cos(nl), snap_rot(nl);
-    std::vector<double> op = e->init_coeffs;
+    // replay order: the stored layers, reversed in the Schrodinger picture (the observable evolves)
+    auto at = [&](size_t l) -> const Layer & { return e->graph[e->schr ? nl - 1 - l : l]; };
+    std::vector<double> op = e->schr ? e->observable_vector() : e->init_coeffs;
     op.resize(e->size(), 0.0);
     std::vector<char> endpoint(e->size(), 0);
     std::vector<uint32_t> cos;
     for (size_t l = 0; l < nl; ++l) {
-        const Layer &ly = e->graph[l];
+        const Layer &ly = at(l);
         if (ly.param_index >= n_params) return fail("parameters too short");
         e->layer_cos(ly, cos);
         for (const Rotation &r : ly.rots) endpoint[r.src] = endpoint[r.tgt] = 1;
@@ -781,13 +903,18 @@ int orc_graph_grad(void *h, const double *params, int64_t n_params, double *valu
         }
         e->evolve_step(op, ly, cos, 1.0 * params[ly.param_index] * ly.gen_coeff);
     }
-    *value = e->expectation_of(op);
     std::vector<double> lam(e->size(), 0.0);
-    for (size_t i = 0; i < e->size(); ++i)
-        if (e->paired(e->row(i))) lam[i] = e->state_phase(e->row(i));
+    if (e->schr) {
+        *value = e->expectation_schr(e->coeffs, op);
+        lam = e->coeffs;
+    } else {
+        *value = e->expectation_of(op);
+        for (size_t i = 0; i < e->size(); ++i)
+            if (e->paired(e->row(i))) lam[i] = e->state_phase(e->row(i));
+    }
     for (int64_t p = 0; p < n_params; ++p) grad[p] = 0.0;
     for (size_t l = nl; l-- > 0;) {
-        const Layer &ly = e->graph[l];
+        const Layer &ly = at(l);
         const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
         const double k = std::cos(2 * alpha), s = std::sin(2 * alpha);
         double d = 0.0;
