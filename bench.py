@@ -17,8 +17,11 @@ ring brickwork (2304 angles), measured after the timed steps; `batch` times a pa
 eight observables, 10 Trotter steps) batched against one propagation per pair.  The CPU baseline is the oracle
 (a serial restatement of the reference algorithm) on a bounded sample of the same workload.
 
+`--schrodinger Q` measures the Schrodinger picture instead (state terms of at most Q qubits to start,
+one propagation at a time, default theta_h 0.8: at 0.1 and 0.4 the 100-qubit state outgrows the GPU).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--trotter-steps 20] [--no-cpu-baseline]
-       [--no-gradient] [--batch-sets 4] [--unbatched]
+       [--no-gradient] [--batch-sets 4] [--unbatched] [--schrodinger Q] [--theta-h 0.1 0.4 0.8]
 """
 
 from __future__ import annotations
@@ -38,6 +41,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 THETA_H = (0.1, 0.4, 0.8)
+SCHRODINGER_QUBITS: int | None = None  # --schrodinger: the picture and its state cutoff in qubits
 
 
 def parse_args():
@@ -60,6 +64,9 @@ def parse_args():
     ap.add_argument("--unbatched", action="store_true",
                     help="time one propagation per theta_h (default: the three as members of one store)")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the unbatched steps for comparison")
+    ap.add_argument("--schrodinger", type=int, default=None, metavar="Q",
+                    help="Schrodinger picture, initial state terms of at most Q qubits (one propagation at a time)")
+    ap.add_argument("--theta-h", type=float, nargs="+", default=None, help="X angles of the step's propagations")
     return ap.parse_args()
 
 
@@ -114,8 +121,9 @@ def run_gpu_step(L, inputs, profile):
     evs = []
     sims = []
     for monos, gcs, pm, params in inputs:
+        schr = None if SCHRODINGER_QUBITS is None else 2 * SCHRODINGER_QUBITS
         sim = DeviceSimulator(observable_terms(n), 10, [], lower_atol=1e-5, cutoff_type="support",
-                              logical_num_modes=n, basis="pauli")
+                              logical_num_modes=n, basis="pauli", schrodinger_cutoff=schr)
         if profile:
             sim.set_profiling(True)
         sim.propagate(monos, pm, gcs, params)
@@ -127,14 +135,18 @@ def run_gpu_step(L, inputs, profile):
 
 
 def cpu_baseline(L, sample_steps):
-    """Oracle (serial, 1 core) on theta_h = 0.4 truncated to `sample_steps` Trotter steps."""
+    """Oracle (serial, 1 core) on theta_h = 0.4 (the Schrodinger mode: its first theta_h) truncated to
+    `sample_steps` Trotter steps."""
     import oracle
     from monoprop_amd import workloads
 
     n = L * L
-    circ = workloads.kicked_ising_lattice(L, sample_steps, theta_h=0.4)
+    th = 0.4 if SCHRODINGER_QUBITS is None else THETA_H[0]
+    circ = workloads.kicked_ising_lattice(L, sample_steps, theta_h=th)
     monos, gcs, pm, _ = layers_of(circ)
-    o = oracle.Oracle(observable_terms(n), n, basis="pauli", cutoff=10, cutoff_type="support", lower_atol=1e-5)
+    schr = None if SCHRODINGER_QUBITS is None else 2 * SCHRODINGER_QUBITS
+    o = oracle.Oracle(observable_terms(n), n, basis="pauli", cutoff=10, cutoff_type="support", lower_atol=1e-5,
+                      schrodinger_cutoff=schr)
     t0 = time.perf_counter()
     o.propagate(monos, pm, gcs, circ.parameters)
     dt = time.perf_counter() - t0
@@ -144,8 +156,9 @@ def cpu_baseline(L, sample_steps):
         "unit": "term-gate applications/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{L}x{L} kicked Ising theta_h=0.4, first {sample_steps} of 20 Trotter steps "
-                  f"({len(monos)} gates, {t} term-gate applications, {dt:.1f} s), serial oracle",
+        "sample": f"{L}x{L} kicked Ising theta_h={th}, first {sample_steps} of 20 Trotter steps "
+                  f"({len(monos)} gates, {t} term-gate applications, {dt:.1f} s), serial oracle"
+                  + ("" if schr is None else f", Schrodinger picture (state terms of <= {SCHRODINGER_QUBITS} qubits)"),
     }
 
 
@@ -291,7 +304,14 @@ def measured_traffic(kernel):
 
 
 def main():
+    global THETA_H, SCHRODINGER_QUBITS
     args = parse_args()
+    if args.schrodinger is not None:
+        SCHRODINGER_QUBITS = args.schrodinger
+        THETA_H = (0.8,)
+        args.unbatched = True  # the batched store is Heisenberg only
+    if args.theta_h:
+        THETA_H = tuple(args.theta_h)
     if args.copy_calibration:
         copy_calibration()
         return
@@ -365,7 +385,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.lattice}x{args.lattice} kicked-Ising Pauli propagation, {args.trotter_steps} Trotter "
-                        "steps, theta_J=-pi/4, theta_h in {0.1,0.4,0.8}, O=mean Z, cutoff 10, lower_atol 1e-5",
+                        f"steps, theta_J=-pi/4, theta_h in {{{','.join(str(t) for t in THETA_H)}}}, O=mean Z, "
+                        "cutoff 10, lower_atol 1e-5"
+                        + ("" if SCHRODINGER_QUBITS is None else
+                           f", Schrodinger picture (initial state terms of <= {SCHRODINGER_QUBITS} qubits)"),
             "qubits": args.lattice * args.lattice,
             "gates_per_propagation": (2 * args.lattice * (args.lattice - 1) + args.lattice ** 2) * args.trotter_steps,
             "term_gate_applications_per_step": total_t // args.steps,
@@ -379,7 +402,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dname),
+            # the PMC traffic file covers the default (Heisenberg) workload only
+            "traffic": measured_traffic(dname) if SCHRODINGER_QUBITS is None else None,
             "avg_launch_ms": avg_ms,
             "launches": d["launches"],
         },
