@@ -11,7 +11,11 @@ Metric: term-gate applications per second, T / t with T = sum over gates of the 
 the gate (counted on the device); t covers engine creation, propagation and <O> for the three
 circuits (their generator lists are built once, before timing).  The roofline figure is the dominant
 kernel's algorithmic bytes over its HIP-event-timed device duration, from one extra step of the same
-workload run with the per-launch profiler on (kept out of the timed steps).  `gradient` reports the
+workload run with the per-launch profiler on (kept out of the timed steps).  Algorithmic bytes per gate
+are the path's measurement definition: the sweep reads 16 B of x/z limbs per live term (per limb pair
+that meets the generator), reads and writes each anticommuting coefficient, reads the anticommuting
+terms' other limbs and writes each emitted partner record (row + value); the resolve reads those records
+back and appends each new term.  `hbm_GBps` is what the kernel really moves (PMC counters).  `gradient` reports the
 other half of the metric, the full-angle gradient's time against one forward replay, on the 64-qubit
 ring brickwork (2304 angles), measured after the timed steps; `batch` times a parameter sweep (sets x
 eight observables, 10 Trotter steps) batched against one propagation per pair.  The CPU baseline is the oracle
@@ -367,6 +371,8 @@ def main():
     dname, d = dom
     avg_ms = d["ms"] / max(d["launches"], 1)
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    # the PMC traffic file covers the default (Heisenberg) workload only
+    traffic = measured_traffic(dname) if SCHRODINGER_QUBITS is None else None
     kern_ms = sum(v["ms"] for v in prof.values())
     kern_bytes = sum(v["bytes"] for v in prof.values())
 
@@ -402,10 +408,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            # the PMC traffic file covers the default (Heisenberg) workload only
-            "traffic": measured_traffic(dname) if SCHRODINGER_QUBITS is None else None,
+            "traffic": traffic,
             "avg_launch_ms": avg_ms,
             "launches": d["launches"],
+            # what the kernel actually moves through HBM (PMC traffic over the event-timed launch); the
+            # column index needs far fewer bytes than the limb reads of the algorithmic figure
+            "hbm_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
         },
         "kernels": {
             k: {"ms": round(v["ms"], 3), "GB/s": round(v["bytes"] / (v["ms"] * 1e6), 1) if v["ms"] > 0 else 0.0,

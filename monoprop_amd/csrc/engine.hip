@@ -2521,16 +2521,29 @@ struct mpa_engine {
     // row, pre-gate value and tile entry.  Resolve: per emitter its tile entry,
     // pre-gate value, row, index slot (key inline) and coefficient writes; per hit the partner
     // coefficient (read + write); per appended row the row, coefficient, index slot and column bits.
-    void add_algorithmic_bytes_fused(const std::vector<int32_t> &fold_plus_piv) {
-        const double NW = nw;
-        for (size_t g = 0; g < stat_n.size(); ++g) {
-            if (fold_plus_piv[g] == 0) continue;
-            const double words = std::ceil(static_cast<double>(stat_n[g]) / 64.0);
-            const double tiles = std::ceil(words / kWTileWords);
-            const double nA = stat_a[g], E = stat_e[g], I = stat_i[g], hits = stat_p[g];
-            prof_bytes[K_SWEEP] += words * 8.0 * (fold_plus_piv[g] + 1) + tiles * 8 + nA * 16.0 + E * (8 * NW + 12);
-            prof_bytes[K_RESOLVE] += E * (4 + 8 + 8 * NW + 8 * (NW + 1) + 8) + hits * 16 +
-                                     I * (8 * NW + 8 + 8 * (NW + 1) + ncols / 8.0);
+    // Algorithmic bytes of the fused kernels per gate g, the per-unit figure of the propagate path's
+    // measurement definition: the anticommutation test reads, for every live term, the limbs that can
+    // meet supp(G) (lb bytes per limb unit, w_g units); each anticommuting coefficient is read and
+    // written; the anticommuting terms' remaining limbs form the partner keys; each emitted partner
+    // record (row + value) is written by the sweep and read back by the resolve; each new term is
+    // appended.
+    //   k_sweep:   lb*w_g*N_g + 16*A_g + lb*(U - w_g)*A_g + (R + 8)*E_g
+    //   k_resolve: (R + 8)*E_g + (R + 8)*I_g
+    // lb = 16 (an x and a z limb) for Pauli, 8 (one slot word) for Majorana; U the row's limb units;
+    // R = 8*NW the row bytes.  The column index reads far fewer bytes than the lb*w_g*N_g of the
+    // test (the PMC traffic counters show what the kernels actually move).
+    void add_algorithmic_bytes_fused(const std::vector<GateDesc> &gds) {
+        const bool pauli = basis == MPA_BASIS_PAULI;
+        const int units = pauli ? L : nw;
+        const double lb = pauli ? 16.0 : 8.0, R = 8.0 * nw;
+        for (size_t g = 0; g < stat_n.size() && g < gds.size(); ++g) {
+            if (gds[g].identity) continue;
+            int w = 0;
+            for (int u = 0; u < units; ++u) w += pauli ? ((gds[g].g[u] | gds[g].g[L + u]) != 0) : (gds[g].g[u] != 0);
+            const double N = static_cast<double>(stat_n[g]), A = static_cast<double>(stat_a[g]);
+            const double E = static_cast<double>(stat_e[g]), I = static_cast<double>(stat_i[g]);
+            prof_bytes[K_SWEEP] += lb * w * N + 16.0 * A + lb * (units - w) * A + (R + 8.0) * E;
+            prof_bytes[K_RESOLVE] += (R + 8.0) * E + (R + 8.0) * I;
         }
     }
 
@@ -2562,9 +2575,6 @@ struct mpa_engine {
         if (!folds.empty())
             HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemsetAsync(stats.p, 0, std::max<size_t>(ng, 1) * kStatW * 8, stream));
-        std::vector<int32_t> fold_plus_piv(ng, 0);
-        for (size_t g = 0; g < ng; ++g)
-            if (!gds[g].identity) fold_plus_piv[g] = gds[g].fold_cnt + (gds[g].piv0 >= 0) + (gds[g].piv1 >= 0);
         reset_fused_counters(0);
         hipEvent_t ev[2];
         HIP_TRY(hipEventCreate(&ev[0]));
@@ -2672,7 +2682,7 @@ struct mpa_engine {
             stat_l[g] = st[kStatW * g + 4];
             stat_p[g] = st[kStatW * g + 5];
         }
-        if (prof_on) add_algorithmic_bytes_fused(fold_plus_piv);
+        if (prof_on) add_algorithmic_bytes_fused(gds);
     }
 
     // <O> of every batch member (its identity coefficient plus its paired rows), into out[nbatch].
