@@ -118,9 +118,12 @@ int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t 
 /* Profiler (no counterpart in the reference): with profiling on, every kernel launch of propagate
  * is bracketed by HIP events on the engine's stream.  mpa_kernel_profile reports, per kernel class
  * `kind` (0 <= kind < 10; `name` is the kernel's name), the summed device time in ms, the summed
- * algorithmic bytes (the traffic the algorithm must move: columns streamed by the scan, rows and
- * coefficients gathered per anticommuting term, hash slots, appended rows) and the launch count,
- * accumulated since profiling was switched on.  Returns -1 past the last kind. */
+ * algorithmic bytes and the launch count, accumulated since profiling was switched on.  Algorithmic
+ * bytes of the fused kernels per gate: k_sweep 16 w N + 16 A + 16 (L - w) A + (16 L + 8) E, k_resolve
+ * (16 L + 8)(E + I) (N live, A anticommuting, E emitted, I inserted terms, L limbs per plane, w the
+ * limbs meeting the generator; Majorana: 8 bytes per slot word for 16 per limb pair); of the per-gate
+ * pipeline's kernels, the columns, rows, coefficients and hash slots they touch.  Returns -1 past the
+ * last kind. */
 int mpa_set_profiling(mpa_engine *e, int on);
 int mpa_kernel_profile(mpa_engine *e, int kind, const char **name, double *ms, double *bytes,
                        int64_t *launches);
@@ -130,6 +133,12 @@ int mpa_kernel_profile(mpa_engine *e, int kind, const char **name, double *ms, d
  * cos 2a != 0); 0 forces the per-gate pipeline that also serves build_graph.  Both produce the same
  * store and coefficients. */
 int mpa_set_pipeline(mpa_engine *e, int fused);
+
+/* Launch parameters of the fused pipeline (no counterpart in the reference; set by the autotuner,
+ * monoprop_amd/tune.py): the number of k_resolve blocks that take emitter chunks (default 1280) and the
+ * smallest count group, 2^min_group_log2 sweep tiles (default 6; 2..10).  Results do not depend on
+ * them. */
+int mpa_set_launch(mpa_engine *e, int32_t resolve_grid, int32_t min_group_log2);
 
 /* Device time of the last propagate call from first to last launch (HIP events), in ms. */
 int mpa_last_propagate_ms(mpa_engine *e, double *ms);

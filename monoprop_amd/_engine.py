@@ -12,6 +12,7 @@ falling back to any CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
 import os
 from collections.abc import Mapping, Sequence
@@ -21,6 +22,9 @@ import numpy as np
 
 _LIB_PATH = Path(__file__).resolve().parent / "libmonoprop_amd.so"
 _LIB = None
+TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
+_LAUNCH: tuple[int, int] | None = None  # (resolve_grid, min_group_log2) every new engine takes
+_LAUNCH_READ = False
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -51,6 +55,7 @@ _SIGNATURES = {
     "mpa_read_rows": (C.c_int, [_P, _u64p, _f64p]),
     "mpa_gate_stats": (C.c_int, [_P, _i64p, _i64p, _i64p, _i64p, _i64p]),
     "mpa_set_pipeline": (C.c_int, [_P, C.c_int]),
+    "mpa_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "mpa_set_profiling": (C.c_int, [_P, C.c_int]),
     "mpa_kernel_profile": (C.c_int, [_P, C.c_int, C.POINTER(C.c_char_p), _f64p, _f64p, _i64p]),
     "mpa_last_propagate_ms": (C.c_int, [_P, _f64p]),
@@ -91,6 +96,33 @@ def load_library():
             fn.argtypes = args
         _LIB = lib
     return _LIB
+
+
+def default_launch() -> tuple[int, int] | None:
+    """Launch parameters new engines take: those of set_default_launch(), else the autotuner's choice in
+    tuning.json (monoprop_amd/tune.py), else None (the engine's built-in defaults)."""
+    global _LAUNCH, _LAUNCH_READ
+    if _LAUNCH is None and not _LAUNCH_READ:
+        _LAUNCH_READ = True
+        try:
+            best = json.loads(TUNING_FILE.read_text())["best"]
+            _LAUNCH = (int(best["resolve_grid"]), int(best["min_group_log2"]))
+        except (OSError, KeyError, TypeError, ValueError):
+            _LAUNCH = None
+    return _LAUNCH
+
+
+def set_default_launch(resolve_grid: int | None, min_group_log2: int | None = None) -> None:
+    """Launch parameters for engines created from now on (None: the engine's built-in defaults)."""
+    global _LAUNCH, _LAUNCH_READ
+    _LAUNCH_READ = True
+    _LAUNCH = None if resolve_grid is None else (int(resolve_grid), int(min_group_log2))
+
+
+def _apply_launch(h) -> None:
+    launch = default_launch()
+    if launch is not None:
+        _check(load_library().mpa_set_launch(h, *launch))
 
 
 def _check(rc: int) -> None:
@@ -189,6 +221,7 @@ class DeviceSimulator:
             _check(lib.mpa_create_schrodinger(*head, int(schrodinger_cutoff), *tail))
         self._schrodinger = schrodinger_cutoff is not None
         self._h = h
+        _apply_launch(h)
         self._parameter_mapping: list[int] = []
         self._n_gates = 0
 
@@ -357,6 +390,10 @@ class DeviceSimulator:
         """Select the fused single-sweep pipeline (default) or the per-gate pipeline for propagate()."""
         _check(load_library().mpa_set_pipeline(self._h, 1 if fused else 0))
 
+    def set_launch(self, resolve_grid: int, min_group_log2: int) -> None:
+        """Launch parameters of the fused pipeline (results do not depend on them)."""
+        _check(load_library().mpa_set_launch(self._h, int(resolve_grid), int(min_group_log2)))
+
     def set_profiling(self, on: bool) -> None:
         """Bracket every kernel launch with HIP events (resets the accumulators)."""
         _check(load_library().mpa_set_profiling(self._h, 1 if on else 0))
@@ -508,6 +545,7 @@ class BatchSimulator:
                                              _ptr(members, C.c_int32), _ptr(st, C.c_int32), len(list(initial_state)),
                                              default_device()))
             self._h = h
+            _apply_launch(h)
             return
         keys: list[tuple[int, ...]] = []
         vals: list[complex] = []
@@ -527,6 +565,7 @@ class BatchSimulator:
                                     _ptr(im, C.c_double), _ptr(st, C.c_int32), len(list(initial_state)),
                                     default_device()))
         self._h = h
+        _apply_launch(h)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -618,6 +657,10 @@ class BatchSimulator:
     def set_profiling(self, on: bool) -> None:
         """Bracket every kernel launch with HIP events (resets the accumulators)."""
         _check(load_library().mpa_set_profiling(self._h, 1 if on else 0))
+
+    def set_launch(self, resolve_grid: int, min_group_log2: int) -> None:
+        """Launch parameters of the fused pipeline (results do not depend on them)."""
+        _check(load_library().mpa_set_launch(self._h, int(resolve_grid), int(min_group_log2)))
 
     def kernel_profile(self) -> dict[str, dict[str, float]]:
         """Per kernel: summed device ms, algorithmic bytes and launches since profiling was enabled."""

@@ -1859,6 +1859,10 @@ struct mpa_engine {
         t_gsum{&stream}, desc_m{&stream};
     uint64_t gsum_stride = 0;  // per gate parity
     int group_log2 = kMinGroupLog2;
+    // launch parameters of the fused pipeline (mpa_set_launch): resolve blocks that take chunks, and
+    // the smallest count group (2^min_group_log2 sweep tiles)
+    uint32_t res_grid_cap = kMaxGrid;
+    int min_group_log2 = kMinGroupLog2;
     uint64_t gidx_slots = 0;
     uint64_t gidx_rows = 0;     // rows [0, gidx_rows) are indexed
     uint64_t fused_cap = 0;     // capacity the per-row fused arrays were sized for
@@ -2480,7 +2484,7 @@ struct mpa_engine {
             t_item.reserve(tiles * kWTileRows * 4);
             t_cnt.reserve(tiles * 8);
             t_bsum.reserve(((tiles + kWaves - 1) / kWaves + 1) * 8);
-            group_log2 = kMinGroupLog2;
+            group_log2 = min_group_log2;
             while (group_log2 < kMaxGroupLog2 && ((tiles + (uint64_t{1} << group_log2) - 1) >> group_log2) + 1 > kMaxGroups)
                 ++group_log2;
             gsum_stride = ((tiles + (uint64_t{1} << group_log2) - 1) >> group_log2) + 1;
@@ -2601,7 +2605,7 @@ struct mpa_engine {
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
             const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
             const uint32_t rgrid =
-                static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, kMaxGrid));
+                static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, res_grid_cap));
             const uint64_t gmask = gidx_slots - 1;
             const size_t rlds = 2 * (gsum_stride + 1) * sizeof(uint32_t);
             for (size_t g = gi; g < end; ++g) {
@@ -3064,6 +3068,8 @@ std::unique_ptr<mpa_engine> clone_config(const mpa_engine *e) {
     o->occ = e->occ;
     o->device = e->device;
     o->fused_enabled = e->fused_enabled;
+    o->res_grid_cap = e->res_grid_cap;
+    o->min_group_log2 = e->min_group_log2;
     HIP_TRY(hipSetDevice(o->device));
     HIP_TRY(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&o->h_counters), C_COUNT * 8, hipHostMallocDefault));
@@ -3718,6 +3724,19 @@ int mpa_gate_stats(mpa_engine *e, int64_t *len, int64_t *n, int64_t *a, int64_t 
             ins[i] = e->stat_i[i];
         }
         *len = k;
+    });
+}
+
+int mpa_set_launch(mpa_engine *e, int32_t resolve_grid, int32_t min_group_log2) {
+    return guarded([&] {
+        if (resolve_grid < 1 || resolve_grid > 65536) throw ArgError("resolve_grid must be in 1..65536");
+        if (min_group_log2 < 2 || min_group_log2 > kMaxGroupLog2)
+            throw ArgError("min_group_log2 must be in 2.." + std::to_string(kMaxGroupLog2));
+        e->res_grid_cap = static_cast<uint32_t>(resolve_grid);
+        if (e->min_group_log2 != min_group_log2) {
+            e->min_group_log2 = min_group_log2;
+            e->fused_cap = 0;  // the count groups are laid out again before the next fused gate
+        }
     });
 }
 

@@ -36,3 +36,19 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_engine, "_LIB_PATH", tmp_path / "absent.so")
     with pytest.raises(ImportError):
         _engine.load_library()
+
+
+def test_default_launch_reads_the_tuning_file(monkeypatch, tmp_path):
+    from monoprop_amd import _engine
+
+    f = tmp_path / "tuning.json"
+    f.write_text('{"best": {"resolve_grid": 2048, "min_group_log2": 5, "ms": 1.0}}')
+    monkeypatch.setattr(_engine, "TUNING_FILE", f)
+    monkeypatch.setattr(_engine, "_LAUNCH", None)
+    monkeypatch.setattr(_engine, "_LAUNCH_READ", False)
+    assert _engine.default_launch() == (2048, 5)
+    _engine.set_default_launch(None)
+    assert _engine.default_launch() is None
+    monkeypatch.setattr(_engine, "_LAUNCH_READ", False)
+    f.write_text("not json")
+    assert _engine.default_launch() is None

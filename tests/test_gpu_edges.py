@@ -125,3 +125,21 @@ def test_error_paths():
         DeviceSimulator({(0,): 1.0}, 4, [], logical_num_modes=n, basis="pauli", schrodinger_cutoff=-1)
     with pytest.raises(ValueError):
         PauliPropagator(PauliOperator({"Z": 1.0}, 300), [], cutoff=4)  # past 256 qubits
+
+
+@pytest.mark.parametrize("launch", [(640, 4), (2560, 7), (64, 2)])
+def test_launch_parameters_do_not_change_results(oracle_mod, launch):
+    """The autotuner's knobs (resolve grid, smallest count group) leave the store bit-exact."""
+    print(f"case: launch {launch}", flush=True)
+    from monoprop_amd import workloads
+
+    circ = workloads.kicked_ising_lattice(6, 6, theta_h=0.4)
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, 36)
+    terms = workloads.mean_z(36).get_local_operator().terms
+    o = oracle_mod.Oracle(terms, 36, basis="pauli", cutoff=8, cutoff_type="support", lower_atol=1e-6)
+    o.propagate(monos, pm, gcs, circ.parameters)
+    d = DeviceSimulator(terms, 8, [], lower_atol=1e-6, cutoff_type="support", logical_num_modes=36, basis="pauli")
+    d.set_launch(*launch)
+    d.propagate(monos, pm, gcs, circ.parameters)
+    assert o.size() > 50000
+    _same(o, d)
