@@ -54,6 +54,14 @@ int mpa_create_schrodinger(mpa_engine **out, int basis, int num_modes, uint32_t 
                            const double *coeff_im, int64_t n_terms, const int32_t *initial_state,
                            int64_t n_state, int device);
 
+/* Replaces MonomialPropagator::update_initial_operator (MonomialPropagator.inl:353-377,
+ * MPOperator.h:193-235, binder.h:153-156): the core term (empty tuple) is replaced first; in the
+ * Heisenberg picture every other term must already be a row (-2, "Operator term [..] not found in the
+ * operator.", nothing else changed) and the operator -- also the vector a stored graph replays from --
+ * becomes exactly the given terms; in the Schrodinger picture the observable becomes the given terms. */
+int mpa_update_initial_operator(mpa_engine *e, const int64_t *term_offsets, const int32_t *term_slots,
+                                const double *coeff_re, const double *coeff_im, int64_t n_terms);
+
 /* Replaces MonomialPropagator::schrodinger (binder.h:191-193): 1 for the Schrodinger picture. */
 int mpa_is_schrodinger(mpa_engine *e, int32_t *out);
 

@@ -41,6 +41,7 @@ _SIGNATURES = {
                                          C.c_double, C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_int64, _i32p,
                                          C.c_int64, C.c_int]),
     "mpa_is_schrodinger": (C.c_int, [_P, _i32p]),
+    "mpa_update_initial_operator": (C.c_int, [_P, _i64p, _i32p, _f64p, _f64p, C.c_int64]),
     "mpa_destroy": (C.c_int, [_P]),
     "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
@@ -224,6 +225,7 @@ class DeviceSimulator:
         _apply_launch(h)
         self._parameter_mapping: list[int] = []
         self._n_gates = 0
+        self._operator_epoch = 0  # update_initial_operator() calls; functionals created before go stale
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -306,15 +308,41 @@ class DeviceSimulator:
                                                                  C.byref(out), _ptr(g, C.c_double)))
         return out.value, list(g[: len(params)])
 
-    def expectation_value_functional(self, pare_threshold=None):
+    def _functional(self, fn, pare_threshold):
+        """A callable bound to the operator and graph of now: it raises once either has changed
+        (Validation.cpp:102-117), as the reference's functionals do."""
         if pare_threshold is not None:
             raise ValueError("graph paring is not supported by the HIP engine.")
-        return lambda params: self.expectation_value(params)
+        epoch, layers = self._operator_epoch, self.graph_layers()
+
+        def call(params):
+            if self._operator_epoch != epoch:
+                raise RuntimeError("MP object has been modified since the functional was created. The initial "
+                                   "operator was re-weighted, so the coefficients the functional snapshotted are "
+                                   "stale; create a new functional.")
+            if self.graph_layers() != layers:
+                raise RuntimeError("MP object has been modified since the functional was created. Previous number "
+                                   f"of graph layers was {layers} and now is {self.graph_layers()}.")
+            return fn(params)
+
+        return call
+
+    def expectation_value_functional(self, pare_threshold=None):
+        return self._functional(self.expectation_value, pare_threshold)
 
     def expectation_value_and_gradient_functional(self, pare_threshold=None):
-        if pare_threshold is not None:
-            raise ValueError("graph paring is not supported by the HIP engine.")
-        return lambda params: self.expectation_value_and_gradient(params)
+        return self._functional(self.expectation_value_and_gradient, pare_threshold)
+
+    def update_initial_operator(self, op_dict) -> None:
+        """Re-weight the initial operator (Heisenberg: existing terms only; Schrodinger: the observable)."""
+        keys = [tuple(int(i) for i in k) for k in op_dict]
+        vals = [complex(op_dict[k]) for k in op_dict]
+        offs, slots = _csr(keys)
+        re = _f64(v.real for v in vals)
+        im = _f64(v.imag for v in vals)
+        _check(load_library().mpa_update_initial_operator(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
+                                                          _ptr(re, C.c_double), _ptr(im, C.c_double), len(keys)))
+        self._operator_epoch += 1
 
     def evolved_operator(self, parameters, atol: float) -> dict:
         lib = load_library()

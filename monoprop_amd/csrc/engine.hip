@@ -2941,6 +2941,19 @@ struct RunSchrExpect {
     }
 };
 
+// update_initial_operator: terms -> the rows holding them (the observable's matcher), the new
+// coefficient vector scattered over the store into `dst`; returns the terms no row holds.
+struct RunMatchTerms {
+    mpa_engine *e;
+    double *dst;
+    std::vector<int64_t> *hit_row;
+    template <int NW, int PAULI>
+    void operator()() {
+        std::vector<double> hv;
+        e->match_observable<NW>(e->coef.as<double>(), *hit_row, hv, dst);
+    }
+};
+
 struct RunGradient {
     mpa_engine *e;
     const double *params;
@@ -3267,6 +3280,61 @@ int mpa_create_schrodinger(mpa_engine **out, int basis, int num_modes, uint32_t 
         paired_state_rows(e.get(), pairs, hrows, hcoef);
         load_initial(e.get(), hcoef.size(), hrows.data(), hcoef.data(), nullptr);
         *out = e.release();
+    });
+}
+
+int mpa_update_initial_operator(mpa_engine *e, const int64_t *term_offsets, const int32_t *term_slots,
+                                const double *coeff_re, const double *coeff_im, int64_t n_terms) {
+    return guarded([&] {
+        require_single(e);
+        std::vector<uint64_t> rows;
+        std::vector<double> coefs;
+        // the core term changes as soon as it is read, before any term is looked up (MonomialPropagator.inl:363-376)
+        encode_operator(e, term_offsets, term_slots, coeff_re, coeff_im, 0, n_terms, rows, coefs, &e->core);
+        if (e->schr) {  // the observable is replaced; absent terms wait for a row to hold them (MPOperator.h:218-226)
+            e->obs_rows = std::move(rows);
+            e->obs_coef = std::move(coefs);
+            e->obs_tmask = 0;
+            return;
+        }
+        // Heisenberg: every term must be a row; the operator becomes exactly these terms (MPOperator.h:193-235)
+        std::vector<uint64_t> keep_rows = std::move(e->obs_rows);
+        std::vector<double> keep_coef = std::move(e->obs_coef);
+        e->obs_rows = rows;
+        e->obs_coef = coefs;
+        e->obs_tmask = 0;
+        DevArray fresh{&e->stream};
+        fresh.reserve(std::max<uint64_t>(e->N, 1) * 8);
+        std::vector<int64_t> hit;
+        RunMatchTerms mt{e, fresh.as<double>(), &hit};
+        try {
+            e->dispatch(mt);
+        } catch (...) {
+            e->obs_rows = std::move(keep_rows);
+            e->obs_coef = std::move(keep_coef);
+            e->obs_tmask = 0;
+            throw;
+        }
+        e->obs_rows.clear();
+        e->obs_coef.clear();
+        e->obs_tmask = 0;
+        for (size_t j = 0; j < hit.size(); ++j) {
+            if (hit[j] >= 0) continue;
+            std::vector<int32_t> slots;
+            e->words_to_slots(rows.data() + j * e->nw, slots);
+            std::string repr = "[";
+            for (size_t k = 0; k < slots.size(); ++k) repr += (k ? ", " : "") + std::to_string(slots[k]);
+            throw RunError("Operator term " + repr + "] not found in the operator.");
+        }
+        if (e->N) {
+            HIP_TRY(hipMemcpyAsync(e->coef.p, fresh.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+            if (!e->layers.empty()) {  // the graph replays from the new operator
+                e->base.reserve(e->N * 8);
+                HIP_TRY(hipMemcpyAsync(e->base.p, fresh.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
+                e->base_n = e->N;
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(e->stream));
     });
 }
 

@@ -50,7 +50,11 @@ class MajoranaPropagator(MonomialPropagator[MajoranaOperator]):
         return MajoranaOperator(terms, self.num_modes)
 
     def update_initial_operator(self, new_operator: MajoranaOperator) -> None:
-        raise RuntimeError("update_initial_operator is not supported by the HIP engine.")
+        """Replace the initial operator's coefficients (reference MonomialPropagator.inl:353-377): in the
+        Heisenberg picture every term must already be in the operator (RuntimeError otherwise) and the
+        graph is kept; in the Schrodinger picture the observable is replaced.  Functionals created
+        earlier raise from then on."""
+        self._simulator.update_initial_operator(new_operator.terms)
 
     @MonomialPropagator.cutoff_type.setter
     def cutoff_type(self, new_cutoff_type: str) -> None:

@@ -56,4 +56,8 @@ class PauliPropagator(MonomialPropagator[PauliOperator]):
         return circuit.gates
 
     def update_initial_operator(self, new_operator: PauliOperator) -> None:
-        raise RuntimeError("update_initial_operator is not supported by the HIP engine.")
+        """Replace the initial operator's coefficients (reference MonomialPropagator.inl:353-377): in the
+        Heisenberg picture every term must already be in the operator (RuntimeError otherwise) and the
+        graph is kept; in the Schrodinger picture the observable is replaced.  Functionals created
+        earlier raise from then on."""
+        self._simulator.update_initial_operator(new_operator.get_local_operator().terms)

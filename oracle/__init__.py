@@ -42,6 +42,8 @@ def _lib():
                                                C.c_int64, i64p, i32p, f64p, f64p, C.c_int64, i32p, C.c_int64]
         lib.orc_is_schrodinger.restype = C.c_int
         lib.orc_is_schrodinger.argtypes = [P]
+        lib.orc_update_initial_operator.restype = C.c_int
+        lib.orc_update_initial_operator.argtypes = [P, i64p, i32p, f64p, f64p, C.c_int64]
         lib.orc_destroy.argtypes = [P]
         lib.orc_size.restype = C.c_int64
         lib.orc_size.argtypes = [P]
@@ -219,6 +221,17 @@ class Oracle:
         arrs = [np.zeros(max(n, 1), dtype=np.int64) for _ in range(4)]
         _lib().orc_gate_stats(self._h, *[_ptr(a, C.c_int64) for a in arrs])
         return tuple(a[:n] for a in arrs)
+
+    def update_initial_operator(self, terms: Mapping[tuple[int, ...], complex]) -> None:
+        keys = list(terms.keys())
+        offs, slots = _csr(keys)
+        vals = [complex(terms[k]) for k in keys]
+        re = np.asarray([v.real for v in vals] or [0.0], dtype=np.float64)
+        im = np.asarray([v.imag for v in vals] or [0.0], dtype=np.float64)
+        rc = _lib().orc_update_initial_operator(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
+                                                _ptr(re, C.c_double), _ptr(im, C.c_double), len(keys))
+        if rc != 0:
+            raise RuntimeError(_lib().orc_last_error().decode())
 
     @property
     def schrodinger(self) -> bool:

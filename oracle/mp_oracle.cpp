@@ -743,6 +743,54 @@ void *orc_create_schrodinger(int basis, int n_modes, unsigned cutoff, int cutoff
 
 int orc_is_schrodinger(void *h) { return static_cast<Engine *>(h)->schr ? 1 : 0; }
 
+// update_initial_operator (MonomialPropagator.inl:353-377, MPOperator.h:193-235): the core term is
+// replaced first; Schrodinger: the observable becomes the given terms; Heisenberg: every term must be
+// a row, else "Operator term [..] not found in the operator."; the operator (and the vector a stored
+// graph replays from) becomes exactly the given terms at their rows.
+int orc_update_initial_operator(void *h, const int64_t *offsets, const int32_t *slots, const double *re,
+                                const double *im, int64_t n_terms) {
+    auto *e = static_cast<Engine *>(h);
+    std::vector<uint64_t> rows;
+    std::vector<double> coefs;
+    std::vector<uint64_t> m(e->nw);
+    for (int64_t t = 0; t < n_terms; ++t) {
+        const int64_t n = offsets[t + 1] - offsets[t];
+        if (!slots_to_words(*e, slots + offsets[t], n, m.data())) return fail("index out of range");
+        std::complex<double> c(re[t], im[t]);
+        if (e->basis == 0) {
+            static const std::complex<double> kPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+            c = c / kPow[n_choose_2(static_cast<size_t>(n)) % 4];
+        }
+        if (std::abs(c.imag()) > 1e-10) return fail(e->basis == 1 ? "Non-real Pauli coeffs detected" : "Non-Hermitian coeffs detected");
+        if (n == 0) {
+            e->core = c.real();
+            continue;
+        }
+        rows.insert(rows.end(), m.begin(), m.end());
+        coefs.push_back(c.real());
+    }
+    if (e->schr) {
+        e->obs_rows = rows;
+        e->obs_coef = coefs;
+        return 0;
+    }
+    std::vector<double> op(e->size(), 0.0);
+    for (size_t j = 0; j < coefs.size(); ++j) {
+        const int64_t r = e->find(rows.data() + j * e->nw);
+        if (r < 0) {
+            std::vector<int32_t> tmp;
+            words_to_slots(*e, rows.data() + j * e->nw, tmp);
+            std::string repr = "[";
+            for (size_t k = 0; k < tmp.size(); ++k) repr += (k ? ", " : "") + std::to_string(tmp[k]);
+            return fail("Operator term " + repr + "] not found in the operator.");
+        }
+        op[static_cast<size_t>(r)] = coefs[j];
+    }
+    e->coeffs = op;
+    e->init_coeffs = op;
+    return 0;
+}
+
 void orc_destroy(void *h) { delete static_cast<Engine *>(h); }
 
 int64_t orc_size(void *h) { return static_cast<int64_t>(static_cast<Engine *>(h)->size()); }
