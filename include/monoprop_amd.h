@@ -62,8 +62,19 @@ int mpa_create_schrodinger(mpa_engine **out, int basis, int num_modes, uint32_t 
 int mpa_update_initial_operator(mpa_engine *e, const int64_t *term_offsets, const int32_t *term_slots,
                                 const double *coeff_re, const double *coeff_im, int64_t n_terms);
 
+/* Replaces MonomialPropagator::set_parameter_mapping (MonomialPropagator.inl:809-852, binder.h:241-244):
+ * the parameter index of every stored graph layer, in the engine's processing order (the Python
+ * adapter maps the reference's per-layer or per-gate mapping onto it); the graph then takes
+ * max + 1 parameters. */
+int mpa_set_parameter_mapping(mpa_engine *e, const int64_t *param_index, int64_t n_layers);
+
 /* Replaces MonomialPropagator::schrodinger (binder.h:191-193): 1 for the Schrodinger picture. */
 int mpa_is_schrodinger(mpa_engine *e, int32_t *out);
+
+/* Replaces MonomialPropagator's copy constructor behind __deepcopy__ (MonomialPropagator.inl:190-212,
+ * binder.h:112-116): an independent engine with the same store, coefficients, observable, truncation,
+ * initial state and stored graph, on the same device. */
+int mpa_clone(mpa_engine *e, mpa_engine **out);
 
 /* Releases all device memory of the engine. */
 int mpa_destroy(mpa_engine *e);

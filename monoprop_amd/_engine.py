@@ -42,6 +42,8 @@ _SIGNATURES = {
                                          C.c_int64, C.c_int]),
     "mpa_is_schrodinger": (C.c_int, [_P, _i32p]),
     "mpa_update_initial_operator": (C.c_int, [_P, _i64p, _i32p, _f64p, _f64p, C.c_int64]),
+    "mpa_set_parameter_mapping": (C.c_int, [_P, _i64p, C.c_int64]),
+    "mpa_clone": (C.c_int, [_P, C.POINTER(_P)]),
     "mpa_destroy": (C.c_int, [_P]),
     "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
@@ -223,7 +225,8 @@ class DeviceSimulator:
         self._schrodinger = schrodinger_cutoff is not None
         self._h = h
         _apply_launch(h)
-        self._parameter_mapping: list[int] = []
+        self._parameter_mapping: list[int] = []  # per graph layer, optimizer order
+        self._layer_gates: list[int] = []  # the gate of each graph layer, optimizer order
         self._n_gates = 0
         self._operator_epoch = 0  # update_initial_operator() calls; functionals created before go stale
 
@@ -232,6 +235,22 @@ class DeviceSimulator:
         if h and _LIB is not None:
             _LIB.mpa_destroy(h)
             self._h = None
+
+    def __deepcopy__(self, memo):
+        """An independent engine with the same state (the reference's __deepcopy__, binder.h:112-116)."""
+        import copy
+
+        h = _P()
+        _check(load_library().mpa_clone(self._h, C.byref(h)))
+        other = DeviceSimulator.__new__(DeviceSimulator)
+        for k, v in self.__dict__.items():
+            if k != "_h":
+                setattr(other, k, copy.deepcopy(v, memo))
+        other._h = h
+        return other
+
+    def __copy__(self):
+        return self.__deepcopy__({})
 
     # --- evolution -------------------------------------------------------------------------------
     @staticmethod
@@ -286,10 +305,13 @@ class DeviceSimulator:
         # (MonomialPropagator.inl:855-870): a Heisenberg call's layers go after the stored ones, a
         # Schrodinger call's before them (MPGraph.h:41-43)
         new = [int(p) for p in parameter_mapping]
+        new_gates = [offset + int(g) for g in gates]
         if self._schrodinger:
             self._parameter_mapping = list(self._parameter_mapping) + new
+            self._layer_gates = list(self._layer_gates) + new_gates
         else:
             self._parameter_mapping = new + list(self._parameter_mapping)
+            self._layer_gates = new_gates + list(self._layer_gates)
 
     # --- evaluation ------------------------------------------------------------------------------
     def expectation_value(self, parameters) -> float:
@@ -313,7 +335,7 @@ class DeviceSimulator:
         (Validation.cpp:102-117), as the reference's functionals do."""
         if pare_threshold is not None:
             raise ValueError("graph paring is not supported by the HIP engine.")
-        epoch, layers = self._operator_epoch, self.graph_layers()
+        epoch, layers, mapping = self._operator_epoch, self.graph_layers(), list(self._parameter_mapping)
 
         def call(params):
             if self._operator_epoch != epoch:
@@ -323,7 +345,15 @@ class DeviceSimulator:
             if self.graph_layers() != layers:
                 raise RuntimeError("MP object has been modified since the functional was created. Previous number "
                                    f"of graph layers was {layers} and now is {self.graph_layers()}.")
-            return fn(params)
+            if self._parameter_mapping == mapping:
+                return fn(params)
+            # re-wired since: this functional keeps the mapping it was built with
+            now = list(self._parameter_mapping)
+            self._set_engine_mapping(mapping)
+            try:
+                return fn(params)
+            finally:
+                self._set_engine_mapping(now)
 
         return call
 
@@ -391,6 +421,7 @@ class DeviceSimulator:
                                            _ptr(out, C.c_double)))
         if inplace:
             self._parameter_mapping = []
+            self._layer_gates = []
             self._n_gates = 0
         return list(out[: self.size()])
 
@@ -515,7 +546,25 @@ class DeviceSimulator:
 
     @parameter_mapping.setter
     def parameter_mapping(self, mapping) -> None:
-        raise RuntimeError("re-wiring the parameter mapping of a stored graph is not supported by the HIP engine.")
+        """Re-wire the stored graph (MonomialPropagator.inl:809-852): one entry per layer in optimizer
+        order, or one per gate (every layer of a gate takes the gate's entry); per layer wins when the
+        two lengths coincide."""
+        m = [int(x) for x in mapping]
+        if len(m) == len(self._parameter_mapping):
+            per_layer = m
+        elif len(m) == self.n_gates():
+            per_layer = [m[g] for g in self._layer_gates]
+        else:
+            raise RuntimeError(f"parameter_mapping has {len(m)} entries; expected {len(self._parameter_mapping)} "
+                               f"(per graph layer) or {self.n_gates()} (per gate).")
+        self._set_engine_mapping(per_layer)
+        self._parameter_mapping = per_layer
+
+    def _set_engine_mapping(self, per_layer) -> None:
+        # the engine keeps layers in processing order: the optimizer order reversed (Heisenberg) or as is
+        stored = list(per_layer) if self._schrodinger else list(per_layer)[::-1]
+        a = np.asarray(stored or [0], dtype=np.int64)
+        _check(load_library().mpa_set_parameter_mapping(self._h, _ptr(a, C.c_int64), len(stored)))
 
 
 def dispatch(num_modes: int):

@@ -3338,6 +3338,22 @@ int mpa_update_initial_operator(mpa_engine *e, const int64_t *term_offsets, cons
     });
 }
 
+int mpa_set_parameter_mapping(mpa_engine *e, const int64_t *param_index, int64_t n_layers) {
+    return guarded([&] {
+        require_single(e);
+        if (n_layers != static_cast<int64_t>(e->layers.size()))
+            throw RunError("parameter_mapping has " + std::to_string(n_layers) + " entries; expected " +
+                           std::to_string(e->layers.size()) + " (per graph layer).");
+        int64_t np = 0;
+        for (int64_t i = 0; i < n_layers; ++i) {
+            if (param_index[i] < 0) throw ArgError("parameter indices must be non-negative");
+            np = std::max(np, param_index[i] + 1);
+        }
+        for (int64_t i = 0; i < n_layers; ++i) e->layers[static_cast<size_t>(i)].param_index = param_index[i];
+        e->graph_nparams = np;
+    });
+}
+
 int mpa_is_schrodinger(mpa_engine *e, int32_t *out) {
     *out = e->schr ? 1 : 0;
     return 0;
@@ -3572,6 +3588,61 @@ int mpa_batch_split(mpa_engine *e, const uint8_t *move, mpa_engine **out) {
             HIP_TRY(hipMemcpyAsync(x->counters.p, c0, 8, hipMemcpyHostToDevice, x->stream));
             HIP_TRY(hipStreamSynchronize(x->stream));
         }
+        *out = o.release();
+    });
+}
+
+int mpa_clone(mpa_engine *e, mpa_engine **out) {
+    return guarded([&] {
+        require_single(e);
+        if (!out) throw ArgError("out is null");
+        *out = nullptr;
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        auto o = clone_config(e);
+        o->core = e->core;
+        o->schr = e->schr;
+        o->obs_rows = e->obs_rows;
+        o->obs_coef = e->obs_coef;
+        o->n_init = e->n_init;
+        o->init_coef = e->init_coef;
+        // the store at the same capacity: rows, coefficients and the column index copied as they are;
+        // the row index is rebuilt by the next fused propagate (gidx_rows = 0)
+        o->alloc_store(e->cap);
+        const uint64_t N = e->N;
+        if (N) {
+            HIP_TRY(hipMemcpyAsync(o->rows.p, e->rows.p, N * e->nw * 8, hipMemcpyDeviceToDevice, o->stream));
+            HIP_TRY(hipMemcpyAsync(o->coef.p, e->coef.p, N * 8, hipMemcpyDeviceToDevice, o->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(o->cols.p, e->cols.p, static_cast<size_t>(e->ncols) * (e->cap / 64) * 8,
+                               hipMemcpyDeviceToDevice, o->stream));
+        o->N = N;
+        const int64_t c0[1] = {static_cast<int64_t>(N)};
+        HIP_TRY(hipMemcpyAsync(o->counters.p, c0, 8, hipMemcpyHostToDevice, o->stream));
+        // the graph: layer list, cos-only and rotation records, the vector it replays from
+        o->layers = e->layers;
+        o->graph_nparams = e->graph_nparams;
+        o->gcos_total = e->gcos_total;
+        o->grot_total = e->grot_total;
+        o->base_n = e->base_n;
+        if (e->gcos_total) {
+            o->g_cos.reserve(e->gcos_total * 4);
+            HIP_TRY(hipMemcpyAsync(o->g_cos.p, e->g_cos.p, e->gcos_total * 4, hipMemcpyDeviceToDevice, o->stream));
+        }
+        if (e->grot_total) {
+            o->g_rot.reserve(e->grot_total * 8);
+            HIP_TRY(hipMemcpyAsync(o->g_rot.p, e->g_rot.p, e->grot_total * 8, hipMemcpyDeviceToDevice, o->stream));
+        }
+        if (e->base_n) {
+            o->base.reserve(e->base_n * 8);
+            HIP_TRY(hipMemcpyAsync(o->base.p, e->base.p, e->base_n * 8, hipMemcpyDeviceToDevice, o->stream));
+        }
+        o->stat_n = e->stat_n;
+        o->stat_a = e->stat_a;
+        o->stat_e = e->stat_e;
+        o->stat_i = e->stat_i;
+        o->stat_l = e->stat_l;
+        o->stat_p = e->stat_p;
+        HIP_TRY(hipStreamSynchronize(o->stream));
         *out = o.release();
     });
 }
