@@ -76,6 +76,13 @@ int mpa_is_schrodinger(mpa_engine *e, int32_t *out);
  * initial state and stored graph, on the same device. */
 int mpa_clone(mpa_engine *e, mpa_engine **out);
 
+/* Replaces the memory probes operator_memory_bytes / graph_memory_bytes / operator_memory_breakdown
+ * (binder.h:247-273): device bytes held, into out[0..10]: operator_terms (rows), op_coeffs,
+ * state_coeffs (Schrodinger: the evolved state), indexing (row index), init_operator, initial_state,
+ * inverted_index (bit columns), total (sum of the seven), terms slack (capacity beyond the live rows),
+ * graph (records and the vector it replays from), work arrays kept between calls.  n >= 11. */
+int mpa_memory_breakdown(mpa_engine *e, int64_t *out, int32_t n);
+
 /* Releases all device memory of the engine. */
 int mpa_destroy(mpa_engine *e);
 

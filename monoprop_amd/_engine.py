@@ -44,6 +44,7 @@ _SIGNATURES = {
     "mpa_update_initial_operator": (C.c_int, [_P, _i64p, _i32p, _f64p, _f64p, C.c_int64]),
     "mpa_set_parameter_mapping": (C.c_int, [_P, _i64p, C.c_int64]),
     "mpa_clone": (C.c_int, [_P, C.POINTER(_P)]),
+    "mpa_memory_breakdown": (C.c_int, [_P, _i64p, C.c_int32]),
     "mpa_destroy": (C.c_int, [_P]),
     "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
@@ -251,6 +252,25 @@ class DeviceSimulator:
 
     def __copy__(self):
         return self.__deepcopy__({})
+
+    # --- memory probes (binder.h:247-273) -----------------------------------------------------------
+    _MEM_KEYS = ("operator_terms_bytes", "op_coeffs_bytes", "state_coeffs_bytes", "indexing_bytes",
+                 "init_operator_bytes", "initial_state_bytes", "inverted_index_bytes", "total_bytes",
+                 "d_terms_slack_bytes", "graph_bytes", "d_work_bytes")
+
+    def operator_memory_breakdown(self) -> dict[str, int]:
+        """Device bytes held by the operator, by part (total_bytes sums the first seven)."""
+        out = np.zeros(len(self._MEM_KEYS), dtype=np.int64)
+        _check(load_library().mpa_memory_breakdown(self._h, _ptr(out, C.c_int64), len(out)))
+        return {k: int(v) for k, v in zip(self._MEM_KEYS, out) if k != "graph_bytes"}
+
+    def operator_memory_bytes(self) -> int:
+        return self.operator_memory_breakdown()["total_bytes"]
+
+    def graph_memory_bytes(self) -> int:
+        out = np.zeros(len(self._MEM_KEYS), dtype=np.int64)
+        _check(load_library().mpa_memory_breakdown(self._h, _ptr(out, C.c_int64), len(out)))
+        return int(out[self._MEM_KEYS.index("graph_bytes")])
 
     # --- evolution -------------------------------------------------------------------------------
     @staticmethod

@@ -51,6 +51,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kScanBlock = 1024;
 constexpr int kExpectBlocks = 1024;
 constexpr uint64_t kMaxSchrodingerRows = uint64_t{1} << 28;  // initial Schrodinger state terms
+constexpr int kMemFields = 11;  // mpa_memory_breakdown entries
 
 // counters[] on the device
 enum : int {
@@ -3644,6 +3645,34 @@ int mpa_clone(mpa_engine *e, mpa_engine **out) {
         o->stat_p = e->stat_p;
         HIP_TRY(hipStreamSynchronize(o->stream));
         *out = o.release();
+    });
+}
+
+int mpa_memory_breakdown(mpa_engine *e, int64_t *out, int32_t n) {
+    return guarded([&] {
+        if (n < kMemFields) throw ArgError("memory breakdown needs " + std::to_string(kMemFields) + " entries");
+        const int64_t coef_b = static_cast<int64_t>(e->coef.bytes);
+        const int64_t obs_b = static_cast<int64_t>(e->obs_coef.size() * 8 + e->obs_tab.bytes + e->obs_c.bytes);
+        const int64_t init_b = static_cast<int64_t>(e->obs_rows.size() * 8 + e->init_coef.size() * 8);
+        int64_t f[kMemFields] = {};
+        f[0] = static_cast<int64_t>(e->rows.bytes + e->bmember.bytes);   // operator_terms_bytes
+        f[1] = e->schr ? obs_b : coef_b;                                  // op_coeffs_bytes
+        f[2] = e->schr ? coef_b : 0;                                      // state_coeffs_bytes (Heisenberg: scored on the fly)
+        f[3] = static_cast<int64_t>(e->gidx.bytes + e->table.bytes);      // indexing_bytes
+        f[4] = init_b;                                                    // init_operator_bytes
+        f[5] = static_cast<int64_t>(e->occ_dev.bytes);                    // initial_state_bytes
+        f[6] = static_cast<int64_t>(e->cols.bytes);                       // inverted_index_bytes (bit columns)
+        f[7] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6];            // total_bytes
+        f[8] = static_cast<int64_t>((e->cap - std::min(e->cap, e->N)) * e->nw * 8);  // d_terms_slack_bytes
+        f[9] = static_cast<int64_t>(e->g_cos.bytes + e->g_rot.bytes + e->base.bytes);  // graph bytes
+        size_t work = 0;  // propagate / evaluation scratch the engine keeps between calls
+        for (const DevArray *a : {&e->f_elead, &e->t_val, &e->t_item, &e->t_cnt, &e->t_bsum, &e->t_gsum, &e->desc_m,
+                                  &e->AL, &e->lead, &e->foll, &e->blk_a, &e->blk_b, &e->rec, &e->lv, &e->match,
+                                  &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam,
+                                  &e->gpartial, &e->partial})
+            work += a->bytes;
+        f[10] = static_cast<int64_t>(work);
+        std::copy(f, f + kMemFields, out);
     });
 }
 

@@ -143,3 +143,26 @@ def test_launch_parameters_do_not_change_results(oracle_mod, launch):
     d.propagate(monos, pm, gcs, circ.parameters)
     assert o.size() > 50000
     _same(o, d)
+
+
+def test_memory_probes():
+    """operator_memory_breakdown / operator_memory_bytes / graph_memory_bytes (binder.h:247-273)."""
+    print("case: memory probes", flush=True)
+    n = 20
+    from monoprop_amd import workloads
+
+    circ = workloads.kicked_ising_chain(n, 3)
+    monos, gcs, pm, gidx = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    d = DeviceSimulator(workloads.z_observable(n, [10]).get_local_operator().terms, 8, [], lower_atol=1e-8,
+                        cutoff_type="support", logical_num_modes=n, basis="pauli")
+    d.propagate(monos, pm, gcs, circ.parameters)
+    b = d.operator_memory_breakdown()
+    parts = ("operator_terms_bytes", "op_coeffs_bytes", "state_coeffs_bytes", "indexing_bytes",
+             "init_operator_bytes", "initial_state_bytes", "inverted_index_bytes")
+    assert b["total_bytes"] == sum(b[k] for k in parts) == d.operator_memory_bytes()
+    assert b["operator_terms_bytes"] >= d.size() * 16 and b["op_coeffs_bytes"] >= d.size() * 8
+    assert d.graph_memory_bytes() == 0
+    g = DeviceSimulator(workloads.z_observable(n, [10]).get_local_operator().terms, 8, [], lower_atol=1e-8,
+                        cutoff_type="support", logical_num_modes=n, basis="pauli")
+    g.build_graph(monos, pm, gcs, gidx)
+    assert g.graph_memory_bytes() > 0
