@@ -19,7 +19,8 @@ back and appends each new term.  `hbm_GBps` is what the kernel really moves (PMC
 other half of the metric, the full-angle gradient's time against one forward replay, on the 64-qubit
 ring brickwork (2304 angles), measured after the timed steps; `batch` times a parameter sweep (sets x
 eight observables, 10 Trotter steps) batched against one propagation per pair.  The CPU baseline is the oracle
-(a serial restatement of the reference algorithm) on a bounded sample of the same workload.
+(a restatement of the reference algorithm) on a bounded sample of the same workload, run on the host's
+thread budget (its serial rate beside it).
 
 `--schrodinger Q` measures the Schrodinger picture instead (state terms of at most Q qubits to start,
 one propagation at a time, default theta_h 0.8: at 0.1 and 0.4 the 100-qubit state outgrows the GPU).
@@ -64,7 +65,7 @@ def parse_args():
     ap.add_argument("--traffic-from", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "FETCH", "WRITE"),
                     help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
     ap.add_argument("--traffic-out", default=str(TRAFFIC_FILE))
-    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--cpu-sample-steps", type=int, default=6)
     ap.add_argument("--unbatched", action="store_true",
                     help="time one propagation per theta_h (default: the three as members of one store)")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the unbatched steps for comparison")
@@ -138,31 +139,63 @@ def run_gpu_step(L, inputs, profile):
     return total_t, evs, sims
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads() -> int:
+    """Threads for the CPU baseline: the job's thread budget (OMP_NUM_THREADS), else the cores."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return max(1, os.cpu_count() or 1)
+
+
 def cpu_baseline(L, sample_steps):
-    """Oracle (serial, 1 core) on theta_h = 0.4 (the Schrodinger mode: its first theta_h) truncated to
-    `sample_steps` Trotter steps."""
+    """The oracle on theta_h = 0.4 (the Schrodinger mode: its first theta_h) truncated to
+    `sample_steps` Trotter steps: the thread-parallel restatement over the host's thread budget, and
+    the serial one (1 core) on the first 4 steps beside it."""
     import oracle
     from monoprop_amd import workloads
 
     n = L * L
     th = 0.4 if SCHRODINGER_QUBITS is None else THETA_H[0]
-    circ = workloads.kicked_ising_lattice(L, sample_steps, theta_h=th)
-    monos, gcs, pm, _ = layers_of(circ)
     schr = None if SCHRODINGER_QUBITS is None else 2 * SCHRODINGER_QUBITS
-    o = oracle.Oracle(observable_terms(n), n, basis="pauli", cutoff=10, cutoff_type="support", lower_atol=1e-5,
-                      schrodinger_cutoff=schr)
-    t0 = time.perf_counter()
-    o.propagate(monos, pm, gcs, circ.parameters)
-    dt = time.perf_counter() - t0
-    t = int(o.gate_stats()[0].sum())
+    picture = "" if schr is None else f", Schrodinger picture (state terms of <= {SCHRODINGER_QUBITS} qubits)"
+
+    def run(steps, threads):
+        circ = workloads.kicked_ising_lattice(L, steps, theta_h=th)
+        monos, gcs, pm, _ = layers_of(circ)
+        oracle.set_threads(threads)
+        try:
+            o = oracle.Oracle(observable_terms(n), n, basis="pauli", cutoff=10, cutoff_type="support",
+                              lower_atol=1e-5, schrodinger_cutoff=schr)
+            t0 = time.perf_counter()
+            o.propagate(monos, pm, gcs, circ.parameters)
+            dt = time.perf_counter() - t0
+        finally:
+            oracle.set_threads(1)
+        t = int(o.gate_stats()[0].sum())
+        return t / dt, (f"{L}x{L} kicked Ising theta_h={th}, first {steps} of 20 Trotter steps ({len(monos)} gates, "
+                        f"{t} term-gate applications, {dt:.1f} s){picture}")
+
+    threads = _host_threads()
+    value, sample = run(sample_steps, threads)
+    serial, serial_sample = run(min(sample_steps, 4), 1)
     return {
-        "value": t / dt,
+        "value": value,
         "unit": "term-gate applications/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{L}x{L} kicked Ising theta_h={th}, first {sample_steps} of 20 Trotter steps "
-                  f"({len(monos)} gates, {t} term-gate applications, {dt:.1f} s), serial oracle"
-                  + ("" if schr is None else f", Schrodinger picture (state terms of <= {SCHRODINGER_QUBITS} qubits)"),
+        "sample": sample + f", oracle restatement on {threads} threads",
+        "cpu_model": _cpu_model(),
+        "serial": {"value": serial, "cores": 1, "sample": serial_sample + ", serial oracle"},
     }
 
 

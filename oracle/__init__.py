@@ -42,6 +42,8 @@ def _lib():
                                                C.c_int64, i64p, i32p, f64p, f64p, C.c_int64, i32p, C.c_int64]
         lib.orc_is_schrodinger.restype = C.c_int
         lib.orc_is_schrodinger.argtypes = [P]
+        lib.orc_set_threads.restype = None
+        lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_update_initial_operator.restype = C.c_int
         lib.orc_update_initial_operator.argtypes = [P, i64p, i32p, f64p, f64p, C.c_int64]
         lib.orc_destroy.argtypes = [P]
@@ -73,6 +75,12 @@ def _lib():
         lib.orc_gate_stats_len.argtypes = [P]
         _LIB = lib
     return _LIB
+
+
+def set_threads(n: int) -> None:
+    """Threads of the per-gate scan, resolve and apply (1: the serial restatement); the store, the
+    coefficients and every counter are the same for any count."""
+    _lib().orc_set_threads(int(n))
 
 
 def _ptr(a: np.ndarray, ctype):

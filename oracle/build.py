@@ -2,7 +2,8 @@
 
 Compiles ``mp_oracle.cpp`` with g++ into ``oracle/liboracle_mp.so``.  ``-ffp-contract=off`` keeps the
 floating-point operation order of the reference (no FMA contraction); ``-mpopcnt`` is the only ISA
-extension used so the library also runs on the GPU host's CPUs.
+extension used so the library also runs on the GPU host's CPUs; ``-fopenmp`` serves the optional
+thread-parallel restatement (``orc_set_threads``), whose results equal the serial ones.
 
 The reference itself cannot be built here (it needs C++23 ``<format>``/``<print>``, Boost >= 1.85,
 hwloc and nanobind), so there is no ``oracle/_ref`` build.
@@ -23,6 +24,7 @@ CXXFLAGS = [
     "-mpopcnt",
     "-ffp-contract=off",
     "-fno-fast-math",
+    "-fopenmp",
     "-fPIC",
     "-shared",
     "-Wall",

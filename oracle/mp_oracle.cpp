@@ -40,6 +40,16 @@ constexpr uint64_t kEven = 0x5555555555555555ULL;  // even Majorana slots (2j) i
 
 thread_local std::string g_last_error;
 
+// Threads of the per-gate scan, resolve and apply (orc_set_threads); 1 = the serial restatement.  The
+// parallel ranges are joined in their serial order, so every result is the same for any count.
+int g_threads = 1;
+constexpr size_t kParMin = 1 << 14;  // rows or queries below which a step stays serial
+
+inline size_t ranges_for(size_t n) {
+    if (g_threads <= 1 || n < kParMin) return 1;
+    return std::min<size_t>(static_cast<size_t>(g_threads) * 8, (n + kParMin / 4 - 1) / (kParMin / 4));
+}
+
 inline int popc(uint64_t x) { return __builtin_popcountll(x); }
 
 // n*(n-1)/2 in size_t arithmetic, as reference cpp/monoprop/Utilities.h:64-66.
@@ -372,30 +382,37 @@ struct Engine {
         followers.clear();
         std::vector<uint32_t> cos_set;
         const Pivot pv = pivot_of(g);
-        uint64_t tbuf[kMaxWords];
 
         auto is_below = [&](double abs_c) { return check_atol && (abs_sin * abs_c <= lower_atol); };
-        auto emit = [&](int mono_pop, uint32_t i, double abs_c, double v_src, bool foll) {
-            if (only_rot >= 0 && mono_pop > only_rot) return;
-            if (is_below(abs_c)) return;
-            const uint64_t *m = row(i);
-            for (int w = 0; w < nw; ++w) tbuf[w] = m[w] ^ g[w];
-            const bool struct_pass = keep(tbuf);
-            if (!struct_pass && !(check_upper && (abs_sin * abs_c >= upper_atol))) return;
-            const int phase = emit_phase(m, g, tbuf);
-            ++n_emit;
-            Queries &q = foll ? followers : leaders;
-            q.key.insert(q.key.end(), tbuf, tbuf + nw);
-            q.phase.push_back(phase);
-            q.src.push_back(i);
-            q.v_src.push_back(v_src);
+        // One range of rows: anticommutation, the fused cos sweep, the emission gate; queries and the cos
+        // set in ascending row order.  Rows are independent, so ranges run in parallel and are joined in
+        // range order (the serial order).
+        struct ScanOut {
+            Queries lead, foll;
+            std::vector<uint32_t> cos;
+            int64_t n_anti = 0, n_emit = 0;
         };
-
-        if (gpop != 0) {  // Scan.h:272-274: an empty fold generator builds nothing
-            for (size_t i = 0; i < n0; ++i) {
+        auto scan_range = [&](size_t lo, size_t hi, ScanOut &out) {
+            uint64_t tbuf[kMaxWords];
+            auto emit = [&](int mono_pop, uint32_t i, double abs_c, double v_src, bool foll) {
+                if (only_rot >= 0 && mono_pop > only_rot) return;
+                if (is_below(abs_c)) return;
+                const uint64_t *m = row(i);
+                for (int w = 0; w < nw; ++w) tbuf[w] = m[w] ^ g[w];
+                const bool struct_pass = keep(tbuf);
+                if (!struct_pass && !(check_upper && (abs_sin * abs_c >= upper_atol))) return;
+                const int phase = emit_phase(m, g, tbuf);
+                ++out.n_emit;
+                Queries &q = foll ? out.foll : out.lead;
+                q.key.insert(q.key.end(), tbuf, tbuf + nw);
+                q.phase.push_back(phase);
+                q.src.push_back(i);
+                q.v_src.push_back(v_src);
+            };
+            for (size_t i = lo; i < hi; ++i) {
                 const uint64_t *m = row(i);
                 if (!anticommutes(m, g, g_odd)) continue;
-                ++n_anti;
+                ++out.n_anti;
                 const bool foll = is_follower(m, pv);
                 if (only_rot < 0 && fused) {  // Scan.h:387-401 fused cos sweep
                     std::vector<double> &c = *cp;
@@ -411,7 +428,7 @@ struct Engine {
                     mono_pop = slot_pop(m);
                     if (mono_pop > only_rot) continue;
                 }
-                cos_set.push_back(static_cast<uint32_t>(i));
+                out.cos.push_back(static_cast<uint32_t>(i));
                 double v_src = 0.0, abs_c = 0.0;
                 if (contract) {
                     v_src = (*cp)[i];
@@ -424,6 +441,26 @@ struct Engine {
                     mono_pop = slot_pop(m);
                 }
                 emit(mono_pop, static_cast<uint32_t>(i), abs_c, v_src, foll);
+            }
+        };
+        auto append_queries = [nw = nw](Queries &dst, Queries &src) {
+            dst.key.insert(dst.key.end(), src.key.begin(), src.key.end());
+            dst.phase.insert(dst.phase.end(), src.phase.begin(), src.phase.end());
+            dst.src.insert(dst.src.end(), src.src.begin(), src.src.end());
+            dst.v_src.insert(dst.v_src.end(), src.v_src.begin(), src.v_src.end());
+            (void)nw;
+        };
+        if (gpop != 0) {  // Scan.h:272-274: an empty fold generator builds nothing
+            const size_t nr = ranges_for(n0);
+            std::vector<ScanOut> parts(nr);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads) if (nr > 1)
+            for (size_t r = 0; r < nr; ++r) scan_range(n0 * r / nr, n0 * (r + 1) / nr, parts[r]);
+            for (ScanOut &o : parts) {
+                append_queries(leaders, o.lead);
+                append_queries(followers, o.foll);
+                cos_set.insert(cos_set.end(), o.cos.begin(), o.cos.end());
+                n_anti += o.n_anti;
+                n_emit += o.n_emit;
             }
         }
         stat_a.push_back(n_anti);
@@ -443,18 +480,30 @@ struct Engine {
             cur_epoch_ = 0;
         }
         ++cur_epoch_;
+        // A pass over one side's queries; lookups are read-only and the leader pass marks distinct rows
+        // (f = src ^ G is one-to-one), so ranges run in parallel and are joined in query order.
         auto resolve = [&](const Queries &qs, bool leader_pass) {
-            for (size_t q = 0; q < qs.size(); ++q) {
-                if (!leader_pass && epoch_[qs.src[q]] == cur_epoch_) continue;
-                const int64_t f = find(qs.key.data() + q * nw);
-                if (f >= 0 && static_cast<size_t>(f) < n0) {
-                    if (leader_pass) epoch_[f] = cur_epoch_;
-                    double vt = 0.0;
-                    if (contract) vt = fused ? (*cp)[f] * inv_cos : (*cp)[f];
-                    hits.push_back({qs.src[q], static_cast<uint32_t>(f), qs.v_src[q], vt, qs.phase[q]});
-                } else {
-                    deferred.emplace_back(&qs, q);
+            const size_t nr = ranges_for(qs.size());
+            std::vector<std::vector<Rec>> rh(nr);
+            std::vector<std::vector<std::pair<const Queries *, size_t>>> rd(nr);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads) if (nr > 1)
+            for (size_t r = 0; r < nr; ++r) {
+                for (size_t q = qs.size() * r / nr; q < qs.size() * (r + 1) / nr; ++q) {
+                    if (!leader_pass && epoch_[qs.src[q]] == cur_epoch_) continue;
+                    const int64_t f = find(qs.key.data() + q * nw);
+                    if (f >= 0 && static_cast<size_t>(f) < n0) {
+                        if (leader_pass) epoch_[f] = cur_epoch_;
+                        double vt = 0.0;
+                        if (contract) vt = fused ? (*cp)[f] * inv_cos : (*cp)[f];
+                        rh[r].push_back({qs.src[q], static_cast<uint32_t>(f), qs.v_src[q], vt, qs.phase[q]});
+                    } else {
+                        rd[r].emplace_back(&qs, q);
+                    }
                 }
+            }
+            for (size_t r = 0; r < nr; ++r) {
+                hits.insert(hits.end(), rh[r].begin(), rh[r].end());
+                deferred.insert(deferred.end(), rd[r].begin(), rd[r].end());
             }
         };
         resolve(leaders, true);
@@ -481,10 +530,13 @@ struct Engine {
             const double cos_val = std::cos(2 * apply_alpha);
             const double sin_val = std::sin(2 * apply_alpha);
             if (!fused) {
-                for (uint32_t i : cos_set) c[i] *= cos_val;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1 && cos_set.size() > kParMin)
+                for (size_t j = 0; j < cos_set.size(); ++j) c[cos_set[j]] *= cos_val;
                 for (size_t k = n0; k < size(); ++k) c[k] *= cos_val;
             }
+            // every row is an endpoint of at most one record, so the records apply in parallel
             const size_t n_hit = hits.size();
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1 && n_hit + inserts.size() > kParMin)
             for (size_t k = 0; k < n_hit + inserts.size(); ++k) {
                 const bool is_insert = k >= n_hit;
                 const Rec &r = is_insert ? inserts[k - n_hit] : hits[k];
@@ -792,6 +844,9 @@ int orc_update_initial_operator(void *h, const int64_t *offsets, const int32_t *
 }
 
 void orc_destroy(void *h) { delete static_cast<Engine *>(h); }
+
+// Threads of the parallel restatement (1: serial); results are identical for every count.
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
 int64_t orc_size(void *h) { return static_cast<int64_t>(static_cast<Engine *>(h)->size()); }
 int orc_words(void *h) { return static_cast<Engine *>(h)->nw; }
