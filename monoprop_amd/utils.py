@@ -12,6 +12,22 @@ def validate_system_size(size: int, *, argument_name: str) -> int:
     return int(size)
 
 
+def is_antihermitian(indices) -> bool:
+    """Whether the Majorana product over ``indices`` is anti-Hermitian: (m_a1 ... m_aL)^dagger equals
+    (-1)^C(L,2) times the product, so it is anti-Hermitian iff C(L,2) = L(L-1)/2 is odd, i.e. iff L/2
+    (integer division) is odd (reference ``MajoranaAlgebra.h:44-47``, exported from ``bindings.cpp.in:123-126``)."""
+    return (len(indices) // 2) % 2 == 1
+
+
+def antihermitian_generator_correction(indices) -> complex:
+    """``c = i^((C(L,2) + 1) mod 4)`` for a Majorana product M of ``L = len(indices)`` distinct indices:
+    since M^dagger = (-1)^C(L,2) M, (c M)^dagger = -c M, so c M is anti-Hermitian and exp(theta c M) is
+    unitary for real theta (reference ``MajoranaAlgebra.h:49-51``, exported from
+    ``bindings.cpp.in:127-130``)."""
+    n = len(indices)
+    return (1, 1j, -1, -1j)[(n * (n - 1) // 2 + 1) % 4]
+
+
 def jordan_wigner_basis_change(n_qubits: int) -> list[list[int]]:
     """Slot support of the Jordan-Wigner image of each Majorana ``m_k`` in the native Pauli slots."""
     out: list[list[int]] = []
