@@ -174,6 +174,14 @@ int mpa_last_propagate_ms(mpa_engine *e, double *ms);
 int mpa_set_truncation(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower_atol,
                        double upper_atol);
 
+/* Replaces the basis_change constructor argument and setter (binder.h:71,185-189,
+ * MonomialPropagator.h:204-211): a CSR list of n_rows = 2 * num_modes slot lists, row s the image of
+ * Majorana slot s; the structural cutoff then tests the XOR of the images of a partner's slots
+ * (MonomialPropagatorCommon.h:44-63, MajoranaAlgebra.h:178-189).  n_rows < 0 restores the native
+ * basis.  -1 for a Pauli or batch engine or a row count other than 2 * num_modes, -2 for a slot
+ * >= 2 * num_modes; nothing changes on error.  Applies to gates propagated afterwards. */
+int mpa_set_basis_change(mpa_engine *e, const int64_t *offsets, const int32_t *slots, int64_t n_rows);
+
 /* Batched propagation.  One engine holds `n_members` independent propagations of one layer sequence:
  * member q starts from terms [member_offsets[q], member_offsets[q + 1]) of the CSR term list and is
  * propagated at its own parameter row.  It replaces n_members reference MonomialPropagator objects

@@ -11,7 +11,6 @@ the HIP target.
 """
 
 from ._engine import DeviceSimulator as _DeviceSimulator
-from .build import FLAGS as _FLAGS
 from .circuit import Circuit, ExpGate, expand_monomials, validate_parameter_mapping
 from .majorana import Majorana, MajoranaOperator
 from .majorana_propagator import MajoranaPropagator
@@ -25,7 +24,17 @@ MAX_NUM_MODES = _DeviceSimulator.storage_num_modes
 has_mpi = False
 __variant__ = "hip-gfx950"
 __build_type__ = "Release"
-__compiler_flags__ = " ".join(_FLAGS)
+
+
+def __getattr__(name):
+    # the build flags are read from the build script on first use, so that running it as
+    # ``python -m monoprop_amd.build`` does not import it twice
+    if name == "__compiler_flags__":
+        from .build import FLAGS
+
+        return " ".join(FLAGS)
+    raise AttributeError(f"module 'monoprop_amd' has no attribute {name!r}")
+
 
 __all__ = [
     "MAX_NUM_MODES",

@@ -64,6 +64,7 @@ _SIGNATURES = {
     "mpa_kernel_profile": (C.c_int, [_P, C.c_int, C.POINTER(C.c_char_p), _f64p, _f64p, _i64p]),
     "mpa_last_propagate_ms": (C.c_int, [_P, _f64p]),
     "mpa_set_truncation": (C.c_int, [_P, C.c_uint32, C.c_int, C.c_double, C.c_double]),
+    "mpa_set_basis_change": (C.c_int, [_P, _i64p, _i32p, C.c_int64]),
     "mpa_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_double, C.c_double,
                                    C.c_int32, _i64p, _i64p, _i32p, _f64p, _f64p, _i32p, C.c_int64, C.c_int]),
     "mpa_batch_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int64,
@@ -193,8 +194,6 @@ class DeviceSimulator:
             raise ValueError("MPI communicators are not supported: the engine holds the whole operator on one GPU.")
         if schrodinger_cutoff is not None and (isinstance(schrodinger_cutoff, bool) or int(schrodinger_cutoff) < 0):
             raise ValueError(f"schrodinger_cutoff must be a non-negative integer; got {schrodinger_cutoff!r}.")
-        if basis_change is not None:
-            raise ValueError("basis_change is not supported by the HIP engine.")
         if basis not in ("majorana", "pauli"):
             raise ValueError(f"Unknown basis {basis!r}; expected 'majorana' or 'pauli'.")
         if cutoff_type not in ("length", "support"):
@@ -225,6 +224,9 @@ class DeviceSimulator:
             _check(lib.mpa_create_schrodinger(*head, int(schrodinger_cutoff), *tail))
         self._schrodinger = schrodinger_cutoff is not None
         self._h = h
+        self._basis_change = None
+        if basis_change is not None:
+            self.basis_change = basis_change  # validated by the engine; __del__ frees it on error
         _apply_launch(h)
         self._parameter_mapping: list[int] = []  # per graph layer, optimizer order
         self._layer_gates: list[int] = []  # the gate of each graph layer, optimizer order
@@ -546,7 +548,21 @@ class DeviceSimulator:
 
     @property
     def basis_change(self):
-        return None
+        """The cutoff basis: 2 * num_modes slot lists (row s the image of Majorana slot s), or None for
+        the native basis (binder.h:185-189)."""
+        return None if self._basis_change is None else [list(r) for r in self._basis_change]
+
+    @basis_change.setter
+    def basis_change(self, rows) -> None:
+        lib = load_library()
+        if rows is None:
+            _check(lib.mpa_set_basis_change(self._h, None, None, -1))
+            self._basis_change = None
+            return
+        table = [tuple(int(i) for i in r) for r in rows]
+        offs, slots = _csr(table)
+        _check(lib.mpa_set_basis_change(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(table)))
+        self._basis_change = table
 
     @property
     def schrodinger(self) -> bool:

@@ -179,7 +179,31 @@ struct Trunc {
     int32_t check_lower, check_upper;
     double lower, upper;
     int32_t only_rot;  // -1: none
+    // Cutoff basis (Majorana only; null: native): row s (NW words) is the image of slot s, for every
+    // slot of the row width, rows past 2n zero (MonomialPropagator.inl:484-495).
+    const uint64_t *bchg;
 };
+
+// Structural keep test of a partner t, in the cutoff basis when one is set: the test runs on the XOR
+// of the images of t's slots (change_basis, MajoranaAlgebra.h:178-189, applied by the cutoff function
+// of MonomialPropagatorCommon.h:44-63), the fully-paired exemption included.
+template <int NW, int PAULI>
+__device__ __forceinline__ bool keep_cut(const uint64_t *t, const Trunc &tr) {
+    if (!PAULI && tr.bchg) {
+        uint64_t mp[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) mp[k] = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            for (uint64_t b = t[w]; b; b &= b - 1) {
+                const uint64_t *r = tr.bchg + (static_cast<uint64_t>(w) * 64 + static_cast<uint64_t>(__builtin_ctzll(b))) * NW;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) mp[k] ^= r[k];
+            }
+        return mpa::keep_partner<NW, PAULI>(mp, tr.cutoff, tr.cutoff_type);
+    }
+    return mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type);
+}
 
 // Rotation constants of one (gate, batch member) of a batched propagation: the member's own angle
 // alpha = theta[param_map] * g gives k = cos 2a, s = sin 2a, inv_k = 1 / cos 2a, abs_s = |sin 2a|.
@@ -322,7 +346,7 @@ __device__ __forceinline__ uint32_t classify(const uint64_t *m, const GateDesc &
     for (int w = 0; w < NW; ++w) t[w] = m[w] ^ gd.g[w];
     int phase = 1;
     if (emit) {
-        const bool keep = mpa::keep_partner<NW, PAULI>(t, tr.cutoff, tr.cutoff_type);
+        const bool keep = keep_cut<NW, PAULI>(t, tr);
         emit = keep || (tr.check_upper && gd.abs_s * abs_c >= tr.upper);
         if (emit) phase = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
     }
@@ -1040,7 +1064,7 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
             if (id[q] < 0x8000u) {
 #pragma unroll
                 for (int k = 0; k < NW; ++k) m[q][k] ^= gd.g[k];
-                e = mpa::keep_partner<NW, PAULI>(m[q], tr.cutoff, tr.cutoff_type);
+                e = keep_cut<NW, PAULI>(m[q], tr);
             }
             const uint32_t i = id[q] & 0x7fffu;
             if (e) atomicOr(reinterpret_cast<unsigned long long *>(&sem[i >> 6]), uint64_t{1} << (i & 63));
@@ -1837,6 +1861,21 @@ struct mpa_engine {
     DevArray AL{&stream}, lead{&stream}, foll{&stream}, blk_a{&stream}, blk_b{&stream}, rec{&stream}, lv{&stream},
         match{&stream}, miss{&stream}, ins{&stream}, table{&stream}, counters{&stream}, stats{&stream},
         gdesc{&stream}, fold{&stream}, occ_dev{&stream}, partial{&stream};
+    // Cutoff basis (basis_change): 64 * nw rows of nw words, row s the image of slot s; empty: native.
+    std::vector<uint64_t> bchg_host;
+    DevArray bchg_dev{&stream};
+
+    // Upload bchg_host and point the truncation at it (or at nothing for the native basis).
+    void upload_basis() {
+        if (bchg_host.empty()) {
+            tr.bchg = nullptr;
+            return;
+        }
+        bchg_dev.reserve(bchg_host.size() * 8);
+        HIP_TRY(hipMemcpyAsync(bchg_dev.p, bchg_host.data(), bchg_host.size() * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        tr.bchg = bchg_dev.as<uint64_t>();
+    }
 
     // Graph (build_graph): per-layer cos-only rows and rotation records, the coefficient vector the
     // seeded build evolves, the operator coefficients the graph starts from, and replay scratch.
@@ -3055,6 +3094,7 @@ std::unique_ptr<mpa_engine> new_engine(int basis, int num_modes, uint32_t cutoff
     e->ncols = basis == MPA_BASIS_PAULI ? 2 * num_modes : 2 * num_modes + 1;
     set_trunc(e.get(), cutoff, cutoff_type, lower_atol, upper_atol);
     e->tr.only_rot = -1;
+    e->tr.bchg = nullptr;
     e->device = device;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
@@ -3092,6 +3132,8 @@ std::unique_ptr<mpa_engine> clone_config(const mpa_engine *e) {
     o->occ_dev.reserve(mpa::kMaxWords * 8);
     HIP_TRY(hipMemcpyAsync(o->occ_dev.p, o->occ.data(), mpa::kMaxWords * 8, hipMemcpyHostToDevice, o->stream));
     HIP_TRY(hipStreamSynchronize(o->stream));
+    o->bchg_host = e->bchg_host;  // the copied truncation still points at e's table until this
+    o->upload_basis();
     return o;
 }
 
@@ -3237,6 +3279,37 @@ void require_single(const mpa_engine *e) {
 }
 void require_batch(const mpa_engine *e) {
     if (!e->nbatch) throw ArgError("this engine is not a batch (create it with mpa_batch_create)");
+}
+
+// The cutoff basis of a lone Majorana propagator (update_basis_change, MonomialPropagator.h:204-211,
+// validated as MonomialPropagator.inl:461-495): exactly 2n rows, every slot below 2n; the whole table
+// is checked before anything changes.  n_rows < 0 restores the native basis.
+void set_basis_change(mpa_engine *e, const int64_t *offsets, const int32_t *slots, int64_t n_rows) {
+    require_single(e);
+    if (n_rows < 0) {
+        e->bchg_host.clear();
+        e->upload_basis();
+        return;
+    }
+    if (e->basis == MPA_BASIS_PAULI)
+        throw ArgError("Pauli basis does not accept a basis_change (the encoding is already the Jordan-Wigner image).");
+    const int64_t want = 2 * static_cast<int64_t>(e->n_modes);
+    if (n_rows != want)
+        throw ArgError("basis_change must have exactly 2*logical_num_modes (" + std::to_string(want) + ") rows; got " +
+                       std::to_string(n_rows) + ".");
+    std::vector<uint64_t> tab(static_cast<size_t>(64) * e->nw * e->nw, 0);
+    for (int64_t s = 0; s < n_rows; ++s) {
+        if (offsets[s + 1] < offsets[s]) throw ArgError("basis_change offsets must be non-decreasing");
+        for (int64_t k = offsets[s]; k < offsets[s + 1]; ++k) {
+            const int32_t j = slots[k];
+            if (j < 0 || j >= want)
+                throw RunError("Majorana/Pauli index " + std::to_string(j) + " is out of range; must be less than " +
+                               std::to_string(want) + ".");
+            tab[static_cast<size_t>(s) * e->nw + (j >> 6)] |= uint64_t{1} << (j & 63);  // a repeated index sets once
+        }
+    }
+    e->bchg_host.swap(tab);
+    e->upload_basis();
 }
 
 }  // namespace
@@ -3393,6 +3466,10 @@ int mpa_destroy(mpa_engine *e) {
 
 int mpa_set_truncation(mpa_engine *e, uint32_t cutoff, int cutoff_type, double lower_atol, double upper_atol) {
     return guarded([&] { set_trunc(e, cutoff, cutoff_type, lower_atol, upper_atol); });
+}
+
+int mpa_set_basis_change(mpa_engine *e, const int64_t *offsets, const int32_t *slots, int64_t n_rows) {
+    return guarded([&] { set_basis_change(e, offsets, slots, n_rows); });
 }
 
 int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,

@@ -46,6 +46,7 @@ def _lib():
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_update_initial_operator.restype = C.c_int
         lib.orc_update_initial_operator.argtypes = [P, i64p, i32p, f64p, f64p, C.c_int64]
+        lib.orc_set_basis_change.argtypes = [P, i64p, i32p, C.c_int64]
         lib.orc_destroy.argtypes = [P]
         lib.orc_size.restype = C.c_int64
         lib.orc_size.argtypes = [P]
@@ -120,6 +121,7 @@ class Oracle:
         upper_atol: float | None = None,
         initial_state: Sequence[int] = (),
         schrodinger_cutoff: int | None = None,
+        basis_change=None,
     ) -> None:
         lib = _lib()
         keys = list(initial_operator.keys())
@@ -140,6 +142,8 @@ class Oracle:
             self._h = lib.orc_create_schrodinger(*head, int(schrodinger_cutoff), *tail)
         if not self._h:
             raise RuntimeError(lib.orc_last_error().decode())
+        if basis_change is not None:
+            self.set_basis_change(basis_change)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -238,6 +242,16 @@ class Oracle:
         im = np.asarray([v.imag for v in vals] or [0.0], dtype=np.float64)
         rc = _lib().orc_update_initial_operator(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32),
                                                 _ptr(re, C.c_double), _ptr(im, C.c_double), len(keys))
+        if rc != 0:
+            raise RuntimeError(_lib().orc_last_error().decode())
+
+    def set_basis_change(self, rows) -> None:
+        """Cutoff basis: 2 * num_modes slot lists (row s the image of slot s), or None for native."""
+        if rows is None:
+            rc = _lib().orc_set_basis_change(self._h, None, None, -1)
+        else:
+            offs, slots = _csr([tuple(r) for r in rows])
+            rc = _lib().orc_set_basis_change(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(rows))
         if rc != 0:
             raise RuntimeError(_lib().orc_last_error().decode())
 

@@ -20,7 +20,8 @@
 // (pauli_build_layer_tests.cpp:126-259 method), exhaustive Pauli algebra on all 4^n strings,
 // the reference's closed-form truncation table (tests/test_coeff_trunc.py:59-99, 142-227),
 // the reference's msgpack fixtures with exact energies (tests/test_infinite_cutoff.py:42-70),
-// the native-Pauli vs Jordan-Wigner-Majorana equivalence (pauli_build_layer_tests.cpp:325-390),
+// the native-Pauli vs Jordan-Wigner-Majorana equivalence (pauli_build_layer_tests.cpp:325-390;
+// checked through the Jordan-Wigner cutoff basis, gate by gate),
 // and central finite differences / parameter shift for the gradient.
 
 #include <algorithm>
@@ -89,6 +90,7 @@ struct Engine {
     int cutoff_type;  // 0 length, 1 support
     bool has_lower, has_upper;
     double lower_atol, upper_atol;
+    std::vector<uint64_t> bchg;  // cutoff basis: 2n rows of nw words (row s = image of slot s); empty: native
     double core = 0.0;
     std::vector<uint64_t> rows;  // nw words per row, append-only
     std::vector<double> coeffs;  // current-picture coefficients (MPOperator::op_coeffs)
@@ -181,7 +183,22 @@ struct Engine {
     // Structural keep test on the partner (AlgebraCommon.h:285-342): keep iff fully paired
     // (xor_sum == 0) or the measure <= cutoff.  Pauli: xor_sum = |x-plane|, support = |x|z|
     // (PauliAlgebra.h:17-21).  Majorana: per-mode pair sums over even/odd slots.
+    // With a cutoff basis (MonomialPropagator.inl:484-495, MonomialPropagatorCommon.h:44-63) the test
+    // runs on change_basis(t) (MajoranaAlgebra.h:178-189): the XOR of the images of t's slots.
     bool keep(const uint64_t *t) const {
+        if (!bchg.empty()) {
+            uint64_t mapped[kMaxWords] = {};
+            for (int w = 0; w < nw; ++w)
+                for (uint64_t b = t[w]; b; b &= b - 1) {
+                    const size_t s = static_cast<size_t>(w) * 64 + static_cast<size_t>(__builtin_ctzll(b));
+                    for (int k = 0; k < nw; ++k) mapped[k] ^= bchg[s * nw + k];
+                }
+            return keep_native(mapped);
+        }
+        return keep_native(t);
+    }
+
+    bool keep_native(const uint64_t *t) const {
         size_t xs = 0, measure = 0;
         if (basis == 1) {
             for (int l = 0; l < L; ++l) {
@@ -840,6 +857,28 @@ int orc_update_initial_operator(void *h, const int64_t *offsets, const int32_t *
     }
     e->coeffs = op;
     e->init_coeffs = op;
+    return 0;
+}
+
+// update_basis_change (MonomialPropagator.h:204-211): Majorana only, exactly 2n rows, slots below 2n
+// (MonomialPropagator.inl:461-495, AlgebraCommon.h:56-65); n_rows < 0 restores the native basis.
+int orc_set_basis_change(void *h, const int64_t *offsets, const int32_t *slots, int64_t n_rows) {
+    auto *e = static_cast<Engine *>(h);
+    if (n_rows < 0) {
+        e->bchg.clear();
+        return 0;
+    }
+    if (e->basis == 1) return fail("Pauli basis does not accept a basis_change");
+    const int64_t want = 2 * static_cast<int64_t>(e->n_modes);
+    if (n_rows != want) return fail("basis_change must have exactly 2*logical_num_modes rows");
+    std::vector<uint64_t> tab(static_cast<size_t>(want) * e->nw, 0);
+    for (int64_t r = 0; r < n_rows; ++r)
+        for (int64_t k = offsets[r]; k < offsets[r + 1]; ++k) {
+            const int32_t j = slots[k];
+            if (j < 0 || j >= want) return fail("index out of range");
+            tab[static_cast<size_t>(r) * e->nw + (j >> 6)] |= uint64_t{1} << (j & 63);
+        }
+    e->bchg.swap(tab);
     return 0;
 }
 
