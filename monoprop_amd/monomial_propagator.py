@@ -16,6 +16,7 @@ from ._engine import dispatch
 from .circuit import Circuit, ExpGate, expand_monomials, validate_parameter_mapping
 from .majorana import MajoranaOperator
 from .pauli import PauliOperator
+from .utils import validate_basis_change
 
 logger = logging.getLogger(__name__)
 
@@ -47,8 +48,7 @@ class MonomialPropagator(ABC, Generic[T_op]):
         num_modes = majorana_operator.num_modes
         logger.debug("_init_simulator. num_modes=%d, cutoff=%d, schrodinger_cutoff=%s", num_modes, cutoff,
                      schrodinger_cutoff)
-        if basis_change is not None and len(basis_change) != 2 * num_modes:
-            raise ValueError(f"Basis change must have length {2 * num_modes}, but got {len(basis_change)}.")
+        validate_basis_change(basis_change, num_modes)
         self._comm = comm
         self._n_params = 0
         self._system_size = num_modes

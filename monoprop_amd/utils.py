@@ -12,6 +12,13 @@ def validate_system_size(size: int, *, argument_name: str) -> int:
     return int(size)
 
 
+def validate_basis_change(basis_change, num_modes: int) -> None:
+    """Accept None or a table of exactly ``2 * num_modes`` rows (reference ``utils.py:60-72``); the
+    engine checks the slots themselves when the table is set."""
+    if basis_change is not None and len(basis_change) != 2 * num_modes:
+        raise ValueError(f"Basis change must have length {2 * num_modes}, but got {len(basis_change)}.")
+
+
 def is_antihermitian(indices) -> bool:
     """Whether the Majorana product over ``indices`` is anti-Hermitian: (m_a1 ... m_aL)^dagger equals
     (-1)^C(L,2) times the product, so it is anti-Hermitian iff C(L,2) = L(L-1)/2 is odd, i.e. iff L/2
