@@ -507,10 +507,33 @@ class DeviceSimulator:
     def cutoff(self) -> int:
         return self._cutoff
 
+    def _set_truncation_field(self, name: str, value) -> None:
+        """Assign one truncation field and push it; on any error the old value stays in place."""
+        old = getattr(self, name)
+        setattr(self, name, value)
+        try:
+            self._push_truncation()
+        except Exception:
+            setattr(self, name, old)
+            raise
+
+    @staticmethod
+    def _optional_float(v, name: str):
+        """None or a real number, as the binding's std::optional<double> conversion accepts."""
+        if v is None:
+            return None
+        if isinstance(v, bool) or not isinstance(v, (int, float, np.integer, np.floating)):
+            raise TypeError(f"{name} must be a float or None (not {type(v).__name__}).")
+        return float(v)
+
     @cutoff.setter
     def cutoff(self, v: int) -> None:
-        self._cutoff = int(v)
-        self._push_truncation()
+        # an unsigned int in the binding: bools, floats and strings are type errors, negatives too
+        if isinstance(v, bool) or not isinstance(v, (int, np.integer)):
+            raise TypeError(f"cutoff must be a non-negative integer (not {type(v).__name__}).")
+        if v < 0 or v >= 2**32:
+            raise TypeError(f"cutoff must be a non-negative 32-bit integer; got {v}.")
+        self._set_truncation_field("_cutoff", int(v))
 
     @property
     def lower_atol(self):
@@ -518,8 +541,12 @@ class DeviceSimulator:
 
     @lower_atol.setter
     def lower_atol(self, v) -> None:
-        self._lower = v
-        self._push_truncation()
+        # MonomialPropagator.h:168-176
+        v = self._optional_float(v, "lower_atol")
+        if v is not None and self._upper is not None and v > self._upper:
+            raise RuntimeError(f"New lower_atol ({v!r}) must be less than or equal to current upper_atol "
+                               f"({self._upper!r}).")
+        self._set_truncation_field("_lower", v)
 
     @property
     def upper_atol(self):
@@ -527,8 +554,12 @@ class DeviceSimulator:
 
     @upper_atol.setter
     def upper_atol(self, v) -> None:
-        self._upper = v
-        self._push_truncation()
+        # MonomialPropagator.h:178-186
+        v = self._optional_float(v, "upper_atol")
+        if v is not None and self._lower is not None and v < self._lower:
+            raise RuntimeError(f"New upper_atol ({v!r}) must be greater than or equal to current lower_atol "
+                               f"({self._lower!r}).")
+        self._set_truncation_field("_upper", v)
 
     @property
     def cutoff_type(self) -> str:
@@ -536,6 +567,8 @@ class DeviceSimulator:
 
     @cutoff_type.setter
     def cutoff_type(self, v: str) -> None:
+        if not isinstance(v, str):
+            raise TypeError(f"cutoff_type must be 'length' or 'support' (not {type(v).__name__}).")
         if v not in ("length", "support"):
             raise ValueError(f"Unknown cutoff type {v!r}; expected 'length' or 'support'.")
         old = self._cutoff_type
