@@ -3008,6 +3008,16 @@ struct RunGradient {
 // Layer arrays (authoring order) -> per-gate descriptors in processing order (Heisenberg: back to
 // front; Schrodinger: front to back, MonomialPropagator.inl:720-733).  alpha = (1.0 * theta[map]) * g (MPFunctions.cpp:57-69);
 // without params (structural graph build) alpha = 0.  `fused`: the contract-immediately cos sweep.
+// validate_parameters_length (Validation.cpp:79-91) for a lone propagator's propagate and seeded
+// build_graph; the batch entry points resume with a suffix of the layers and whole parameter rows.
+void check_params_length(const int64_t *param_map, int64_t n_layers, const double *params, int64_t n_params) {
+    if (params == nullptr || n_layers <= 0) return;
+    const int64_t need = *std::max_element(param_map, param_map + n_layers) + 1;
+    if (n_params != need)
+        throw RunError("The length of parameters (" + std::to_string(n_params) +
+                       ") must be the same as max(parameter_mapping)+1 (" + std::to_string(need) + ").");
+}
+
 void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
                 const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
                 bool fused_allowed, std::vector<GateDesc> &gds, std::vector<int32_t> &folds) {
@@ -3483,6 +3493,7 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
             throw RunError("Cannot propagate() on top of a non-empty graph of " + std::to_string(e->layers.size()) +
                            " layer(s): propagate() evolves and contracts in place and assumes no stored graph.");
         if (n_layers == 0) return;
+        check_params_length(param_map, n_layers, params, n_params);
         std::vector<GateDesc> gds;
         std::vector<int32_t> folds;
         make_gates(e, gen_offsets, gen_slots, n_layers, param_map, gen_coeffs, params, n_params,
@@ -3790,6 +3801,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
     return guarded([&] {
         require_single(e);
         if (n_layers == 0) return;
+        check_params_length(param_map, n_layers, params, n_params);
         std::vector<GateDesc> gds;
         std::vector<int32_t> folds;
         make_gates(e, gen_offsets, gen_slots, n_layers, param_map, gen_coeffs, params, n_params, false, gds, folds);
