@@ -6,11 +6,12 @@ API, backed by hand-written HIP kernels behind a C ABI (``include/monoprop_amd.h
 
 The module attributes of the reference's compiled core (``src/monoprop/__init__.py:21-29``,
 ``bindings/bindings.cpp.in:111-130``) are provided with this build's values: ``MAX_NUM_MODES`` is the
-engine's storage width, ``has_mpi`` is False (one GPU holds the whole operator), ``__variant__`` names
+reference's limit of 250 logical modes (the engine's rows are 256 modes wide), ``has_mpi`` is False (one GPU holds the whole operator), ``__variant__`` names
 the HIP target.
 """
 
-from ._engine import DeviceSimulator as _DeviceSimulator
+from ._engine import MAX_LOGICAL_NUM_MODES as _MAX_LOGICAL_NUM_MODES
+from .exceptions import NumberOfModesInvalidError
 from .circuit import Circuit, ExpGate, expand_monomials, validate_parameter_mapping
 from .majorana import Majorana, MajoranaOperator
 from .majorana_propagator import MajoranaPropagator
@@ -20,7 +21,7 @@ from .pauli_propagator import PauliPropagator
 from .utils import antihermitian_generator_correction, is_antihermitian, jordan_wigner_basis_change
 
 __version__ = "0.1.0"
-MAX_NUM_MODES = _DeviceSimulator.storage_num_modes
+MAX_NUM_MODES = _MAX_LOGICAL_NUM_MODES
 has_mpi = False
 __variant__ = "hip-gfx950"
 __build_type__ = "Release"
@@ -44,6 +45,7 @@ __all__ = [
     "MajoranaOperator",
     "MajoranaPropagator",
     "MonomialPropagator",
+    "NumberOfModesInvalidError",
     "Pauli",
     "PauliOperator",
     "PauliPropagator",

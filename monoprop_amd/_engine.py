@@ -11,14 +11,18 @@ falling back to any CPU path.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import json
 import math
 import os
+import weakref
 from collections.abc import Mapping, Sequence
 from pathlib import Path
 
 import numpy as np
+
+from .exceptions import NumberOfModesInvalidError
 
 _LIB_PATH = Path(__file__).resolve().parent / "libmonoprop_amd.so"
 _LIB = None
@@ -81,6 +85,36 @@ _SIGNATURES = {
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 
+# Engines alive in this process.  At exit they are destroyed by an atexit hook, which Python runs
+# before the C runtime's exit handlers (where the HIP runtime and any profiler attached to it shut
+# down); a destructor that runs after the hook, during interpreter teardown, makes no HIP call.
+_LIVE: weakref.WeakSet = weakref.WeakSet()
+_SHUTDOWN = False
+
+
+def _destroy_live_engines() -> None:
+    global _SHUTDOWN
+    for obj in list(_LIVE):
+        obj._destroy()
+    _SHUTDOWN = True
+
+
+class _EngineHandle:
+    """Owner of one engine handle (``self._h``), destroyed once: by ``__del__`` or at process exit."""
+
+    def _own(self, h) -> None:
+        self._h = h
+        _LIVE.add(self)
+
+    def _destroy(self) -> None:
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h and _LIB is not None and not _SHUTDOWN:
+            _LIB.mpa_destroy(h)
+
+    def __del__(self):
+        self._destroy()
+
 
 def library_path() -> Path:
     return _LIB_PATH
@@ -100,6 +134,7 @@ def load_library():
             fn.restype = res
             fn.argtypes = args
         _LIB = lib
+        atexit.register(_destroy_live_engines)
     return _LIB
 
 
@@ -169,7 +204,7 @@ def default_device() -> int:
     return int(os.environ.get("MONOPROP_AMD_DEVICE", "0"))
 
 
-class DeviceSimulator:
+class DeviceSimulator(_EngineHandle):
     """HIP-engine counterpart of the reference's compiled ``MonomialPropagatorNNN``."""
 
     storage_num_modes = 256
@@ -223,7 +258,7 @@ class DeviceSimulator:
         else:
             _check(lib.mpa_create_schrodinger(*head, int(schrodinger_cutoff), *tail))
         self._schrodinger = schrodinger_cutoff is not None
-        self._h = h
+        self._own(h)
         self._basis_change = None
         if basis_change is not None:
             self.basis_change = basis_change  # validated by the engine; __del__ frees it on error
@@ -232,12 +267,6 @@ class DeviceSimulator:
         self._layer_gates: list[int] = []  # the gate of each graph layer, optimizer order
         self._n_gates = 0
         self._operator_epoch = 0  # update_initial_operator() calls; functionals created before go stale
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h and _LIB is not None:
-            _LIB.mpa_destroy(h)
-            self._h = None
 
     def __deepcopy__(self, memo):
         """An independent engine with the same state (the reference's __deepcopy__, binder.h:112-116)."""
@@ -249,7 +278,7 @@ class DeviceSimulator:
         for k, v in self.__dict__.items():
             if k != "_h":
                 setattr(other, k, copy.deepcopy(v, memo))
-        other._h = h
+        other._own(h)
         return other
 
     def __copy__(self):
@@ -296,10 +325,7 @@ class DeviceSimulator:
                 "evolves and contracts in place and assumes no stored graph."
             )
         offs, slots, pm, gc = self._layers(majoranas, parameter_mapping, gen_coeffs)
-        params = list(parameters)
-        need = (max(int(p) for p in parameter_mapping) + 1) if len(parameter_mapping) else 0
-        if len(params) < need:
-            raise ValueError(f"Expected at least {need} parameter value(s), got {len(params)}.")
+        params = list(parameters)  # a length other than max(mapping) + 1 is the engine's RuntimeError
         th = _f64(params)
         k = -1 if only_rotate_len_k is None else int(only_rotate_len_k)
         _check(load_library().mpa_propagate(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(majoranas),
@@ -636,10 +662,23 @@ class DeviceSimulator:
         _check(load_library().mpa_set_parameter_mapping(self._h, _ptr(a, C.c_int64), len(stored)))
 
 
+# Largest logical mode count the front-end accepts: the reference's monoprop_MAX_NUM_MODES
+# (CMakeLists.txt:45-50).  The engine itself stores up to storage_num_modes (256) modes per row.
+MAX_LOGICAL_NUM_MODES = 250
+
+
 def dispatch(num_modes: int):
-    """Adapter factory for a mode count (reference tools/generate-dispatch.py:145-190)."""
-    if num_modes <= 0 or num_modes > DeviceSimulator.storage_num_modes:
-        raise ValueError(f"num_modes={num_modes} is out of range (1..{DeviceSimulator.storage_num_modes}).")
+    """Adapter factory for a mode count, with the reference's checks and messages
+    (tools/generate-dispatch.py:145-160)."""
+    if not isinstance(num_modes, int) or num_modes <= 0:
+        raise NumberOfModesInvalidError(f"Number of Fermionic modes {num_modes} invalid. num_modes must be > 0.")
+    if num_modes > MAX_LOGICAL_NUM_MODES:
+        raise NumberOfModesInvalidError(
+            f"Number of Fermionic modes {num_modes} invalid."
+            f" num_modes must be <= {MAX_LOGICAL_NUM_MODES}."
+            f" Contact monoprop developers if more than {MAX_LOGICAL_NUM_MODES}"
+            " Fermionic modes are required."
+        )
 
     def make(**kwargs):
         kwargs.setdefault("logical_num_modes", num_modes)
@@ -648,7 +687,7 @@ def dispatch(num_modes: int):
     return make
 
 
-class BatchSimulator:
+class BatchSimulator(_EngineHandle):
     """Many independent propagations of one gate sequence in one device store.
 
     Member ``q`` is the propagation of ``operators[q]`` at parameter row ``q``; all members share the
@@ -690,7 +729,7 @@ class BatchSimulator:
                                              _ptr(words, C.c_uint64), _ptr(coeffs, C.c_double),
                                              _ptr(members, C.c_int32), _ptr(st, C.c_int32), len(list(initial_state)),
                                              default_device()))
-            self._h = h
+            self._own(h)
             _apply_launch(h)
             return
         keys: list[tuple[int, ...]] = []
@@ -710,14 +749,8 @@ class BatchSimulator:
                                     _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), _ptr(re, C.c_double),
                                     _ptr(im, C.c_double), _ptr(st, C.c_int32), len(list(initial_state)),
                                     default_device()))
-        self._h = h
+        self._own(h)
         _apply_launch(h)
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h and _LIB is not None:
-            _LIB.mpa_destroy(h)
-            self._h = None
 
     @property
     def members(self) -> int:
@@ -754,7 +787,7 @@ class BatchSimulator:
         other = BatchSimulator.__new__(BatchSimulator)
         other._basis, other._num_modes = self._basis, self._num_modes
         other._members = int(mv.sum())
-        other._h = h
+        other._own(h)
         self._members -= other._members
         return other
 

@@ -38,4 +38,7 @@ def expectation(prop, theta: torch.Tensor) -> torch.Tensor:
         raise RuntimeError("build_graph must be called before a differentiable expectation value")
     if theta.dim() != 1 or theta.numel() != prop.n_parameters:
         raise ValueError(f"theta must be a vector of {prop.n_parameters} parameters, got shape {tuple(theta.shape)}")
+    if not (torch.is_grad_enabled() and theta.requires_grad):  # no backward can follow: the replay alone
+        params = theta.detach().to("cpu", torch.float64).numpy()
+        return torch.tensor(prop.expectation_value(params), dtype=theta.dtype, device=theta.device)
     return _GraphExpectation.apply(theta, prop)

@@ -51,6 +51,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kScanBlock = 1024;
 constexpr int kExpectBlocks = 1024;
 constexpr uint64_t kMaxSchrodingerRows = uint64_t{1} << 28;  // initial Schrodinger state terms
+constexpr double kIdxSlotsPerRow = 1.6;  // row index slots per row of store capacity
 constexpr int kMemFields = 11;  // mpa_memory_breakdown entries
 
 // counters[] on the device
@@ -76,6 +77,12 @@ const char *const kKindNames[K_KINDS] = {"k_scan", "k_scan_sums", "k_expand", "k
                                          "k_advance", "k_sweep", "k_resolve"};
 
 thread_local std::string g_error;
+
+// sin and cos of one angle from one glibc sincos call, on the host.  The reference evaluates std::sin
+// and std::cos of the same angle side by side (FusedApply.h:47-48, Evolution.cpp:475-476, Scan.h:44
+// next to Engine.h:552), which GCC, the compiler of its Linux builds, turns into a single sincos
+// call; glibc's sincos can differ from its sin or cos in the last bit, so every angle goes through it.
+void sin_cos(double x, double *s, double *c) { ::sincos(x, s, c); }
 
 int set_error(int code, const std::string &msg) {
     g_error = msg;
@@ -704,64 +711,71 @@ __global__ void k_advance(uint64_t *__restrict__ counters, int64_t *__restrict__
 // Fused contract-immediately pipeline (the default propagate: no orbital cap, cos != 0).
 //
 // A global open-addressing index over every stored row answers "is M^G stored?" for the emitting
-// terms only.  Slot = NW + 1 words: tag (member << 48) | (fingerprint << 32) | (row + 1), 0 = empty,
-// then the key itself, so a probe reads one slot and needs no second access to the store.  `member`
-// is the batch member owning the row (0 for a lone propagator): it enters the slot position and the
-// tag, so equal monomials of different members are different keys.  The tag is claimed first and the
-// key written after; lookups skip rows appended during the current gate (they are nobody's partner),
-// so a key still being written is never compared.
+// terms only.  It holds no keys: a slot is one u64, (fingerprint << 32) | (row + 1), 0 = empty, and a
+// probe whose fingerprint matches confirms against the row itself (the store is the key array), with
+// the row's coefficient loaded in the same round trip.  `member` is the batch member owning the row
+// (0 for a lone propagator): it is mixed into the hash and compared with the row's member, so equal
+// monomials of different members are different keys.  The home slot of hash h is the high word of
+// h * gslots (any slot count works), the fingerprint h's low word; probes run forward from the home
+// slot and never wrap (kIdxPad slots past the last home slot absorb the runs; a run past them is an
+// index overflow, reported, never silent).  A slot is claimed whole by one CAS, so a reader never sees
+// half an entry; lookups skip rows appended during the current gate (they are nobody's partner), so no
+// row still being written is compared.
+constexpr uint64_t kIdxPad = 1024;
+constexpr int kIdxGroup = 4;  // slots read per probe round trip
 
 template <int NW>
 __device__ __forceinline__ uint64_t gidx_hash(const uint64_t *key, uint32_t member) {
     return mpa::hash_key<NW>(key) ^ mpa::mix64(static_cast<uint64_t>(member) * 0x9e3779b97f4a7c15ull);
 }
 
-__device__ __forceinline__ uint64_t gidx_tag_hi(uint64_t h, uint32_t member) {
-    return (static_cast<uint64_t>(member) << 48) | (h & 0x0000ffff00000000ull);
-}
+__device__ __forceinline__ uint64_t gidx_home(uint64_t h, uint64_t gslots) { return __umul64hi(h, gslots); }
 
 template <int NW>
-__device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_t gmask, const uint64_t *key,
+__device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_t gslots, const uint64_t *key,
                                             uint32_t row, uint32_t member) {
-    constexpr int SW = NW + 1;
     const uint64_t h = gidx_hash<NW>(key, member);
-    const uint64_t val = gidx_tag_hi(h, member) | (static_cast<uint64_t>(row) + 1);
-    uint64_t pos = h & gmask;
-    for (uint64_t probe = 0; probe <= gmask; ++probe) {
-        uint64_t *slot = gidx + pos * SW;
-        const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(slot), 0ull, val);
-        if (old == 0ull) {
-#pragma unroll
-            for (int w = 0; w < NW; ++w) slot[1 + w] = key[w];
-            return true;
-        }
-        pos = (pos + 1) & gmask;
+    const uint64_t val = (h << 32) | (static_cast<uint64_t>(row) + 1);
+    const uint64_t end = gslots + kIdxPad;
+    for (uint64_t pos = gidx_home(h, gslots); pos < end; ++pos) {
+        if (__hip_atomic_load(gidx + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
+        if (atomicCAS(reinterpret_cast<unsigned long long *>(gidx + pos), 0ull, val) == 0ull) return true;
     }
     return false;
 }
 
-// Row of (`key`, `member`) among rows [0, limit), or -1.
+// Row of (`key`, `member`) among rows [0, limit), or -1; *val receives the row's coefficient.
+// member_of null: every row is member 0.
 template <int NW>
-__device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gmask,
-                                                   const uint64_t *key, uint64_t limit, uint32_t member) {
-    constexpr int SW = NW + 1;
+__device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gslots,
+                                                   const uint64_t *__restrict__ rows,
+                                                   const uint16_t *__restrict__ member_of,
+                                                   const double *__restrict__ coef, const uint64_t *key,
+                                                   uint64_t limit, uint32_t member, double *val) {
     const uint64_t h = gidx_hash<NW>(key, member);
-    const uint64_t fp = gidx_tag_hi(h, member);
-    uint64_t pos = h & gmask;
-    for (uint64_t probe = 0; probe <= gmask; ++probe) {
-        const uint64_t *slot = gidx + pos * SW;
-        uint64_t e[SW];
+    const uint32_t fp = static_cast<uint32_t>(h);
+    const uint64_t end = gslots + kIdxPad;
+    for (uint64_t pos = gidx_home(h, gslots); pos < end; pos += kIdxGroup) {
+        uint64_t e[kIdxGroup];
 #pragma unroll
-        for (int w = 0; w < SW; ++w) e[w] = slot[w];
-        if (e[0] == 0) return -1;
-        const uint64_t r = (e[0] & 0xffffffffull) - 1;
-        if ((e[0] & 0xffffffff00000000ull) == fp && r < limit) {
-            bool eq = true;
+        for (int i = 0; i < kIdxGroup; ++i) e[i] = gidx[pos + i];  // the array has kIdxGroup slots past end
 #pragma unroll
-            for (int w = 0; w < NW; ++w) eq = eq && (e[1 + w] == key[w]);
-            if (eq) return static_cast<int64_t>(r);
+        for (int i = 0; i < kIdxGroup; ++i) {
+            if (e[i] == 0 || pos + i >= end) return -1;
+            const uint64_t r = (e[i] & 0xffffffffull) - 1;
+            if (static_cast<uint32_t>(e[i] >> 32) != fp || r >= limit) continue;
+            uint64_t m[NW];
+            load_row<NW>(rows, static_cast<uint32_t>(r), m);
+            const double c = coef[r];
+            const uint32_t q = member_of ? member_of[r] : 0u;
+            bool eq = q == member;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) eq = eq && (m[w] == key[w]);
+            if (eq) {
+                *val = c;
+                return static_cast<int64_t>(r);
+            }
         }
-        pos = (pos + 1) & gmask;
     }
     return -1;
 }
@@ -770,14 +784,14 @@ __device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_gidx_build(const uint64_t *__restrict__ rows, uint64_t start,
                                                        uint64_t count, const uint16_t *__restrict__ member,
-                                                       uint64_t *__restrict__ gidx, uint64_t gmask,
+                                                       uint64_t *__restrict__ gidx, uint64_t gslots,
                                                        uint64_t *__restrict__ counters) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= count) return;
     uint64_t m[NW];
     load_row<NW>(rows, static_cast<uint32_t>(start + i), m);
     const uint32_t q = member ? member[start + i] : 0u;
-    if (!gidx_insert<NW>(gidx, gmask, m, static_cast<uint32_t>(start + i), q))
+    if (!gidx_insert<NW>(gidx, gslots, m, static_cast<uint32_t>(start + i), q))
         atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), 2ull);
 }
 
@@ -1172,9 +1186,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                                                     const uint64_t *__restrict__ gsum, uint64_t *__restrict__ gsum_next,
                                                     uint64_t n_groups_next, const double *__restrict__ t_val,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
-                                                    uint64_t gmask, uint64_t *__restrict__ desc_m,
+                                                    uint64_t gslots, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
-                                                    uint64_t cap_rows, uint64_t gidx_slots, uint32_t max_groups,
+                                                    uint64_t cap_rows, uint32_t max_groups,
                                                     int group_log2, uint16_t *__restrict__ member,
                                                     const Trig *__restrict__ gtrig,
                                                     const uint64_t *__restrict__ occ_dev) {
@@ -1325,7 +1339,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                 load_row<NW>(rows, S, m);
 #pragma unroll
                 for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
-                const int64_t T = gidx_find_below<NW>(gidx, gmask, t, N, q);
+                double wT = 0.0;
+                const int64_t T = gidx_find_below<NW>(gidx, gslots, rows, BATCH ? member : nullptr, coef, t, N, q, &wT);
                 const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
                 if (!skip) {
                     const double gk = BATCH ? gtrig[q].k : gd.k;
@@ -1334,7 +1349,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                     const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
                     const double wS = vS * gk;
                     if (T >= 0) {
-                        const double wT = coef[T];
                         const double vTu = wT * ginv;
                         coef[S] = wS + gs * static_cast<double>(-ph) * vTu;
                         coef[T] = wT + gs * static_cast<double>(ph) * vS;
@@ -1373,7 +1387,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                         const uint64_t I = excl + pend_miss, n = N + I;
                         st[3] = static_cast<int64_t>(I);
                         counters[C_NB + nxt] = n;
-                        if (2 * n > cap_rows || 2 * n > gidx_slots) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
+                        if (2 * n > cap_rows) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
                     }
                 }
             }
@@ -1394,7 +1408,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                 coef[dest] = st_ins[pb][threadIdx.x];
                 if (BATCH) member[dest] = static_cast<uint16_t>(q);
                 gather_column_bits<NW, PAULI>(lcol, ncols, n_modes, t, (dest >> 6) - wlo, uint64_t{1} << (dest & 63));
-                if (!gidx_insert<NW>(gidx, gmask, t, static_cast<uint32_t>(dest), q))
+                if (!gidx_insert<NW>(gidx, gslots, t, static_cast<uint32_t>(dest), q))
                     atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
             }
             __syncthreads();
@@ -1425,30 +1439,102 @@ __global__ void k_advance_fused(uint64_t *__restrict__ counters, int64_t *__rest
 }
 
 
-// <O> partials: sum over fully paired rows of phase * c, per block in a fixed order.
-template <int NW, int PAULI>
-__global__ __launch_bounds__(kBlock) void k_expect(const uint64_t *__restrict__ rows,
-                                                   const double *__restrict__ coef, uint64_t N,
-                                                   const uint64_t *__restrict__ occ_dev,
-                                                   double *__restrict__ partial) {
-    __shared__ double red[kBlock];
+// Ordered sums.  The reference adds the terms of <O> one after another in ascending row order
+// (EvalState::dot and inner_product, MPFunctions.cpp:86-94,135-148): phase * c over the fully paired
+// rows, or state * op over every row.  That sum is reproduced to the bit: each block owns a contiguous
+// segment of rows and compacts its nonzero terms in row order (k_terms, a counting pass then a writing
+// pass at the scanned block offsets), then one thread adds them in that order (k_seq_sum).  A term
+// that is exactly zero leaves a running sum that starts at +0.0 unchanged (it cannot turn it into
+// -0.0), so leaving such terms out changes no bit.  With `member` set (a batch), every term carries its
+// row's member and each member's terms are added in row order, which is the order of that member's
+// own store.
+enum : int { TERM_PAIRED = 0, TERM_DOT = 1 };
+
+template <int NW, int PAULI, int MODE, int EMIT>
+__global__ __launch_bounds__(kBlock) void k_terms(const uint64_t *__restrict__ rows, const double *__restrict__ a,
+                                                  const double *__restrict__ b, const uint16_t *__restrict__ member,
+                                                  uint64_t N, uint64_t seg, const uint64_t *__restrict__ occ_dev,
+                                                  uint32_t *__restrict__ blk, double *__restrict__ out_v,
+                                                  uint16_t *__restrict__ out_m) {
+    __shared__ uint32_t lds[kWaves];
     uint64_t occ[NW];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
-    double acc = 0.0;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N;
-         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
-        uint64_t m[NW];
-        load_row<NW>(rows, static_cast<uint32_t>(i), m);
-        if (mpa::is_paired<NW, PAULI>(m)) acc += mpa::state_phase<NW, PAULI>(m, occ) * coef[i];
+    for (int w = 0; w < NW; ++w) occ[w] = MODE == TERM_PAIRED ? occ_dev[w] : 0;
+    const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * seg;
+    const uint64_t hi = lo + seg < N ? lo + seg : N;
+    uint32_t run = EMIT ? blk[blockIdx.x] : 0u;
+    for (uint64_t base = lo; base < hi; base += kBlock) {
+        const uint64_t i = base + threadIdx.x;
+        double x = 0.0;
+        if (i < hi) {
+            if (MODE == TERM_PAIRED) {
+                uint64_t m[NW];
+                load_row<NW>(rows, static_cast<uint32_t>(i), m);
+                if (mpa::is_paired<NW, PAULI>(m)) x = mpa::state_phase<NW, PAULI>(m, occ) * a[i];
+            } else {
+                x = a[i] * b[i];
+            }
+        }
+        const uint32_t f = x != 0.0 ? 1u : 0u;  // NaN is kept
+        uint32_t tot;
+        const uint32_t r = block_excl_scan(f, lds, &tot);
+        if (EMIT && f) {
+            out_v[run + r] = x;
+            if (member) out_m[run + r] = member[i];
+        }
+        run += tot;
     }
-    red[threadIdx.x] = acc;
+    if (!EMIT && threadIdx.x == 0) blk[blockIdx.x] = run;
+}
+
+// In-place exclusive scan of nb block counts in one block; the total goes to *total.
+__global__ __launch_bounds__(kScanBlock) void k_scan_counts(uint32_t *__restrict__ a, uint32_t nb,
+                                                            uint64_t *__restrict__ total) {
+    __shared__ uint32_t lds[kScanBlock / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < static_cast<unsigned>(s)) red[threadIdx.x] += red[threadIdx.x + s];
+    for (uint32_t base = 0; base < nb; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? a[i] : 0u;
+        uint32_t tot;
+        const uint32_t x = block_excl_scan<kScanBlock / 64>(v, lds, &tot);
+        if (i < nb) a[i] = carry + x;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// Adds the K compacted terms in order: out[0] (no members), or out[q] for member q = global thread q.
+constexpr int kSeqStage = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_seq_sum(const double *__restrict__ v, const uint16_t *__restrict__ m,
+                                                    uint64_t K, uint32_t nb, double *__restrict__ out) {
+    __shared__ double sv[kSeqStage];
+    __shared__ uint16_t sm[kSeqStage];
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    const bool owner = m ? q < nb : q == 0;
+    double r = 0.0;
+    for (uint64_t base = 0; base < K; base += kSeqStage) {
+        const uint32_t n = static_cast<uint32_t>(K - base < kSeqStage ? K - base : kSeqStage);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
+            sv[j] = v[base + j];
+            if (m) sm[j] = m[base + j];
+        }
+        __syncthreads();
+        if (owner) {
+            if (m) {
+                for (uint32_t j = 0; j < n; ++j)
+                    if (sm[j] == q) r += sv[j];
+            } else {
+                for (uint32_t j = 0; j < n; ++j) r += sv[j];
+            }
+        }
+    }
+    if (owner) out[q] = r;
 }
 
 // One graph layer replayed at angle alpha (reference evolve_step, Evolution.cpp:471-512): cos-only
@@ -1561,24 +1647,8 @@ __global__ __launch_bounds__(kBlock) void k_obs_match(const uint64_t *__restrict
     }
 }
 
-// Fixed-order block partials of sum_i a[i] b[i] (the reference's state . op, MPFunctions.cpp:86-94).
-__global__ __launch_bounds__(kBlock) void k_dot(const double *__restrict__ a, const double *__restrict__ b, uint64_t N,
-                                                double *__restrict__ partial) {
-    __shared__ double red[kBlock];
-    double acc = 0.0;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N;
-         i += static_cast<uint64_t>(gridDim.x) * kBlock)
-        acc += a[i] * b[i];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < static_cast<unsigned>(s)) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
+constexpr uint32_t kMaxBatch = 2048;  // members per batch (16-bit member ids; the ordered sum's threads)
 
-// Per-layer sum of the adjoint partials, in block order: one block per layer.
 // Stable partition of a batch store by member: side[member] = 1 moves the row out.  Per 256-row block,
 // the number of moving rows.
 __global__ __launch_bounds__(kBlock) void k_part_count(const uint16_t *__restrict__ member, uint64_t N,
@@ -1621,73 +1691,6 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const uint64_t *__restr
     (f ? mem_mv : mem_st)[dest] = renum[q];
 }
 
-// Batched <O>: per member, the sum over its fully paired rows of phase * c.  Each wavefront keeps one
-// accumulator per member in LDS and walks a fixed set of 64-row windows; inside a window the lanes of
-// one member are summed by a fixed butterfly and added by a single lane, so the result does not depend
-// on scheduling.  partial[block * nb + member] = the block's four wave sums in wave order.
-constexpr uint32_t kMaxBatch = 2048;  // members per batch: kWaves * kMaxBatch doubles of LDS per block
-
-template <int NW, int PAULI>
-__global__ __launch_bounds__(kBlock) void k_expect_batch(const uint64_t *__restrict__ rows,
-                                                         const double *__restrict__ coef,
-                                                         const uint16_t *__restrict__ member, uint64_t N,
-                                                         uint32_t nb, const uint64_t *__restrict__ occ_dev,
-                                                         double *__restrict__ partial) {
-    extern __shared__ double acc[];  // [kWaves][nb]
-    for (uint32_t i = threadIdx.x; i < kWaves * nb; i += kBlock) acc[i] = 0.0;
-    __syncthreads();
-    uint64_t occ[NW];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) occ[w] = occ_dev[w];
-    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
-    double *mine = acc + static_cast<uint64_t>(wave) * nb;
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock + wave * 64; base < N; base += stride) {
-        const uint64_t i = base + lane;
-        bool have = false;
-        double v = 0.0;
-        uint32_t q = 0;
-        if (i < N) {
-            uint64_t m[NW];
-            load_row<NW>(rows, static_cast<uint32_t>(i), m);
-            if (mpa::is_paired<NW, PAULI>(m)) {
-                v = mpa::state_phase<NW, PAULI>(m, occ) * coef[i];
-                q = member[i];
-                have = true;
-            }
-        }
-        uint64_t pending = __ballot(have);
-        while (pending) {
-            const int lead = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-            const uint32_t ql = static_cast<uint32_t>(__shfl(static_cast<int>(q), lead, 64));
-            const bool in = have && q == ql;
-            const uint64_t group = __ballot(in);
-            double x = in ? v : 0.0;
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
-            if (static_cast<int>(lane) == lead) mine[ql] += x;
-            pending &= ~group;
-        }
-    }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < nb; q += kBlock) {
-        double sum = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) sum += acc[static_cast<uint64_t>(w) * nb + q];
-        partial[static_cast<uint64_t>(blockIdx.x) * nb + q] = sum;
-    }
-}
-
-// out[q] = sum over blocks b (in order) of partial[b * nb + q].
-__global__ __launch_bounds__(kBlock) void k_reduce_members(const double *__restrict__ partial, uint32_t nblocks,
-                                                           uint32_t nb, double *__restrict__ out) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= nb) return;
-    double sum = 0.0;
-    for (uint32_t b = 0; b < nblocks; ++b) sum += partial[static_cast<uint64_t>(b) * nb + q];
-    out[q] = sum;
-}
-
 // Stored rows per batch member (counts[nb], zeroed by the caller).
 __global__ __launch_bounds__(kBlock) void k_count_members(const uint16_t *__restrict__ member, uint64_t N, uint32_t nb,
                                                           unsigned long long *__restrict__ counts) {
@@ -1702,6 +1705,7 @@ __global__ __launch_bounds__(kBlock) void k_count_members(const uint16_t *__rest
         if (hist[i]) atomicAdd(&counts[i], static_cast<unsigned long long>(hist[i]));
 }
 
+// Per-layer sum of the adjoint partials, in block order: one block per layer.
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
                                                             const uint64_t *__restrict__ offs,
                                                             double *__restrict__ out) {
@@ -1860,7 +1864,9 @@ struct mpa_engine {
     DevArray rows{&stream}, coef{&stream}, cols{&stream};
     DevArray AL{&stream}, lead{&stream}, foll{&stream}, blk_a{&stream}, blk_b{&stream}, rec{&stream}, lv{&stream},
         match{&stream}, miss{&stream}, ins{&stream}, table{&stream}, counters{&stream}, stats{&stream},
-        gdesc{&stream}, fold{&stream}, occ_dev{&stream}, partial{&stream};
+        gdesc{&stream}, fold{&stream}, occ_dev{&stream};
+    // ordered sums (ordered_sum): block counts + total, compacted terms, their members, results
+    DevArray sum_blk{&stream}, sum_v{&stream}, sum_m{&stream}, sum_out{&stream};
     // Cutoff basis (basis_change): 64 * nw rows of nw words, row s the image of slot s; empty: native.
     std::vector<uint64_t> bchg_host;
     DevArray bchg_dev{&stream};
@@ -1921,7 +1927,7 @@ struct mpa_engine {
     std::vector<double> bcore;
     uint64_t cap_limit = 0;  // fused pipeline: stop before a gate that would grow the store past this (0: none)
     size_t fused_done = 0;   // gates the last fused propagate applied
-    DevArray bmember{&stream}, btrig{&stream}, bpartial{&stream}, bout{&stream};
+    DevArray bmember{&stream}, btrig{&stream}, bout{&stream};
 
     // Schrodinger picture (schrodinger_cutoff set): the store holds the evolved state; the observable's
     // terms (core excluded) are matched against the rows when <O> is evaluated (obs_tab: their
@@ -2031,8 +2037,9 @@ struct mpa_engine {
         }
     }
 
-    // Capacities are powers of two (at least 16384 rows), so every per-row array of a store, and of the
-    // next propagator, recurs at the same sizes and the memory pool hands back cached blocks.
+    // Capacities are multiples of the 16384-row granule that grow by at least half each time, so every
+    // per-row array of a store, and of the next propagator of the same workload, recurs at the same
+    // sizes and the memory pool hands back cached blocks.
     // A fresh store of `c` rows of capacity (a multiple of 64; contents undefined, column index zero).
     void alloc_store(uint64_t c) {
         rows.reserve(c * nw * 8);
@@ -2055,17 +2062,28 @@ struct mpa_engine {
         gidx_rows = static_cast<uint64_t>(-1);
     }
 
-    // Capacity grow_store(need) picks: a power of two, at least 64 * kBlock, need and twice the current.
+    static constexpr uint64_t kGranule = 64ull * kBlock;
+
+    // Capacity grow_store(need) picks: a multiple of the granule, at least need and 1.5 times the current.
     uint64_t grown_capacity(uint64_t need) const {
-        uint64_t nc = 64ull * kBlock;
-        while (nc < need || nc < 2 * cap) nc <<= 1;
-        return nc;
+        const uint64_t want = std::max<uint64_t>({need, cap + cap / 2, kGranule});
+        return (want + kGranule - 1) / kGranule * kGranule;
     }
 
-    // Device bytes per row of capacity: store, column index, fused-pipeline arrays, row index at two
-    // slots per row, per-gate pipeline item list.
+    // Device bytes per row of capacity the fused pipeline holds: rows, coefficient, column index, row
+    // index slots, the emitter regions (t_item + t_val over half the capacity), leader masks.
     double bytes_per_row() const {
-        return 8.0 * nw + 8 + ncols / 8.0 + (nbatch ? 2 : 0) + 6 + 16.0 * (nw + 1) + 4 + 0.25;
+        return 8.0 * nw + 8 + ncols / 8.0 + (nbatch ? 2 : 0) + 8.0 * kIdxSlotsPerRow + 6 + 0.125;
+    }
+
+    // Scratch of the per-gate pipeline, sized by the capacity (the current gate's item list kept).
+    void ensure_gate_arrays() {
+        grow_keep(AL, cap * 4);
+        lead.reserve((cap / 64) * 8);
+        foll.reserve((cap / 64) * 8);
+        const uint64_t nblk = cap / 64 / kBlock + 2;
+        blk_a.reserve(std::max<uint64_t>(nblk, cap / kBlock + 2) * 4);
+        blk_b.reserve(nblk * 4);
     }
 
     void grow_store(uint64_t need) {
@@ -2106,20 +2124,7 @@ struct mpa_engine {
             std::swap(seedc.bytes, s2.bytes);
         }
         cap = nc;
-        // per-term scratch sized by capacity; the item list of the current gate is kept
-        if (AL.bytes < cap * 4) {
-            DevArray a2{&stream};
-            a2.reserve(cap * 4);
-            if (AL.p) HIP_TRY(hipMemcpyAsync(a2.p, AL.p, AL.bytes, hipMemcpyDeviceToDevice, stream));
-            HIP_TRY(hipStreamSynchronize(stream));
-            std::swap(AL.p, a2.p);
-            std::swap(AL.bytes, a2.bytes);
-        }
-        lead.reserve((cap / 64) * 8);
-        foll.reserve((cap / 64) * 8);
-        const uint64_t nblk = cap / 64 / kBlock + 2;
-        blk_a.reserve(std::max<uint64_t>(nblk, cap / kBlock + 2) * 4);
-        blk_b.reserve(nblk * 4);
+        if (AL.p) ensure_gate_arrays();  // the per-gate pipeline is in use on this engine
     }
 
     void ensure_items(uint64_t nL, uint64_t nA) {
@@ -2176,6 +2181,8 @@ struct mpa_engine {
         if (ng) HIP_TRY(hipMemcpyAsync(gdesc.p, gds.data(), ng * sizeof(GateDesc), hipMemcpyHostToDevice, stream));
         if (!folds.empty())
             HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
+        ensure_capacity(std::max<uint64_t>(N, 1));
+        ensure_gate_arrays();
         uint64_t n_ub = N;  // upper bound of the live count at the current gate
         hipEvent_t ev[2];
         HIP_TRY(hipEventCreate(&ev[0]));
@@ -2372,10 +2379,11 @@ struct mpa_engine {
             const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
             if (nc + nr == 0) continue;
             const double alpha = phase * params[ly.param_index] * ly.gen_coeff;
+            double sn, cs;
+            sin_cos(2 * alpha, &sn, &cs);
             const uint32_t blocks = static_cast<uint32_t>((nc + nr + kBlock - 1) / kBlock);
             hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, stream, g_cos.as<uint32_t>(),
-                               g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr, std::cos(2 * alpha),
-                               std::sin(2 * alpha), rep.as<double>(), snapshots ? snap_c.as<double>() : nullptr,
+                               g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr, cs, sn, rep.as<double>(), snapshots ? snap_c.as<double>() : nullptr,
                                snapshots ? snap_r.as<double>() : nullptr);
             HIP_TRY(hipGetLastError());
         }
@@ -2410,9 +2418,11 @@ struct mpa_engine {
             const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
             if (nc + nr == 0) continue;
             const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
+            double sn, cs;
+            sin_cos(2 * alpha, &sn, &cs);
             hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(offs[l + 1] - offs[l])), dim3(kBlock), 0, stream,
                                g_cos.as<uint32_t>(), g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr,
-                               std::cos(2 * alpha), std::sin(2 * alpha), snap_c.as<double>(), snap_r.as<double>(),
+                               cs, sn, snap_c.as<double>(), snap_r.as<double>(),
                                lam.as<double>(), gpartial.as<double>(), offs[l]);
             HIP_TRY(hipGetLastError());
         }
@@ -2495,16 +2505,49 @@ struct mpa_engine {
         return core + r;
     }
 
-    // core + sum_i a[i] b[i] in fixed-order block partials.
-    double dot_core(const double *a, const double *b) {
-        partial.reserve(kExpectBlocks * 8);
-        hipLaunchKernelGGL(k_dot, dim3(kExpectBlocks), dim3(kBlock), 0, stream, a, b, N, partial.as<double>());
+    // The reference's ordered sum over rows [0, N) (k_terms): MODE TERM_PAIRED adds phase * a[i] over the
+    // fully paired rows, TERM_DOT adds a[i] * b[i] over all rows.  out[0], or out[q] per member q of
+    // `member` (nb members).
+    template <int NW, int PAULI, int MODE>
+    void ordered_sum(const double *a, const double *b, const uint16_t *member, uint32_t nb, double *out) {
+        for (uint32_t q = 0; q < nb; ++q) out[q] = 0.0;
+        if (N == 0) return;
+        const uint64_t nblk = std::min<uint64_t>((N + kBlock - 1) / kBlock, 2048);
+        const uint64_t seg = ((N + nblk - 1) / nblk + kBlock - 1) / kBlock * kBlock;
+        const uint32_t grid = static_cast<uint32_t>((N + seg - 1) / seg);
+        sum_blk.reserve((static_cast<size_t>(grid) + 2) * 4 + 8);
+        uint64_t *total = reinterpret_cast<uint64_t *>(sum_blk.as<uint32_t>() + ((grid + 1) & ~1u));
+        const uint64_t *occ_s = occ_dev.as<uint64_t>();
+        hipLaunchKernelGGL((k_terms<NW, PAULI, MODE, 0>), dim3(grid), dim3(kBlock), 0, stream, rows.as<uint64_t>(), a,
+                           b, member, N, seg, occ_s, sum_blk.as<uint32_t>(), static_cast<double *>(nullptr),
+                           static_cast<uint16_t *>(nullptr));
         HIP_TRY(hipGetLastError());
-        std::vector<double> h(kExpectBlocks);
-        HIP_TRY(hipMemcpyAsync(h.data(), partial.p, kExpectBlocks * 8, hipMemcpyDeviceToHost, stream));
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanBlock), 0, stream, sum_blk.as<uint32_t>(), grid, total);
+        HIP_TRY(hipGetLastError());
+        uint64_t K = 0;
+        HIP_TRY(hipMemcpyAsync(&K, total, 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        if (K == 0) return;
+        sum_v.reserve(K * 8);
+        if (member) sum_m.reserve(K * 2);
+        hipLaunchKernelGGL((k_terms<NW, PAULI, MODE, 1>), dim3(grid), dim3(kBlock), 0, stream, rows.as<uint64_t>(), a,
+                           b, member, N, seg, occ_s, sum_blk.as<uint32_t>(), sum_v.as<double>(),
+                           member ? sum_m.as<uint16_t>() : static_cast<uint16_t *>(nullptr));
+        HIP_TRY(hipGetLastError());
+        sum_out.reserve(static_cast<size_t>(nb) * 8);
+        hipLaunchKernelGGL(k_seq_sum, dim3(member ? (nb + kBlock - 1) / kBlock : 1), dim3(kBlock), 0, stream,
+                           sum_v.as<double>(), member ? sum_m.as<uint16_t>() : static_cast<uint16_t *>(nullptr), K,
+                           nb, sum_out.as<double>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, sum_out.p, static_cast<size_t>(nb) * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+
+    // core + sum_i a[i] b[i], the reference's state . op (MPFunctions.cpp:86-94), to the bit.
+    template <int NW, int PAULI>
+    double dot_core(const double *a, const double *b) {
         double r = 0.0;
-        for (double x : h) r += x;
+        ordered_sum<NW, PAULI, TERM_DOT>(a, b, nullptr, 1, &r);
         return core + r;
     }
 
@@ -2540,19 +2583,19 @@ struct mpa_engine {
             fused_cap = cap;
         }
         if (desc_epoch >= (1u << 30) - kChunk - 1) clear_descriptors();
-        uint64_t want = 1024;
-        while (want < cap) want <<= 1;  // a gate starts at N <= cap / 2: load <= 1/2 after it
+        // a gate starts at N <= cap / 2 and ends at N <= cap: load <= 1 / kIdxSlotsPerRow after it
+        const uint64_t want = (static_cast<uint64_t>(static_cast<double>(cap) * kIdxSlotsPerRow) + 63) / 64 * 64;
         if (want > gidx_slots) {
-            gidx.reserve(want * (nw + 1) * 8);
+            gidx.reserve((want + kIdxPad + kIdxGroup) * 8);
             gidx_slots = want;
             gidx_rows = static_cast<uint64_t>(-1);  // force a rebuild
         }
         if (gidx_rows != N) {
-            HIP_TRY(hipMemsetAsync(gidx.p, 0, gidx_slots * (nw + 1) * 8, stream));
+            HIP_TRY(hipMemsetAsync(gidx.p, 0, (gidx_slots + kIdxPad + kIdxGroup) * 8, stream));
             if (N) {
                 hipLaunchKernelGGL((k_gidx_build<NW>), dim3(static_cast<uint32_t>((N + kBlock - 1) / kBlock)),
                                    dim3(kBlock), 0, stream, rows.as<uint64_t>(), 0, N,
-                                   BATCH ? bmember.as<uint16_t>() : nullptr, gidx.as<uint64_t>(), gidx_slots - 1,
+                                   BATCH ? bmember.as<uint16_t>() : nullptr, gidx.as<uint64_t>(), gidx_slots,
                                    counters.as<uint64_t>());
                 HIP_TRY(hipGetLastError());
             }
@@ -2627,8 +2670,8 @@ struct mpa_engine {
         size_t gi = 0;
         double rate = 1.0;  // per-gate growth of the live count over the last chunk
         while (gi < ng) {
-            if (2 * N > cap) {
-                const uint64_t need = std::max<uint64_t>(3 * N, 64ull * kBlock);
+            if (2 * N > cap) {  // a gate may at most double the live count
+                const uint64_t need = 2 * N + N / 4;
                 if (cap_limit && grown_capacity(need) > cap_limit) break;  // stop before gate gi
                 ensure_capacity(need);
             }
@@ -2646,7 +2689,6 @@ struct mpa_engine {
             const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
             const uint32_t rgrid =
                 static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, res_grid_cap));
-            const uint64_t gmask = gidx_slots - 1;
             const size_t rlds = 2 * (gsum_stride + 1) * sizeof(uint32_t);
             for (size_t g = gi; g < end; ++g) {
                 const GateDesc &gd = gds[g];
@@ -2680,8 +2722,8 @@ struct mpa_engine {
                                    ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
-                                   gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gmask,
-                                   desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap, gidx_slots,
+                                   gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gidx_slots,
+                                   desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap,
                                    static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig,
                                    schr ? occ_dev.as<uint64_t>() : static_cast<uint64_t *>(nullptr));
                 HIP_TRY(hipGetLastError());
@@ -2732,18 +2774,8 @@ struct mpa_engine {
     // <O> of every batch member (its identity coefficient plus its paired rows), into out[nbatch].
     template <int NW, int PAULI>
     void expect_batch(double *out) {
-        bpartial.reserve(static_cast<size_t>(kExpectBlocks) * nbatch * 8);
-        bout.reserve(static_cast<size_t>(nbatch) * 8);
-        hipLaunchKernelGGL((k_expect_batch<NW, PAULI>), dim3(kExpectBlocks), dim3(kBlock),
-                           static_cast<size_t>(kWaves) * nbatch * 8, stream, rows.as<uint64_t>(), coef.as<double>(),
-                           bmember.as<uint16_t>(), N, nbatch, occ_dev.as<uint64_t>(), bpartial.as<double>());
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_reduce_members, dim3((nbatch + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
-                           bpartial.as<double>(), static_cast<uint32_t>(kExpectBlocks), nbatch, bout.as<double>());
-        HIP_TRY(hipGetLastError());
-        std::vector<double> h(nbatch);
-        HIP_TRY(hipMemcpyAsync(h.data(), bout.p, nbatch * 8, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
+        std::vector<double> h(nbatch, 0.0);
+        ordered_sum<NW, PAULI, TERM_PAIRED>(coef.as<double>(), nullptr, bmember.as<uint16_t>(), nbatch, h.data());
         for (uint32_t q = 0; q < nbatch; ++q) out[q] = bcore[q] + h[q];
     }
 
@@ -2765,15 +2797,8 @@ struct mpa_engine {
 
     template <int NW, int PAULI>
     double expect_impl(const double *c) {
-        partial.reserve(kExpectBlocks * 8);
-        hipLaunchKernelGGL((k_expect<NW, PAULI>), dim3(kExpectBlocks), dim3(kBlock), 0, stream, rows.as<uint64_t>(),
-                           c, N, occ_dev.as<uint64_t>(), partial.as<double>());
-        HIP_TRY(hipGetLastError());
-        std::vector<double> h(kExpectBlocks);
-        HIP_TRY(hipMemcpyAsync(h.data(), partial.p, kExpectBlocks * 8, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
         double r = 0.0;
-        for (double x : h) r += x;
+        ordered_sum<NW, PAULI, TERM_PAIRED>(c, nullptr, nullptr, 1, &r);
         return core + r;
     }
 
@@ -2977,7 +3002,7 @@ struct RunSchrExpect {
         std::vector<double> hv;
         e->match_observable<NW>(e->coef.as<double>(), hr, hv, e->rep.as<double>());
         e->replay(params, snapshots, mpa_engine::kObservable);
-        *out = e->dot_core(e->coef.as<double>(), e->rep.as<double>());
+        *out = e->dot_core<NW, PAULI>(e->coef.as<double>(), e->rep.as<double>());
     }
 };
 
@@ -3040,10 +3065,10 @@ void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *
             // Schrodinger picture (MonomialPropagator.h:411-416)
             const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
             const double apply = e->schr ? -alpha : alpha;
-            gd.k = std::cos(2 * apply);
-            gd.s = std::sin(2 * apply);
-            gd.abs_s = std::abs(std::sin(2 * alpha));
-            const double cos_build = std::cos(2.0 * alpha);
+            sin_cos(2 * apply, &gd.s, &gd.k);
+            double sin_build, cos_build;
+            sin_cos(2 * alpha, &sin_build, &cos_build);
+            gd.abs_s = std::abs(sin_build);
             gd.fused = (fused_allowed && cos_build != 0.0) ? 1 : 0;
             gd.inv_k = gd.fused ? 1.0 / cos_build : 1.0;
         } else {
@@ -3520,10 +3545,9 @@ int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t
             const double avail = static_cast<double>(free_b) +
                                  static_cast<double>(BlockCache::get().cached_bytes(e->device)) +
                                  e->bytes_per_row() * static_cast<double>(e->cap);
-            const double rows = 0.75 * avail / (1.5 * e->bytes_per_row());
-            uint64_t lim = 64ull * kBlock;
-            while (static_cast<double>(lim) * 2 <= rows) lim <<= 1;
-            e->cap_limit = lim;
+            // the last growth holds the old store and the new one (1.5 times larger) at once
+            const double rows = 0.75 * avail / (2.5 * e->bytes_per_row());
+            e->cap_limit = std::max<uint64_t>(static_cast<uint64_t>(rows) / mpa_engine::kGranule, 1) * mpa_engine::kGranule;
         } else {
             e->cap_limit = static_cast<uint64_t>(max_capacity);
         }
@@ -3542,10 +3566,9 @@ int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t
                     continue;
                 }
                 const double alpha = 1.0 * params[static_cast<int64_t>(q) * n_params + gd.param_index] * gd.gen_coeff;
-                t.k = std::cos(2 * alpha);
-                t.s = std::sin(2 * alpha);
-                t.abs_s = std::abs(std::sin(2 * alpha));
-                const double cos_build = std::cos(2.0 * alpha);
+                sin_cos(2 * alpha, &t.s, &t.k);
+                t.abs_s = std::abs(t.s);
+                const double cos_build = t.k;
                 if (cos_build == 0.0)
                     throw RunError("batch member " + std::to_string(q) + " has cos(2 alpha) == 0 at a gate; the batched "
                                    "pipeline needs the fused cos sweep, propagate that member alone");
@@ -3757,7 +3780,7 @@ int mpa_memory_breakdown(mpa_engine *e, int64_t *out, int32_t n) {
         for (const DevArray *a : {&e->f_elead, &e->t_val, &e->t_item, &e->t_cnt, &e->t_bsum, &e->t_gsum, &e->desc_m,
                                   &e->AL, &e->lead, &e->foll, &e->blk_a, &e->blk_b, &e->rec, &e->lv, &e->match,
                                   &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam,
-                                  &e->gpartial, &e->partial})
+                                  &e->gpartial, &e->sum_blk, &e->sum_v, &e->sum_m, &e->sum_out})
             work += a->bytes;
         f[10] = static_cast<int64_t>(work);
         std::copy(f, f + kMemFields, out);

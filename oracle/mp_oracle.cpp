@@ -53,6 +53,14 @@ inline size_t ranges_for(size_t n) {
 
 inline int popc(uint64_t x) { return __builtin_popcountll(x); }
 
+// sin and cos of one angle from one glibc sincos call.  The reference evaluates std::sin and std::cos
+// of the same angle side by side (FusedApply.h:47-48, Evolution.cpp:475-476, Scan.h:44 next to
+// Engine.h:552), which GCC, the compiler of its Linux builds, turns into a single sincos call; glibc's
+// sincos can differ from its sin or cos in the last bit (2a = -2.831787767853196: sin gives
+// -0x1.38309465f345bp-2, sincos -0x1.38309465f345ap-2), so every angle here goes through sincos,
+// whatever the compiler of this file decides.
+inline void sin_cos(double x, double *s, double *c) { ::sincos(x, s, c); }
+
 // n*(n-1)/2 in size_t arithmetic, as reference cpp/monoprop/Utilities.h:64-66.
 inline size_t n_choose_2(size_t n) { return n * (n - 1) / 2; }
 
@@ -380,10 +388,12 @@ struct Engine {
         const bool local_coeffs = cp != nullptr;
         const bool check_atol = has_lower && local_coeffs && have_param;
         const bool check_upper = has_upper && local_coeffs;
-        const double sin_cut = have_param ? std::sin(2 * alpha) : 1.0;  // Scan.h:81-95
+        double sin2 = 1.0, cos2 = 1.0;
+        if (have_param) sin_cos(2 * alpha, &sin2, &cos2);
+        const double sin_cut = sin2;  // Scan.h:81-95
         const double abs_sin = std::abs(sin_cut);
         const bool use_coeff_checks = check_atol || check_upper;
-        const double cos_build = (contract && have_param) ? std::cos(2.0 * alpha) : 1.0;  // Engine.h:552
+        const double cos_build = contract ? cos2 : 1.0;  // Engine.h:552
         const bool fused = contract && only_rot < 0 && have_param && cos_build != 0.0;
         const double inv_cos = fused ? 1.0 / cos_build : 1.0;  // Engine.h:614
 
@@ -544,8 +554,8 @@ struct Engine {
         if (contract) {
             // FusedApply.h:46-82
             std::vector<double> &c = *cp;
-            const double cos_val = std::cos(2 * apply_alpha);
-            const double sin_val = std::sin(2 * apply_alpha);
+            double sin_val, cos_val;
+            sin_cos(2 * apply_alpha, &sin_val, &cos_val);
             if (!fused) {
 #pragma omp parallel for num_threads(g_threads) if (g_threads > 1 && cos_set.size() > kParMin)
                 for (size_t j = 0; j < cos_set.size(); ++j) c[cos_set[j]] *= cos_val;
@@ -593,8 +603,8 @@ struct Engine {
 
     // evolve_step (Evolution.cpp:471-512): snapshot the pre-cos partner values, cos-scale, add sines.
     void evolve_step(std::vector<double> &op, const Layer &ly, const std::vector<uint32_t> &cos, double alpha) const {
-        const double cos_val = std::cos(2 * alpha);
-        const double sin_val = std::sin(2 * alpha);
+        double sin_val, cos_val;
+        sin_cos(2 * alpha, &sin_val, &cos_val);
         std::vector<double> snap_t(ly.rots.size()), snap_s(ly.rots.size());
         for (size_t k = 0; k < ly.rots.size(); ++k) {
             snap_t[k] = op[ly.rots[k].tgt];
@@ -1058,7 +1068,8 @@ int orc_graph_grad(void *h, const double *params, int64_t n_params, double *valu
     for (size_t l = nl; l-- > 0;) {
         const Layer &ly = at(l);
         const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
-        const double k = std::cos(2 * alpha), s = std::sin(2 * alpha);
+        double s, k;
+        sin_cos(2 * alpha, &s, &k);
         double d = 0.0;
         for (size_t j = 0; j < cos_only[l].size(); ++j) {
             const uint32_t i = cos_only[l][j];

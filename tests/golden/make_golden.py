@@ -7,13 +7,24 @@ evolved operator (slot tuple -> coefficient).  The oracle is pinned independentl
 statevector, the reference's fixtures); these vectors freeze its output so that the engine can be
 checked without the oracle and the oracle cannot drift unnoticed.
 
-Usage: python tests/golden/make_golden.py   (from the repository root; rewrites the JSON file)
+The workload cases are the benchmark configurations at their own sizes where the oracle finishes
+them in about a minute on the container's threads: the 10x10 kicked-Ising lattice over all 20
+Trotter steps at theta_h 0.1, 0.4 and 0.8 (cutoff 10, lower_atol 1e-5); the 6x6 spinful Hubbard
+lattice (72 modes, three words per row) over its first 2 and 3 Trotter steps (length cutoff 8,
+lower_atol 1e-6); the 64-qubit brickwork ring over all 12 layers at the seeded angles (cutoff 8,
+lower_atol 1e-6); and the parameter sweep's own eight observables at its first two parameter sets,
+3 Trotter steps (cutoff 8, lower_atol 1e-5).  Cases marked ``heavy`` take the oracle tens of seconds
+on eight threads; the CPU tests re-run them only when MPA_GOLDEN_HEAVY=1.
+
+Usage: python tests/golden/make_golden.py [name ...]   (from the repository root; rewrites the JSON
+file, recomputing only the named cases when names are given)
 """
 
 from __future__ import annotations
 
 import hashlib
 import json
+import os
 import sys
 from pathlib import Path
 
@@ -52,7 +63,40 @@ def cases():
                                  lower=1e-6, upper=None, state=[0, 3, 4, 7], schr=None)
     out["hubbard2_tiny"] = dict(out["hubbard2_short"], circuit=("hubbard_lattice", dict(L=2, steps=1)), cutoff=4,
                                 lower=1e-2)
+    # the benchmark configurations
+    mz = workloads.mean_z(100).get_local_operator().terms
+    for tag, th in (("01", 0.1), ("04", 0.4), ("08", 0.8)):
+        out[f"lattice10_full_h{tag}"] = dict(basis="pauli", n=100, obs=mz,
+                                            circuit=("kicked_ising_lattice", dict(L=10, steps=20, theta_h=th)),
+                                            cutoff=10, cutoff_type="support", lower=1e-5, upper=None, state=[],
+                                            schr=None, heavy=th == 0.4)
+    _, n_h = workloads.hubbard_lattice(6, 1)
+    n0 = workloads.number_operator(0, n_h).terms
+    state6 = list(workloads.hubbard_lattice(6, 1)[0].initial_state)
+    for steps in (2, 3):
+        out[f"hubbard6_{steps}steps"] = dict(basis="majorana", n=n_h, obs=n0,
+                                             circuit=("hubbard_lattice", dict(L=6, steps=steps)), cutoff=8,
+                                             cutoff_type="length", lower=1e-6, upper=None, state=state6, schr=None)
+    out["ring64_12layers"] = dict(basis="pauli", n=64, obs=workloads.ring_zz_mean(64).get_local_operator().terms,
+                                  circuit=("brickwork_ring", dict(n=64, layers=12, seed=0)), cutoff=8,
+                                  cutoff_type="support", lower=1e-6, upper=None, state=[], schr=None, heavy=True)
+    sets = workloads.sweep_parameter_sets()
+    for si in range(2):
+        for j, op in enumerate(workloads.sweep_observables()):
+            out[f"sweep_set{si}_obs{j}"] = dict(basis="pauli", n=100, obs=op.get_local_operator().terms,
+                                                circuit=("kicked_ising_lattice", dict(L=10, steps=3)), cutoff=8,
+                                                cutoff_type="support", lower=1e-5, upper=None, state=[], schr=None,
+                                                param_set=[float(x) for x in sets[si]])
     return out
+
+
+def parameters(case, circ):
+    """The circuit's parameters, or for a sweep case its (theta_J, theta_h) set on the ZZ / X gates."""
+    from monoprop_amd import workloads
+
+    if "param_set" in case:
+        return [float(x) for x in workloads.kicked_ising_rows(circ, [case["param_set"]])[0]]
+    return list(circ.parameters)
 
 
 def build_circuit(spec):
@@ -76,6 +120,8 @@ def digest(a) -> str:
 
 
 def run_oracle(case):
+    """The oracle on the case, on the host's threads (its thread-parallel ranges are joined in serial
+    order, so the result is the serial oracle's bit for bit)."""
     import oracle
 
     circ = build_circuit(case["circuit"])
@@ -83,27 +129,43 @@ def run_oracle(case):
     o = oracle.Oracle(case["obs"], case["n"], basis=case["basis"], cutoff=case["cutoff"],
                       cutoff_type=case["cutoff_type"], lower_atol=case["lower"], upper_atol=case["upper"],
                       initial_state=case["state"], schrodinger_cutoff=case["schr"])
-    o.propagate(monos, pm, gcs, circ.parameters)
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        o.propagate(monos, pm, gcs, parameters(case, circ))
+    finally:
+        oracle.set_threads(1)
     return o
+
+
+def stats_digest(n, a, e, i) -> str:
+    """SHA-256 of the per-gate counters (live, anticommuting, emitted, inserted), int64, gate-major."""
+    import numpy as np
+
+    return digest(np.stack([np.asarray(x, dtype=np.int64) for x in (n, a, e, i)], axis=1))
 
 
 def record(o, keep_terms: bool) -> dict:
     w, c = o.rows()
+    n, a, e, i = o.gate_stats()
     rec = {"terms": int(o.size()), "expectation_hex": float(o.expectation()).hex(), "rows_sha256": digest(w),
-           "coeffs_sha256": digest(c)}
+           "coeffs_sha256": digest(c), "term_gates": int(sum(int(x) for x in n)), "stats_sha256": stats_digest(n, a, e, i)}
     if keep_terms:
         rec["evolved"] = [[list(k), [v.real, v.imag]] for k, v in sorted(o.evolved_operator(0.0).items())]
     return rec
 
 
-def main() -> None:
-    out = {}
+def main(names=()) -> None:
+    out = json.loads(OUT.read_text()) if names and OUT.exists() else {}
     for name, case in cases().items():
+        if names and name not in names:
+            continue
         o = run_oracle(case)
         out[name] = record(o, keep_terms=o.size() <= 400)
         print(name, out[name]["terms"], float.fromhex(out[name]["expectation_hex"]), flush=True)
+        del o
+    out = {k: out[k] for k in cases() if k in out}
     OUT.write_text(json.dumps(out, indent=1) + "\n")
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

@@ -27,7 +27,8 @@ def test_reference_names_exported():
     assert mp.has_mpi is False
     from monoprop_amd._engine import DeviceSimulator
 
-    assert mp.MAX_NUM_MODES == DeviceSimulator.storage_num_modes
+    assert mp.MAX_NUM_MODES == 250  # monoprop_MAX_NUM_MODES (reference CMakeLists.txt:45-50)
+    assert DeviceSimulator.storage_num_modes == 256
     assert "--offload-arch=gfx950" in mp.__compiler_flags__
 
 
@@ -68,3 +69,24 @@ def test_antihermitian_helpers_against_dense_products(n_qubits):
             cM = c * M
             assert np.allclose(cM.conj().T, -cM), (idx, c)
             assert abs(abs(c) - 1.0) == 0.0
+
+
+def test_dispatch_mode_count_errors():
+    """The reference's dispatch checks (tools/generate-dispatch.py:145-160): a ValueError subclass
+    with its messages, for a non-positive count and for more than MAX_NUM_MODES modes."""
+    from monoprop_amd._engine import dispatch
+    from monoprop_amd.exceptions import NumberOfModesInvalidError
+
+    assert issubclass(NumberOfModesInvalidError, ValueError)
+    for bad in (0, -3):
+        with pytest.raises(NumberOfModesInvalidError, match=f"Number of Fermionic modes {bad} invalid. num_modes must be > 0."):
+            dispatch(bad)
+    with pytest.raises(NumberOfModesInvalidError, match="Number of Fermionic modes 251 invalid. num_modes must be <= 250. "
+                                                       "Contact monoprop developers if more than 250 Fermionic modes "
+                                                       "are required."):
+        dispatch(251)
+    with pytest.raises(NumberOfModesInvalidError):
+        dispatch(2.0)
+    assert callable(dispatch(250))
+    with pytest.raises(NumberOfModesInvalidError):
+        mp.PauliPropagator(mp.PauliOperator({"Z": 1.0}, 300), [], cutoff=4)

@@ -49,7 +49,7 @@ def test_propagate_and_graph_match_oracle(oracle_mod, table_kind, cutoff_type, c
     wd, cd = d.rows()
     assert np.array_equal(wo, wd) and np.array_equal(co, cd)
     # the stored terms are bit-exact; <O> is a device reduction, equal up to summation order
-    assert abs(o.expectation() - d.expectation_value([])) <= 1e-12
+    assert o.expectation() == d.expectation_value([])
     # the native basis keeps a different set here, so the table is in effect
     n_native = oracle_mod.Oracle(obs, n_modes, basis="majorana", cutoff=cutoff, cutoff_type=cutoff_type,
                                  lower_atol=1e-9, initial_state=state)

@@ -2,7 +2,7 @@
 
 Every member of a batch store must hold, in store order, exactly the rows and FP64 coefficients the
 oracle produces propagating that member alone (bit-exact), the same per-member term count, and <O>
-within 1e-12.  Members share the circuit structure and differ in observable and angles; identical
+to the bit.  Members share the circuit structure and differ in observable and angles; identical
 members and identity-only members check that equal monomials of different members never merge.
 """
 
@@ -45,7 +45,7 @@ def _check_members(oracle_mod, b, basis, ops, circ, params, **trunc):
         assert np.array_equal(w[sel], wo), f"member {q}: rows differ"
         assert np.array_equal(c[sel], co), f"member {q}: coefficients differ"
         assert sizes[q] == o.size()
-        assert abs(ev[q] - o.expectation()) <= 1e-12, (q, ev[q], o.expectation())
+        assert ev[q] == o.expectation(), (q, ev[q], o.expectation())
 
 
 def _chain_params(circ, sets):
@@ -145,7 +145,7 @@ def test_expectation_grid_small_batches(oracle_mod):
     for i in range(5):
         for j in range(2):
             o = _oracle(oracle_mod, "pauli", obs[j], circ, params[i], cutoff=6, cutoff_type="support", lower=1e-6)
-            assert abs(grid[i, j] - o.expectation()) <= 1e-12
+            assert grid[i, j] == o.expectation()
 
 
 def test_batch_refuses_single_entry_points():
@@ -203,7 +203,7 @@ def test_expectation_grid_splits_match_oracle(oracle_mod):
     for i in range(len(sets)):
         for j in range(len(obs)):
             o = _oracle(oracle_mod, "pauli", obs[j], circ, params[i], cutoff=8, cutoff_type="support", lower=1e-8)
-            assert abs(grid[i, j] - o.expectation()) <= 1e-12, (i, j)
+            assert grid[i, j] == o.expectation(), (i, j)
 
 
 def test_device_split_then_resume_is_bit_exact(oracle_mod):

@@ -33,7 +33,7 @@ def _same(o, d):
     wo, co = o.rows()
     wd, cd = d.rows()
     assert np.array_equal(wo, wd) and np.array_equal(co, cd)
-    assert abs(o.expectation() - d.expectation_value([])) <= 1e-12
+    assert o.expectation() == d.expectation_value([])
 
 
 @pytest.mark.parametrize("schr", [None, 2])
@@ -116,8 +116,12 @@ def test_error_paths():
     d = DeviceSimulator({(0,): 1.0}, 4, [], logical_num_modes=n, basis="pauli", cutoff_type="support")
     with pytest.raises(Exception):
         d.propagate([(0, 9)], [0], [1.0], [0.1])  # generator slot past 2n
-    with pytest.raises(ValueError):
-        d.propagate([(2,)], [3], [1.0], [0.1])  # parameter index past the parameters
+    with pytest.raises(RuntimeError, match=r"The length of parameters \(1\) must be the same as "
+                                           r"max\(parameter_mapping\)\+1 \(4\)\."):
+        d.propagate([(2,)], [3], [1.0], [0.1])  # shorter than max(mapping) + 1 (Validation.cpp:79-91)
+    with pytest.raises(RuntimeError, match=r"The length of parameters \(3\) must be the same as "
+                                           r"max\(parameter_mapping\)\+1 \(1\)\."):
+        d.propagate([(2,)], [0], [1.0], [0.1, 0.2, 0.3])  # longer
     d.build_graph([(2,)], [0], [1.0])
     with pytest.raises(RuntimeError, match="graph"):
         d.propagate([(2,)], [0], [1.0], [0.1])

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import importlib.util
 import json
+import os
 from pathlib import Path
 
 import pytest
@@ -20,6 +21,8 @@ VECTORS = json.loads((GOLDEN / "oracle_vectors.json").read_text())
 @pytest.mark.parametrize("name", sorted(VECTORS))
 def test_oracle_reproduces_golden(oracle_mod, name):
     case = make_golden.cases()[name]
+    if case.get("heavy") and os.environ.get("MPA_GOLDEN_HEAVY") != "1":
+        pytest.skip("heavy workload case (tens of seconds of oracle time); MPA_GOLDEN_HEAVY=1 runs it")
     o = make_golden.run_oracle(case)
     got = make_golden.record(o, keep_terms="evolved" in VECTORS[name])
     assert got == VECTORS[name]
