@@ -150,6 +150,31 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds) {
     return tot;
 }
 
+// Exclusive scan of 64-bit values across a block of kWaves wavefronts (two packed 32-bit counts that
+// never carry into each other); total in *total.
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t *lds, uint64_t *total) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t t = __shfl_up(incl, d, 64);
+        if (lane >= static_cast<uint32_t>(d)) incl += t;
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane == 63) lds[wave] = incl;
+    __syncthreads();
+    uint64_t wave_base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint64_t s = lds[w];
+        if (static_cast<uint32_t>(w) < wave) wave_base += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_base + incl - v;
+}
+
 // Wave-aggregated append: every lane of the wave must call it; lanes with `pred` get consecutive slots
 // of the list whose length lives at *counter (one atomic per wave).
 __device__ __forceinline__ uint64_t wave_append(bool pred, uint64_t *counter) {
@@ -737,10 +762,8 @@ __device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_
     const uint64_t h = gidx_hash<NW>(key, member);
     const uint64_t val = (h << 32) | (static_cast<uint64_t>(row) + 1);
     const uint64_t end = gslots + kIdxPad;
-    for (uint64_t pos = gidx_home(h, gslots); pos < end; ++pos) {
-        if (__hip_atomic_load(gidx + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
+    for (uint64_t pos = gidx_home(h, gslots); pos < end; ++pos)
         if (atomicCAS(reinterpret_cast<unsigned long long *>(gidx + pos), 0ull, val) == 0ull) return true;
-    }
     return false;
 }
 
@@ -924,14 +947,14 @@ constexpr int kWTileWords = 16;                          // sweep: one wavefront
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
 // Count groups: 2^group_log2 tiles (64 tiles = 65536 terms, 16 atomics per word, up to 1024 tiles for
 // the largest stores) so that at most kMaxGroups prefixes sit in LDS.
-constexpr int kMinGroupLog2 = 6, kMaxGroupLog2 = 10;
+constexpr int kMinGroupLog2 = 4, kMaxGroupLog2 = 10;
 constexpr uint32_t kMaxGroups = 4096;
 // per-group count word: leaders | followers << 21 | anticommuting << 42 (each <= 2^20)
 constexpr uint32_t kGrpBits = 21;
 constexpr uint64_t kGrpMask = (uint64_t{1} << kGrpBits) - 1;
 constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
 constexpr uint32_t kMaxGrid = 1280;                       // resolve blocks that take chunks (5 per CU)
-constexpr int kBsumBatch = 8;                             // resolve: sweep-block sums read per round
+constexpr int kBsumBatch = 16;                            // resolve: sweep-block sums read per round
 // per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
 constexpr uint32_t kCntBits = 16;
 constexpr uint64_t kCntMask = (uint64_t{1} << kCntBits) - 1;
@@ -950,10 +973,29 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Column ids of one gate for the sweep, passed by value: G's fold columns, then its pivot columns.
+// Generators with more than kColArgs columns read the rest from the gate's fold list in memory.
+constexpr int kColArgs = 8;
+struct SweepCols {
+    int32_t col[kColArgs];
+    int32_t nfold, ncol;  // ncol = nfold + pivot columns
+    int32_t piv0, piv1;
+};
+
+// Column id of slot cc (< kColArgs) for lane slot sl (wave-uniform base `pass`): a select over the
+// by-value ids, so no memory access is needed to know which column to read.
+__device__ __forceinline__ int sweep_col(const SweepCols &sc, int pass, int sl) {
+    const int b = pass * 4;
+    // the ids as values (scalar registers), so the select is over values, not over their addresses
+    const int c0 = __builtin_amdgcn_readfirstlane(sc.col[b]), c1 = __builtin_amdgcn_readfirstlane(sc.col[b + 1]);
+    const int c2 = __builtin_amdgcn_readfirstlane(sc.col[b + 2]), c3 = __builtin_amdgcn_readfirstlane(sc.col[b + 3]);
+    return sl == 0 ? c0 : sl == 1 ? c1 : sl == 2 ? c2 : c3;
+}
+
 // Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate.  One wavefront
 // (one 64-thread block) per 1024-term tile, no dependence between tiles:
-//   A  G's fold and pivot columns of the tile's 16 words, all loads in flight (lane = slot * 16 + word),
-//      XOR-reduced across lanes -> anticommuting mask a_j, pivot mask p_j
+//   A  G's fold and pivot columns of the tile's 16 words (lane = slot * 16 + word), loaded by the kernel
+//      before anything waits (xc), XOR-reduced across lanes -> anticommuting mask a_j, pivot mask p_j
 //   B  c <- c k on every anticommuting term (the reference's in-place sweep); lane b owns bit b of every
 //      word, so at most 16 loads per lane, all in flight; the pre-gate values stay in the lane's
 //      registers.  Terms past the lower_atol gate go to the tile's candidate list (LDS)
@@ -963,17 +1005,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 //      leaders ascending then followers ascending (pivot clear / set): lane b writes those of bit b of
 //      every word, with the values it kept from B, at positions from per-word popcounts.  The tile's
 //      counts, also summed per group of tiles; leader emit masks per word
-//   BATCH  rows belong to batch members (member[row]); each term takes its member's k and |s|
-//          (gtrig[member]) instead of the gate's
+//   BATCH  rows belong to batch members (member[row]); each term takes its member's k and |s| from the
+//          block's copy of the gate's per-member constants (s_trig: k, |s| per member, LDS)
 template <int NW, int PAULI, int BATCH>
 __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
-                                               uint64_t colstride, const int32_t *__restrict__ fold, int nfold,
-                                               int piv0, int piv1, const uint64_t *__restrict__ rows,
+                                               uint64_t colstride, const SweepCols &sc, const int32_t *__restrict__ fold,
+                                               const uint64_t *xc, const uint64_t *__restrict__ rows,
                                                double *__restrict__ coef, double *__restrict__ t_val,
                                                uint32_t *__restrict__ t_item, uint64_t *__restrict__ elead,
                                                uint64_t *__restrict__ tcnt, uint64_t N, uint64_t tile,
                                                uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx,
-                                               const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig) {
+                                               const uint16_t *__restrict__ member, const double *__restrict__ s_trig) {
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t w0 = tile * kWTileWords;
     const uint32_t lane = lane_id();
@@ -983,33 +1025,21 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     {
         const int j = static_cast<int>(lane & (kWTileWords - 1)), sl = static_cast<int>(lane / kWTileWords);
         const uint64_t w = w0 + j;
-        const int npiv = (piv0 >= 0) + (piv1 >= 0);
-        const int ncol = nfold + npiv;
         constexpr int kSlots = 64 / kWTileWords;  // 4 column slots per pass
-        uint64_t x[2];
 #pragma unroll
         for (int pass = 0; pass < 2; ++pass) {
             const int cc = pass * kSlots + sl;
-            x[pass] = 0;
-            if (w < nwords && cc < ncol) {
-                const int col = cc < nfold ? fold[cc] : (cc == nfold ? piv0 : piv1);
-                x[pass] = cols[static_cast<uint64_t>(col) * colstride + w];
-            }
+            if (cc < sc.nfold) a ^= xc[pass];
+            else p ^= xc[pass];
         }
-#pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
-            const int cc = pass * kSlots + sl;
-            if (cc < nfold) a ^= x[pass];
-            else p ^= x[pass];
-        }
-        for (int cc0 = 2 * kSlots; cc0 < ncol; cc0 += kSlots) {  // generators wider than 8 columns
+        for (int cc0 = 2 * kSlots; cc0 < sc.ncol; cc0 += kSlots) {  // generators wider than 8 columns
             const int cc = cc0 + sl;
             uint64_t y = 0;
-            if (w < nwords && cc < ncol) {
-                const int col = cc < nfold ? fold[cc] : (cc == nfold ? piv0 : piv1);
+            if (w < nwords && cc < sc.ncol) {
+                const int col = cc < sc.nfold ? fold[cc] : (cc == sc.nfold ? sc.piv0 : sc.piv1);
                 y = cols[static_cast<uint64_t>(col) * colstride + w];
             }
-            if (cc < nfold) a ^= y;
+            if (cc < sc.nfold) a ^= y;
             else p ^= y;
         }
 #pragma unroll
@@ -1025,7 +1055,10 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     uint32_t cand = 0, resc = 0;  // candidates; those the upper_atol rescue emits regardless
     double v[kWTileWords];        // pre-gate values, kept for D
     {
+        // BATCH: the 32-bit word holding the row's 16-bit member (a whole-word load, so the loads of
+        // all 16 words stay in flight together)
         uint32_t q[BATCH ? kWTileWords : 1];
+        const uint32_t *mw = reinterpret_cast<const uint32_t *>(member);
         uint32_t act = 0;
 #pragma unroll
         for (int j = 0; j < kWTileWords; ++j) {
@@ -1033,8 +1066,9 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
             v[j] = 0.0;
             if (BATCH) q[j] = 0;
             if ((aj >> lane) & 1u) {
-                v[j] = coef[row0 + static_cast<uint64_t>(j) * 64 + lane];
-                if (BATCH) q[j] = member[row0 + static_cast<uint64_t>(j) * 64 + lane];
+                const uint64_t r = row0 + static_cast<uint64_t>(j) * 64 + lane;
+                v[j] = coef[r];
+                if (BATCH) q[j] = mw[r >> 1];
                 act |= 1u << j;
             }
         }
@@ -1042,8 +1076,9 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
         for (int j = 0; j < kWTileWords; ++j) {
             if (!((act >> j) & 1u)) continue;
             const uint64_t r = row0 + static_cast<uint64_t>(j) * 64 + lane;
-            const double k = BATCH ? gtrig[q[BATCH ? j : 0]].k : gd.k;
-            const double abs_s = BATCH ? gtrig[q[BATCH ? j : 0]].abs_s : gd.abs_s;
+            const uint32_t qm = BATCH ? ((q[BATCH ? j : 0] >> (16 * (lane & 1u))) & 0xffffu) : 0u;  // r & 1 == lane & 1
+            const double k = BATCH ? s_trig[2 * qm] : gd.k;
+            const double abs_s = BATCH ? s_trig[2 * qm + 1] : gd.abs_s;
             coef[r] = v[j] * k;
             if (!(tr.check_lower && abs_s * fabs(v[j]) <= tr.lower)) {
                 cand |= 1u << j;
@@ -1121,33 +1156,61 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
            (static_cast<uint64_t>(na) << (2 * kGrpBits));
 }
 
+// Column words of G's first kColArgs columns for tile `tile` (lane = slot * 16 + word); the tiles cover
+// the capacity, so the words are in bounds for any live count.
+__device__ __forceinline__ void sweep_load_cols(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                const SweepCols &sc, uint64_t tile, uint64_t *xc) {
+    const uint32_t lane = lane_id();
+    const int j = static_cast<int>(lane & (kWTileWords - 1)), sl = static_cast<int>(lane / kWTileWords);
+    const uint64_t w = tile * kWTileWords + j;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const int cc = pass * 4 + sl;
+        xc[pass] = cc < sc.ncol ? cols[static_cast<uint64_t>(sweep_col(sc, pass, sl)) * colstride + w] : 0;
+    }
+}
+
 // One 1024-term tile per wavefront, four per block; the block stores its tiles' summed counts
-// (bsum[block]) and adds them into their group word with one atomic.
+// (bsum[block]) and adds them into their group word with one atomic.  The live count, the stop flag,
+// the tile's column words and (BATCH) the per-member constants are all requested before anything
+// waits on any of them.  (A persistent grid of resident blocks walking the tiles, with the next
+// tiles' columns prefetched, was measured slower: more registers, and a barrier per tile set.)
 template <int NW, int PAULI, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ gdesc, int gi, Trunc tr,
-                                                  const uint64_t *__restrict__ cols, uint64_t colstride,
-                                                  const int32_t *__restrict__ fold, int nfold, int piv0, int piv1,
+__global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr, const uint64_t *__restrict__ cols,
+                                                  uint64_t colstride, SweepCols sc, const int32_t *__restrict__ fold,
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                   double *__restrict__ t_val, uint32_t *__restrict__ t_item,
                                                   uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
                                                   uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
                                                   int group_log2, uint64_t *__restrict__ counters,
-                                                  const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig) {
+                                                  const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig,
+                                                  uint32_t nbatch) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
-    if (counters[C_STOP]) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_TK_RES] = 0;
+    extern __shared__ double s_trig[];  // BATCH: k, |s| per member
+    const uint64_t stop = counters[C_STOP];
     const uint64_t N = counters[C_NB + (gi & 1)];
-    const uint64_t nwords = (N + 63) / 64;
-    const uint64_t tile0 = static_cast<uint64_t>(blockIdx.x) * kWaves;
-    if (tile0 * kWTileWords >= nwords) return;  // whole block idle
     const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t tile0 = static_cast<uint64_t>(blockIdx.x) * kWaves;
     const uint64_t tile = tile0 + wave;
+    uint64_t xc[2];
+    sweep_load_cols(cols, colstride, sc, tile, xc);
+    if (BATCH) {
+        for (uint32_t q = threadIdx.x; q < nbatch; q += kBlock) {
+            s_trig[2 * q] = gtrig[q].k;
+            s_trig[2 * q + 1] = gtrig[q].abs_s;
+        }
+        __syncthreads();
+    }
+    if (stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_TK_RES] = 0;
+    const uint64_t nwords = (N + 63) / 64;
+    if (tile0 * kWTileWords >= nwords) return;  // whole block idle
     uint64_t grp = 0;
     if (tile * kWTileWords < nwords)
-        grp = sweep_tile<NW, PAULI, BATCH>(gdesc[gi], tr, cols, colstride, fold, nfold, piv0, piv1, rows, coef, t_val,
-                                           t_item, elead, tcnt, N, tile, sem[wave], cidx[wave], member, gtrig);
+        grp = sweep_tile<NW, PAULI, BATCH>(gdv, tr, cols, colstride, sc, fold, xc, rows, coef, t_val, t_item, elead,
+                                           tcnt, N, tile, sem[wave], cidx[wave], member, s_trig);
     if (lane_id() == 0) s_grp[wave] = grp;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1178,7 +1241,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const GateDesc *__restrict__ g
 // With BATCH, an emitter's member (member[S]) selects its rotation constants (gtrig[member]) and keys its
 // index lookup; the partner it creates joins the same member.
 template <int NW, int PAULI, int BATCH, int SCHR>
-__global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__ gdesc, int gi, uint32_t ep,
+__global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
                                                     const uint32_t *__restrict__ t_item,
@@ -1197,41 +1260,49 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
     __shared__ uint64_t lcol[kColWords * NC];
     extern __shared__ uint32_t dyn_lds[];  // group prefixes: leaders [0, max_groups], followers after
     __shared__ uint32_t lds[kWaves];
+    __shared__ uint64_t lds64[kWaves];
     __shared__ uint32_t s_ticket, s_base;
     __shared__ uint32_t st_row[2][kResChunk];  // a chunk's misses: emitter row, new coefficient
     __shared__ double st_ins[2][kResChunk];
     uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
-    if (counters[C_STOP]) return;
+    // The stop flag, the live count and every group word of the capacity (groups past this gate's are
+    // zero) are all requested before anything waits on them.
+    const uint64_t stop = counters[C_STOP];
+    const uint64_t N = counters[C_NB + (gi & 1)];
+    const uint32_t gper = (max_groups + kBlock - 1) / kBlock;  // <= kMaxGroups / kBlock
+    const uint32_t g0 = threadIdx.x * gper;
+    uint64_t gv[kMaxGroups / kBlock];
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxGroups / kBlock; ++i) gv[i] = (i < gper && g0 + i < max_groups) ? gsum[g0 + i] : 0;
+    if (stop) return;
     for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; g < n_groups_next;
          g += static_cast<uint64_t>(gridDim.x) * kBlock)
         gsum_next[g] = 0;
-    const GateDesc &gd = gdesc[gi];
     const int nxt = (gi & 1) ^ 1;
-    const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t ntiles = ((N + 63) / 64 + kWTileWords - 1) / kWTileWords;
     const uint32_t ng = static_cast<uint32_t>((ntiles + (uint64_t{1} << group_log2) - 1) >> group_log2);
     const uint64_t nblocks = (ntiles + kWaves - 1) / kWaves;  // sweep blocks of this gate
-    // ---- group prefixes (every block)
-    uint32_t nEL = 0, nEF = 0, nA = 0;
-    for (uint32_t base = 0; base < ng; base += kBlock) {
-        const uint32_t g = base + threadIdx.x;
-        const uint64_t v = g < ng ? gsum[g] : 0;
-        uint32_t tl, tf, ta;
-        const uint32_t xl = block_excl_scan(static_cast<uint32_t>(v & kGrpMask), lds, &tl);
-        const uint32_t xf = block_excl_scan(static_cast<uint32_t>((v >> kGrpBits) & kGrpMask), lds, &tf);
-        (void)block_excl_scan(static_cast<uint32_t>(v >> (2 * kGrpBits)), lds, &ta);
-        if (g < ng) {
-            gpl[g] = nEL + xl;
-            gpf[g] = nEF + xf;
-        }
-        nEL += tl;
-        nEF += tf;
-        nA += ta;
+    // ---- group prefixes (every block): thread t sums its gper consecutive groups, one block scan of
+    // (leaders | followers << 32) places them
+    uint64_t mine = 0;
+    uint32_t mine_a = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxGroups / kBlock; ++i) {
+        mine += (gv[i] & kGrpMask) | (((gv[i] >> kGrpBits) & kGrpMask) << 32);
+        mine_a += static_cast<uint32_t>(gv[i] >> (2 * kGrpBits));
     }
-    if (threadIdx.x == 0) {
-        gpl[ng] = nEL;
-        gpf[ng] = nEF;
+    uint64_t tot;
+    uint64_t x = block_excl_scan64(mine, lds64, &tot);
+    const uint32_t nA = block_sum(mine_a, lds);
+    const uint32_t nEL = static_cast<uint32_t>(tot), nEF = static_cast<uint32_t>(tot >> 32);
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxGroups / kBlock; ++i) {
+        if (i < gper && g0 + i <= ng) {
+            gpl[g0 + i] = static_cast<uint32_t>(x);
+            gpf[g0 + i] = static_cast<uint32_t>(x >> 32);
+        }
+        x += (gv[i] & kGrpMask) | (((gv[i] >> kGrpBits) & kGrpMask) << 32);
     }
     const uint32_t cL = (nEL + kResChunk - 1) / kResChunk;
     const uint32_t nchunks = cL + (nEF + kResChunk - 1) / kResChunk;
@@ -1245,14 +1316,22 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
         st[4] = static_cast<int64_t>(nEL);
         if (nchunks == 0) counters[C_NB + nxt] = N;
     }
-    if (blockIdx.x >= nchunks) return;  // only blocks with work take tickets
+    if (blockIdx.x >= nchunks) return;  // only blocks with work take chunks
+    // Chunks in order: block b takes chunk b first (blocks are dispatched in index order, so every
+    // earlier chunk's block has started), then work tickets after the first min(nchunks, grid) chunks.
     // One chunk in flight behind the current one: chunk c is resolved and its misses staged (LDS) while
     // the previous chunk of this block, whose look-back has had chunk c's time to settle, is appended.
+    const uint32_t first_round = nchunks < gridDim.x ? nchunks : gridDim.x;
     uint32_t pend = 0xffffffffu, pend_miss = 0, buf = 0;
-    for (;;) {
+    for (bool first = true;; first = false) {
         __syncthreads();  // LDS of the previous iteration fully consumed (and the group prefixes written)
+        // (when every chunk was handed out in the first round, no ticket is needed)
         if (threadIdx.x == 0)
-            s_ticket = static_cast<uint32_t>(atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
+            s_ticket = first ? blockIdx.x
+                       : first_round == nchunks
+                           ? 0xffffffffu
+                           : first_round + static_cast<uint32_t>(atomicAdd(
+                                               reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
         __syncthreads();
         const uint32_t chunk = s_ticket;
         const bool have = chunk < nchunks;
@@ -1280,50 +1359,74 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                     else hi = mid;
                 }
                 uint32_t rem = f - gp[lo];
-                const uint32_t blocks_per_group = 1u << (group_log2 - 2);
-                const uint64_t b0 = static_cast<uint64_t>(lo) << (group_log2 - 2);
-                uint64_t blk = b0;
-                for (uint32_t base = 0; base < blocks_per_group; base += kBsumBatch) {
-                    uint32_t cb[kBsumBatch];
+                uint64_t tile = 0;
+                uint32_t nl_t = 0;
+                if (group_log2 <= 4) {
+                    // groups of at most 16 tiles: their tile counts in one round of loads
+                    uint64_t tc[16];
+                    const uint64_t t0 = static_cast<uint64_t>(lo) << group_log2;
+                    const uint32_t tpg = 1u << group_log2;
 #pragma unroll
-                    for (int k = 0; k < kBsumBatch; ++k) {
-                        const uint64_t bb = b0 + base + k;
-                        const uint64_t v = bb < nblocks ? bsum[bb] : 0;
-                        cb[k] = static_cast<uint32_t>(leader ? (v & kGrpMask) : ((v >> kGrpBits) & kGrpMask));
-                    }
+                    for (uint32_t k = 0; k < 16; ++k) tc[k] = (k < tpg && t0 + k < ntiles) ? tcnt[t0 + k] : 0;
                     bool found = false;
 #pragma unroll
-                    for (int k = 0; k < kBsumBatch; ++k) {
+                    for (uint32_t k = 0; k < 16; ++k) {
                         if (found) continue;
-                        if (rem < cb[k]) {
-                            blk = b0 + base + k;
+                        const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
+                        const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
+                        if (rem < own) {
+                            tile = t0 + k;
+                            nl_t = nl1;
                             found = true;
                         } else {
-                            rem -= cb[k];
+                            rem -= own;
                         }
                     }
-                    if (found) break;
-                }
-                uint64_t tc[kWaves];
+                } else {
+                    const uint32_t blocks_per_group = 1u << (group_log2 - 2);
+                    const uint64_t b0 = static_cast<uint64_t>(lo) << (group_log2 - 2);
+                    uint64_t blk = b0;
+                    for (uint32_t base = 0; base < blocks_per_group; base += kBsumBatch) {
+                        uint32_t cb[kBsumBatch];
 #pragma unroll
-                for (int k = 0; k < kWaves; ++k) {
-                    const uint64_t tt = blk * kWaves + k;
-                    tc[k] = tt < ntiles ? tcnt[tt] : 0;
-                }
-                uint64_t tile = blk * kWaves;
-                uint32_t nl_t = 0;
-                bool found = false;
+                        for (int k = 0; k < kBsumBatch; ++k) {
+                            const uint64_t bb = b0 + base + k;
+                            const uint64_t v = bb < nblocks ? bsum[bb] : 0;
+                            cb[k] = static_cast<uint32_t>(leader ? (v & kGrpMask) : ((v >> kGrpBits) & kGrpMask));
+                        }
+                        bool found = false;
 #pragma unroll
-                for (int k = 0; k < kWaves; ++k) {
-                    if (found) continue;
-                    const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
-                    const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
-                    if (rem < own) {
-                        tile = blk * kWaves + k;
-                        nl_t = nl1;
-                        found = true;
-                    } else {
-                        rem -= own;
+                        for (int k = 0; k < kBsumBatch; ++k) {
+                            if (found) continue;
+                            if (rem < cb[k]) {
+                                blk = b0 + base + k;
+                                found = true;
+                            } else {
+                                rem -= cb[k];
+                            }
+                        }
+                        if (found) break;
+                    }
+                    uint64_t tc[kWaves];
+#pragma unroll
+                    for (int k = 0; k < kWaves; ++k) {
+                        const uint64_t tt = blk * kWaves + k;
+                        tc[k] = tt < ntiles ? tcnt[tt] : 0;
+                    }
+                    tile = blk * kWaves;
+                    bool found = false;
+#pragma unroll
+                    for (int k = 0; k < kWaves; ++k) {
+                        if (found) continue;
+                        const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
+                        const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
+                        if (rem < own) {
+                            tile = blk * kWaves + k;
+                            nl_t = nl1;
+                            found = true;
+                        } else {
+                            rem -= own;
+                        }
                     }
                 }
                 sidx = tile * kWTileRows + (leader ? 0u : nl_t) + rem;
@@ -1372,9 +1475,19 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
                 if (n_hit) atomicAdd(reinterpret_cast<unsigned long long *>(st + 5), static_cast<unsigned long long>(n_hit));
             }
         }
-        if (pend != 0xffffffffu) {
+        // A chunk that creates no term has nothing to place: its aggregate (0) is all its successors need,
+        // so only the last chunk (which publishes the gate's total) looks back without misses.
+        if (pend != 0xffffffffu && (pend_miss != 0 || pend == nchunks - 1)) {
             // ---- append the previous chunk: its misses land at N + (misses of earlier chunks) + rank
             const uint32_t pb = buf ^ 1u;
+            // the emitters' rows are requested before the look-back, so both round trips overlap
+            uint64_t t[NW];
+            uint32_t q = 0;
+            if (threadIdx.x < pend_miss) {
+                const uint32_t S = st_row[pb][threadIdx.x];
+                q = BATCH ? member[S] : 0u;
+                load_row<NW>(rows, S, t);
+            }
             if (threadIdx.x < 64) {
                 uint32_t excl = 0;
                 if (pend != 0) {
@@ -1395,10 +1508,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const GateDesc *__restrict__
             __syncthreads();
             const uint64_t first_row = N + s_base, wlo = first_row >> 6;
             if (threadIdx.x < pend_miss) {
-                const uint32_t S = st_row[pb][threadIdx.x];
-                const uint32_t q = BATCH ? member[S] : 0u;
-                uint64_t t[NW];
-                load_row<NW>(rows, S, t);
 #pragma unroll
                 for (int k = 0; k < NW; ++k) t[k] ^= gd.g[k];
                 const uint64_t dest = first_row + threadIdx.x;
@@ -2707,19 +2816,33 @@ struct mpa_engine {
                 uint64_t *gsum_nxt = t_gsum.as<uint64_t>() + ((g + 1) & 1) * gsum_stride;
                 uint16_t *mem = BATCH ? bmember.as<uint16_t>() : nullptr;
                 const Trig *gtrig = BATCH ? btrig.as<Trig>() + g * nbatch : nullptr;
+                SweepCols sc{};
+                sc.nfold = gd.fold_cnt;
+                sc.piv0 = gd.piv0;
+                sc.piv1 = gd.piv1;
+                {
+                    int c = 0;
+                    for (int f = 0; f < gd.fold_cnt; ++f, ++c)
+                        if (c < kColArgs) sc.col[c] = folds[static_cast<size_t>(gd.fold_off + f)];
+                    for (int pv : {gd.piv0, gd.piv1})
+                        if (pv >= 0) {
+                            if (c < kColArgs) sc.col[c] = pv;
+                            ++c;
+                        }
+                    sc.ncol = c;
+                }
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI, BATCH>), dim3(sgrid), dim3(kBlock), 0, stream, gdesc.as<GateDesc>(),
-                                   ig, tr, cols.as<uint64_t>(), colstride(), fold.as<int32_t>() + gd.fold_off,
-                                   gd.fold_cnt, gd.piv0, gd.piv1, rows.as<uint64_t>(), coef.as<double>(),
-                                   t_val.as<double>(), t_item.as<uint32_t>(), f_elead.as<uint64_t>(),
-                                   t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur, group_log2,
-                                   counters.as<uint64_t>(), mem, gtrig);
+                hipLaunchKernelGGL((k_sweep<NW, PAULI, BATCH>), dim3(sgrid), dim3(kBlock),
+                                   BATCH ? static_cast<size_t>(nbatch) * 16 : 0, stream, gd, ig, tr,
+                                   cols.as<uint64_t>(), colstride(), sc, fold.as<int32_t>() + gd.fold_off,
+                                   rows.as<uint64_t>(), coef.as<double>(), t_val.as<double>(), t_item.as<uint32_t>(),
+                                   f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur,
+                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
                 auto *kres = (!BATCH && schr) ? k_resolve<NW, PAULI, 0, 1> : k_resolve<NW, PAULI, BATCH, 0>;
-                hipLaunchKernelGGL(kres, dim3(rgrid), dim3(kBlock), rlds, stream, gdesc.as<GateDesc>(),
-                                   ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
+                hipLaunchKernelGGL(kres, dim3(rgrid), dim3(kBlock), rlds, stream, gd, ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
                                    gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gidx_slots,

@@ -5,7 +5,8 @@ cutoff 10, lower_atol 1e-5) for every pair of
 
 * ``resolve_grid``: k_resolve blocks that take emitter chunks, and
 * ``min_group_log2``: the smallest count group (2^g sweep tiles), which sets how many group prefixes
-  every resolve block scans against how many sweep-block sums an emitter reads to find its tile,
+  every resolve block scans against how many tile counts (groups of up to 16 tiles) or sweep-block
+  sums (larger groups) an emitter reads to find its tile,
 
 checks that every pair gives the same <O> to the bit (the results do not depend on launch
 parameters), and writes the table and the fastest pair to ``tuning.json`` next to the library, which
@@ -28,8 +29,9 @@ import numpy as np
 from . import _engine, workloads
 from .circuit import expand_monomials
 
-RESOLVE_GRIDS = (640, 1024, 1280, 2048, 2560)
-MIN_GROUP_LOG2 = (4, 5, 6, 7)
+RESOLVE_GRIDS = (320, 640, 1280, 2560)
+MIN_GROUP_LOG2 = (2, 3, 4, 5, 6)
+DEFAULT = (1280, 4)  # the engine's built-in launch parameters
 THETA_H = (0.1, 0.4, 0.8)
 
 
@@ -60,7 +62,7 @@ def tune(L: int = 10, steps: int = 20, repeats: int = 2, progress=None) -> dict:
     _engine.load_library()
     inputs = _inputs(L, steps)
     table, ref = [], None
-    _step(L, inputs, (1280, 6))  # warm the device memory cache
+    _step(L, inputs, DEFAULT)  # warm the device memory cache
     for grid, g in itertools.product(RESOLVE_GRIDS, MIN_GROUP_LOG2):
         times = []
         for _ in range(repeats):
@@ -75,7 +77,7 @@ def tune(L: int = 10, steps: int = 20, repeats: int = 2, progress=None) -> dict:
         if progress:
             progress(row)
     best = min(table, key=lambda r: r["ms"])
-    default = next(r for r in table if r["resolve_grid"] == 1280 and r["min_group_log2"] == 6)
+    default = next(r for r in table if (r["resolve_grid"], r["min_group_log2"]) == DEFAULT)
     return {"workload": f"{L}x{L} kicked Ising, {steps} Trotter steps, theta_h {THETA_H} batched, cutoff 10, "
                         "lower_atol 1e-5",
             "best": best, "default": default, "table": table}
