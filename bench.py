@@ -19,8 +19,8 @@ back and appends each new term.  `hbm_GBps` is what the kernel really moves (PMC
 other half of the metric, the full-angle gradient's time against one forward replay, on the 64-qubit
 ring brickwork (2304 angles), measured after the timed steps; `batch` times a parameter sweep (sets x
 eight observables, 10 Trotter steps) batched against one propagation per pair.  The CPU baseline is the oracle
-(a restatement of the reference algorithm) on a bounded sample of the same workload, run on the host's
-thread budget (its serial rate beside it).
+(a restatement of the reference algorithm) on the timed workload itself, run on the host's thread
+budget, its stores and <O> compared bit for bit with the GPU step's (the serial rate on a sample beside it).
 
 `--schrodinger Q` measures the Schrodinger picture instead (state terms of at most Q qubits to start,
 one propagation at a time, default theta_h 0.8: at 0.1 and 0.4 the 100-qubit state outgrows the GPU).
@@ -65,7 +65,7 @@ def parse_args():
     ap.add_argument("--traffic-from", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "FETCH", "WRITE"),
                     help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
     ap.add_argument("--traffic-out", default=str(TRAFFIC_FILE))
-    ap.add_argument("--cpu-sample-steps", type=int, default=6)
+    ap.add_argument("--cpu-sample-steps", type=int, default=4, help="Trotter steps of the serial CPU sample")
     ap.add_argument("--unbatched", action="store_true",
                     help="time one propagation per theta_h (default: the three as members of one store)")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the unbatched steps for comparison")
@@ -157,19 +157,37 @@ def _host_threads() -> int:
         return max(1, os.cpu_count() or 1)
 
 
-def cpu_baseline(L, sample_steps):
-    """The oracle on theta_h = 0.4 (the Schrodinger mode: its first theta_h) truncated to
-    `sample_steps` Trotter steps: the thread-parallel restatement over the host's thread budget, and
-    the serial one (1 core) on the first 4 steps beside it."""
+def _digests(words, coeffs) -> tuple[str, str]:
+    import hashlib
+
+    return (hashlib.sha256(np.ascontiguousarray(words, dtype=np.uint64).tobytes()).hexdigest(),
+            hashlib.sha256(np.ascontiguousarray(coeffs, dtype=np.float64).tobytes()).hexdigest())
+
+
+def gpu_member_results(L, inputs):
+    """One batched step of the timed workload; per theta_h member its store digests and <O> bits."""
+    _, evs, sims = run_gpu_step_batched(L, inputs, profile=False)
+    w, c, m = sims[0].rows()
+    out = []
+    for q, ev in enumerate(evs):
+        sel = m == q
+        out.append((*_digests(w[sel], c[sel]), float(ev).hex()))
+    return out
+
+
+def cpu_baseline(L, trotter, sample_steps, gpu_results=None):
+    """The oracle on the timed workload itself (every theta_h, all Trotter steps) on the host's thread
+    budget, its stores and <O> compared with the GPU step's; the serial oracle (1 core) on a sample
+    beside it: theta_h = 0.4, the first `sample_steps` Trotter steps.  The Schrodinger mode times only
+    its sample."""
     import oracle
     from monoprop_amd import workloads
 
     n = L * L
-    th = 0.4 if SCHRODINGER_QUBITS is None else THETA_H[0]
     schr = None if SCHRODINGER_QUBITS is None else 2 * SCHRODINGER_QUBITS
     picture = "" if schr is None else f", Schrodinger picture (state terms of <= {SCHRODINGER_QUBITS} qubits)"
 
-    def run(steps, threads):
+    def run(th, steps, threads):
         circ = workloads.kicked_ising_lattice(L, steps, theta_h=th)
         monos, gcs, pm, _ = layers_of(circ)
         oracle.set_threads(threads)
@@ -181,22 +199,45 @@ def cpu_baseline(L, sample_steps):
             dt = time.perf_counter() - t0
         finally:
             oracle.set_threads(1)
-        t = int(o.gate_stats()[0].sum())
-        return t / dt, (f"{L}x{L} kicked Ising theta_h={th}, first {steps} of 20 Trotter steps ({len(monos)} gates, "
-                        f"{t} term-gate applications, {dt:.1f} s){picture}")
+        return int(o.gate_stats()[0].sum()), dt, o
 
     threads = _host_threads()
-    value, sample = run(sample_steps, threads)
-    serial, serial_sample = run(min(sample_steps, 4), 1)
-    return {
+    th_sample = 0.4 if SCHRODINGER_QUBITS is None else THETA_H[0]
+    if SCHRODINGER_QUBITS is None:
+        total_t, total_s, same = 0, 0.0, True
+        for q, th in enumerate(THETA_H):
+            t, dt, o = run(th, trotter, threads)
+            total_t += t
+            total_s += dt
+            if gpu_results is not None:
+                mine = (*_digests(*o.rows()), float(o.expectation()).hex())
+                same = same and mine == gpu_results[q]
+            del o
+        value = total_t / total_s
+        sample = (f"the timed workload: {L}x{L} kicked Ising, {trotter} Trotter steps, theta_h "
+                  f"{{{','.join(str(t) for t in THETA_H)}}} ({total_t} term-gate applications, {total_s:.1f} s), "
+                  f"oracle restatement on {threads} threads")
+    else:
+        t, dt, _ = run(th_sample, sample_steps, threads)
+        value, same = t / dt, None
+        sample = (f"{L}x{L} kicked Ising theta_h={th_sample}, first {sample_steps} of {trotter} Trotter steps "
+                  f"({t} term-gate applications, {dt:.1f} s){picture}, oracle restatement on {threads} threads")
+    st, sdt, _ = run(th_sample, sample_steps, 1)
+    out = {
         "value": value,
         "unit": "term-gate applications/s",
         "cores": threads,
         "kind": "port",
-        "sample": sample + f", oracle restatement on {threads} threads",
+        "sample": sample,
         "cpu_model": _cpu_model(),
-        "serial": {"value": serial, "cores": 1, "sample": serial_sample + ", serial oracle"},
+        "serial": {"value": st / sdt, "cores": 1,
+                   "sample": f"theta_h={th_sample}, first {sample_steps} of {trotter} Trotter steps ({st} "
+                             f"term-gate applications, {sdt:.1f} s){picture}, serial oracle"},
     }
+    if same is not None and gpu_results is not None:
+        # every member's rows, coefficients (SHA-256) and <O> bits equal to the GPU step's
+        out["matches_gpu"] = bool(same)
+    return out
 
 
 def gradient_measurement(reps: int = 3):
@@ -340,6 +381,22 @@ def measured_traffic(kernel):
         return None
 
 
+def propagate_merge_fractions(prof):
+    """The propagate (k_sweep) and merge (k_resolve) kernels together: algorithmic bytes and PMC
+    traffic over their summed device time, each as a fraction of the HBM peak."""
+    ks = [k for k in ("k_sweep", "k_resolve") if k in prof]
+    ms = sum(prof[k]["ms"] for k in ks)
+    if not ks or ms <= 0:
+        return None
+    alg = sum(prof[k]["bytes"] for k in ks) / (ms * 1e-3) / 1e9
+    out = {"ms": round(ms, 3), "algorithmic_GBps": round(alg, 1), "algorithmic_frac": round(alg / HBM_PEAK_GBS, 4)}
+    traffic = [measured_traffic(k) for k in ks] if SCHRODINGER_QUBITS is None else [None]
+    if all(t is not None for t in traffic):
+        hbm = sum(t * prof[k]["launches"] for t, k in zip(traffic, ks)) / (ms * 1e-3) / 1e9
+        out.update({"hbm_GBps": round(hbm, 1), "hbm_frac": round(hbm / HBM_PEAK_GBS, 4)})
+    return out
+
+
 def main():
     global THETA_H, SCHRODINGER_QUBITS
     args = parse_args()
@@ -450,9 +507,13 @@ def main():
         },
         "kernels": {
             k: {"ms": round(v["ms"], 3), "GB/s": round(v["bytes"] / (v["ms"] * 1e6), 1) if v["ms"] > 0 else 0.0,
-                "launches": v["launches"]}
+                "launches": v["launches"],
+                "hbm_GBps": (round(measured_traffic(k) / (v["ms"] / v["launches"] * 1e-3) / 1e9, 1)
+                             if SCHRODINGER_QUBITS is None and measured_traffic(k) and v["launches"] else None)}
             for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
         },
+        "kernel_ms_per_step": round(kern_ms, 3),
+        "propagate_merge": propagate_merge_fractions(prof),
         "all_kernels_GBps": kern_bytes / (kern_ms * 1e6) if kern_ms > 0 else 0.0,
         "profiled_step": {"propagate_ms": round(prof_wall_ms, 2), "kernel_ms": round(kern_ms, 2)},
     }
@@ -465,7 +526,8 @@ def main():
     if args.batch_sets > 0:
         line["batch"] = batch_measurement(args.batch_sets, solo=not args.batch_no_solo)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.lattice, args.cpu_sample_steps)
+        gpu_res = None if args.unbatched else gpu_member_results(args.lattice, inputs)
+        line["cpu_baseline"] = cpu_baseline(args.lattice, args.trotter_steps, args.cpu_sample_steps, gpu_res)
     print(json.dumps(line))
 
 
