@@ -104,6 +104,14 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
  * parameter count; 0 with no graph). */
 int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation): <O> of a contracted
+ * Heisenberg lone propagator (no stored graph), the value mpa_expectation_value(e, NULL, 0, out)
+ * gives, to the bit.  It allocates nothing: the ordered sum's counts, terms and result live in one
+ * static device array per GPU (about 8 MiB), which holds at most 2^20 nonzero <O> terms (more is an
+ * error, -1).  Every call on a GPU shares that GPU's array: calls that overlap on one GPU race. */
+int mpa_expectation_value_noalloc(mpa_engine *e, double *out);
+
 /* Replaces MonomialPropagator::expectation_value_and_gradient (MonomialPropagator.inl:1043-1050,
  * MPFunctions.cpp:207-246): value and d<O>/d params[p] for every p. */
 int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int64_t n_params,

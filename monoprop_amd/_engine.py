@@ -53,6 +53,7 @@ _SIGNATURES = {
     "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
     "mpa_expectation_value": (C.c_int, [_P, _f64p, C.c_int64, _f64p]),
+    "mpa_expectation_value_noalloc": (C.c_int, [_P, _f64p]),
     "mpa_expectation_value_and_gradient": (C.c_int, [_P, _f64p, C.c_int64, _f64p, _f64p]),
     "mpa_contract": (C.c_int, [_P, _f64p, C.c_int64, C.c_int, _f64p]),
     "mpa_size": (C.c_int, [_P, _i64p]),

@@ -1616,6 +1616,16 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_counts(uint32_t *__restrict
     if (threadIdx.x == 0) *total = carry;
 }
 
+// Scratch of the allocation-free <O> (mpa_expectation_value_noalloc): the ordered sum's block counts
+// and total, its compacted terms and its result, in one static array per GPU.  Host code never uses the
+// array's name as an address: the library looks the address up once per device at engine set-up
+// (noalloc_scratch) and passes that.
+constexpr uint64_t kNoAllocTerms = uint64_t{1} << 20;  // nonzero <O> terms one call can add (8 MiB)
+constexpr uint32_t kNoAllocBlocks = 2048;              // ordered-sum blocks (their counts and the total)
+constexpr size_t kNoAllocCountBytes = (kNoAllocBlocks + 2) * 4 + 8;
+constexpr size_t kNoAllocBytes = ((kNoAllocCountBytes + 255) / 256) * 256 + kNoAllocTerms * 8 + 256;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 // Adds the K compacted terms in order: out[0] (no members), or out[q] for member q = global thread q.
 constexpr int kSeqStage = 4096;
 
@@ -1920,6 +1930,25 @@ class BlockCache {
     std::mutex mu_;
     std::vector<Blk> free_;
 };
+
+// Device address of `scratch` on each GPU (each GPU holds its own copy), looked up once per device
+// with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+void *noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw ArgError("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || ptr == nullptr)
+            throw HipFailure(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
 
 // A device array owned by one engine: allocations and frees are ordered on the engine's stream (the
 // null stream once the engine has synchronised and destroyed it).
@@ -2617,38 +2646,65 @@ struct mpa_engine {
     // The reference's ordered sum over rows [0, N) (k_terms): MODE TERM_PAIRED adds phase * a[i] over the
     // fully paired rows, TERM_DOT adds a[i] * b[i] over all rows.  out[0], or out[q] per member q of
     // `member` (nb members).
+    // With `fixed` (a device buffer of kNoAllocBytes, one member), the sum allocates nothing: its counts,
+    // terms and result live there, and more than kNoAllocTerms nonzero terms is an error.
     template <int NW, int PAULI, int MODE>
-    void ordered_sum(const double *a, const double *b, const uint16_t *member, uint32_t nb, double *out) {
+    void ordered_sum(const double *a, const double *b, const uint16_t *member, uint32_t nb, double *out,
+                     unsigned char *fixed = nullptr) {
         for (uint32_t q = 0; q < nb; ++q) out[q] = 0.0;
         if (N == 0) return;
-        const uint64_t nblk = std::min<uint64_t>((N + kBlock - 1) / kBlock, 2048);
+        const uint64_t nblk = std::min<uint64_t>((N + kBlock - 1) / kBlock, kNoAllocBlocks);
         const uint64_t seg = ((N + nblk - 1) / nblk + kBlock - 1) / kBlock * kBlock;
         const uint32_t grid = static_cast<uint32_t>((N + seg - 1) / seg);
-        sum_blk.reserve((static_cast<size_t>(grid) + 2) * 4 + 8);
-        uint64_t *total = reinterpret_cast<uint64_t *>(sum_blk.as<uint32_t>() + ((grid + 1) & ~1u));
+        uint32_t *blk;
+        double *vals, *res;
+        if (fixed) {
+            blk = reinterpret_cast<uint32_t *>(fixed);
+            vals = reinterpret_cast<double *>(fixed + ((kNoAllocCountBytes + 255) / 256) * 256);
+            res = vals + kNoAllocTerms;
+        } else {
+            sum_blk.reserve((static_cast<size_t>(grid) + 2) * 4 + 8);
+            blk = sum_blk.as<uint32_t>();
+        }
+        uint64_t *total = reinterpret_cast<uint64_t *>(blk + ((grid + 1) & ~1u));
         const uint64_t *occ_s = occ_dev.as<uint64_t>();
         hipLaunchKernelGGL((k_terms<NW, PAULI, MODE, 0>), dim3(grid), dim3(kBlock), 0, stream, rows.as<uint64_t>(), a,
-                           b, member, N, seg, occ_s, sum_blk.as<uint32_t>(), static_cast<double *>(nullptr),
+                           b, member, N, seg, occ_s, blk, static_cast<double *>(nullptr),
                            static_cast<uint16_t *>(nullptr));
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanBlock), 0, stream, sum_blk.as<uint32_t>(), grid, total);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanBlock), 0, stream, blk, grid, total);
         HIP_TRY(hipGetLastError());
-        uint64_t K = 0;
-        HIP_TRY(hipMemcpyAsync(&K, total, 8, hipMemcpyDeviceToHost, stream));
+        // the count goes to the engine's pinned words (no staging buffer is needed for the copy)
+        HIP_TRY(hipMemcpyAsync(h_counters + C_COUNT, total, 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        const uint64_t K = static_cast<uint64_t>(h_counters[C_COUNT]);
         if (K == 0) return;
-        sum_v.reserve(K * 8);
-        if (member) sum_m.reserve(K * 2);
+        if (fixed) {
+            if (K > kNoAllocTerms)
+                throw ArgError("the operator's <O> has " + std::to_string(K) + " nonzero terms, more than the " +
+                               std::to_string(kNoAllocTerms) + " the allocation-free entry point holds; use "
+                               "mpa_expectation_value");
+        } else {
+            sum_v.reserve(K * 8);
+            if (member) sum_m.reserve(K * 2);
+            sum_out.reserve(static_cast<size_t>(nb) * 8);
+            vals = sum_v.as<double>();
+            res = sum_out.as<double>();
+        }
         hipLaunchKernelGGL((k_terms<NW, PAULI, MODE, 1>), dim3(grid), dim3(kBlock), 0, stream, rows.as<uint64_t>(), a,
-                           b, member, N, seg, occ_s, sum_blk.as<uint32_t>(), sum_v.as<double>(),
+                           b, member, N, seg, occ_s, blk, vals,
                            member ? sum_m.as<uint16_t>() : static_cast<uint16_t *>(nullptr));
         HIP_TRY(hipGetLastError());
-        sum_out.reserve(static_cast<size_t>(nb) * 8);
-        hipLaunchKernelGGL(k_seq_sum, dim3(member ? (nb + kBlock - 1) / kBlock : 1), dim3(kBlock), 0, stream,
-                           sum_v.as<double>(), member ? sum_m.as<uint16_t>() : static_cast<uint16_t *>(nullptr), K,
-                           nb, sum_out.as<double>());
+        hipLaunchKernelGGL(k_seq_sum, dim3(member ? (nb + kBlock - 1) / kBlock : 1), dim3(kBlock), 0, stream, vals,
+                           member ? sum_m.as<uint16_t>() : static_cast<uint16_t *>(nullptr), K, nb, res);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(out, sum_out.p, static_cast<size_t>(nb) * 8, hipMemcpyDeviceToHost, stream));
+        if (fixed) {
+            HIP_TRY(hipMemcpyAsync(h_counters + C_COUNT, res, 8, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            std::memcpy(out, h_counters + C_COUNT, 8);
+            return;
+        }
+        HIP_TRY(hipMemcpyAsync(out, res, static_cast<size_t>(nb) * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
     }
 
@@ -3102,6 +3158,18 @@ struct RunExpect {
     }
 };
 
+struct RunExpectFixed {
+    mpa_engine *e;
+    unsigned char *scratch_dev;
+    double *out;
+    template <int NW, int PAULI>
+    void operator()() {
+        double r = 0.0;
+        e->ordered_sum<NW, PAULI, TERM_PAIRED>(e->coef.as<double>(), nullptr, nullptr, 1, &r, scratch_dev);
+        *out = e->core + r;
+    }
+};
+
 // Schrodinger <O>: without a graph the state against the observable's rows; with one, the observable
 // replayed newest layer first and dotted with the state (the reference's functional).
 struct RunSchrExpect {
@@ -3256,7 +3324,8 @@ std::unique_ptr<mpa_engine> new_engine(int basis, int num_modes, uint32_t cutoff
     e->device = device;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&e->h_counters), C_COUNT * 8, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&e->h_counters), (C_COUNT + 1) * 8, hipHostMallocDefault));
+    (void)noalloc_scratch(device);  // the scratch array's address on this device, looked up at set-up
     e->occ.assign(mpa::kMaxWords, 0);
     for (int64_t k = 0; k < n_state; ++k) {
         const int j = initial_state[k];
@@ -3284,7 +3353,7 @@ std::unique_ptr<mpa_engine> clone_config(const mpa_engine *e) {
     o->min_group_log2 = e->min_group_log2;
     HIP_TRY(hipSetDevice(o->device));
     HIP_TRY(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&o->h_counters), C_COUNT * 8, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&o->h_counters), (C_COUNT + 1) * 8, hipHostMallocDefault));
     o->counters.reserve(C_COUNT * 8);
     HIP_TRY(hipMemsetAsync(o->counters.p, 0, C_COUNT * 8, o->stream));
     o->occ_dev.reserve(mpa::kMaxWords * 8);
@@ -3997,6 +4066,19 @@ int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params,
         }
         const double *c = evaluated_coeffs(e, params, n_params, false);
         RunExpect ev{e, c, out};
+        e->dispatch(ev);
+    });
+}
+
+int mpa_expectation_value_noalloc(mpa_engine *e, double *out) {
+    return guarded([&] {
+        require_single(e);
+        if (e->schr) throw ArgError("the allocation-free <O> serves the Heisenberg picture only");
+        if (!e->layers.empty())
+            throw ArgError("the allocation-free <O> needs a contracted operator (no stored graph); use "
+                           "mpa_expectation_value");
+        HIP_TRY(hipSetDevice(e->device));
+        RunExpectFixed ev{e, static_cast<unsigned char *>(noalloc_scratch(e->device)), out};
         e->dispatch(ev);
     });
 }
