@@ -264,6 +264,7 @@ def gradient_measurement(reps: int = 3):
         grd.append(time.perf_counter() - t0)
     f, g = min(fwd), min(grd)
     cos_records, rot_records = sim.graph_size()
+    segs = sim.gradient_segments()
     return {
         "workload": "64-qubit ring brickwork, 12 layers, 2304 angles, O=mean ZZ, cutoff 8, lower_atol 1e-6",
         "terms": sim.size(),
@@ -272,6 +273,11 @@ def gradient_measurement(reps: int = 3):
         "forward_ms": f * 1e3,
         "gradient_ms": g * 1e3,
         "gradient_over_forward": g / f,
+        # device memory of the adjoint beyond the coefficient vector: the graph (records) and, per
+        # gradient, the checkpoints and one segment's pre-layer values
+        "memory_GB": {"graph": sim.graph_memory_bytes() / 1e9, "checkpoints": segs["checkpoint_bytes"] / 1e9,
+                      "snapshots": segs["snapshot_bytes"] / 1e9, "coefficient_vector": sim.size() * 8 / 1e9},
+        "segments": segs["segments"],
     }
 
 

@@ -104,6 +104,20 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
  * parameter count; 0 with no graph). */
 int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out);
 
+/* No counterpart in the reference (it keeps no snapshots: its adjoint un-evolves with 1/cos,
+ * Evolution.cpp:355-387).  Memory of the exact adjoint: budget_bytes bounds the coefficient-vector
+ * checkpoints plus one segment's pre-layer values (default 16 GiB; the gradient replays each earlier
+ * segment once more, and its result does not depend on the budget; 0 asks for one segment per
+ * layer, up to 256).  keep != 0 makes
+ * mpa_expectation_value run that checkpointed forward, so that mpa_expectation_value_and_gradient
+ * at the same parameters (bit for bit) starts from it instead of replaying again. */
+int mpa_set_gradient_checkpointing(mpa_engine *e, int keep, int64_t budget_bytes);
+
+/* No counterpart in the reference: the last gradient's segment count and the bytes its checkpoints
+ * and snapshots hold. */
+int mpa_gradient_segments(mpa_engine *e, int64_t *segments, int64_t *checkpoint_bytes,
+                          int64_t *snapshot_bytes);
+
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid allocation): <O> of a contracted
  * Heisenberg lone propagator (no stored graph), the value mpa_expectation_value(e, NULL, 0, out)

@@ -54,6 +54,8 @@ _SIGNATURES = {
     "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
     "mpa_expectation_value": (C.c_int, [_P, _f64p, C.c_int64, _f64p]),
     "mpa_expectation_value_noalloc": (C.c_int, [_P, _f64p]),
+    "mpa_set_gradient_checkpointing": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "mpa_gradient_segments": (C.c_int, [_P, _i64p, _i64p, _i64p]),
     "mpa_expectation_value_and_gradient": (C.c_int, [_P, _f64p, C.c_int64, _f64p, _f64p]),
     "mpa_contract": (C.c_int, [_P, _f64p, C.c_int64, C.c_int, _f64p]),
     "mpa_size": (C.c_int, [_P, _i64p]),
@@ -369,6 +371,18 @@ class DeviceSimulator(_EngineHandle):
         _check(load_library().mpa_expectation_value(self._h, _ptr(th, C.c_double), len(list(parameters)),
                                                     C.byref(out)))
         return out.value
+
+    def set_gradient_checkpointing(self, keep: bool, budget_bytes: int = 16 << 30) -> None:
+        """Memory of the exact adjoint (checkpoints + one segment's pre-layer values, at most
+        ``budget_bytes`` when possible); with ``keep`` every expectation_value runs the checkpointed
+        forward so that a gradient at the same parameters starts from it."""
+        _check(load_library().mpa_set_gradient_checkpointing(self._h, 1 if keep else 0, int(budget_bytes)))
+
+    def gradient_segments(self) -> dict[str, int]:
+        """Segments of the last gradient and the device bytes of its checkpoints and snapshots."""
+        seg, ck, sn = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(load_library().mpa_gradient_segments(self._h, C.byref(seg), C.byref(ck), C.byref(sn)))
+        return {"segments": seg.value, "checkpoint_bytes": ck.value, "snapshot_bytes": sn.value}
 
     def expectation_value_and_gradient(self, parameters):
         params = list(parameters)

@@ -2,9 +2,12 @@
 
 ``expectation(prop, theta)`` returns ``<O>(theta)`` of a propagator whose graph was built with
 ``prop.build_graph(circuit)`` as a differentiable scalar tensor: the forward pass is the engine's
-replay of the recorded layers, the backward pass is its exact adjoint sweep over the same layers
-(pre-layer snapshots, no ``1/cos`` recovery), so ``theta.grad`` equals
-``prop.expectation_value_and_gradient(theta)[1]`` scaled by the incoming gradient.  This is the
+checkpointed replay of the recorded layers (it keeps the coefficient vector at segment starts and the
+last segment's pre-layer values), the backward pass its exact adjoint sweep over the same layers,
+which starts from that forward (the parameters are compared bit for bit) and replays each earlier
+segment once more for its pre-layer values (no ``1/cos`` recovery), so ``theta.grad`` equals
+``prop.expectation_value_and_gradient(theta)[1]`` scaled by the incoming gradient.  Without a
+gradient to compute (``torch.no_grad()`` or ``theta`` not requiring one) only the replay runs.  This is the
 engine-side counterpart of the reference's ``expval_and_grad`` (``MPFunctions.cpp:207-246``,
 ``monomial_propagator.py:352-403``).
 
@@ -21,15 +24,17 @@ import torch
 class _GraphExpectation(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta: torch.Tensor, prop) -> torch.Tensor:
-        params = theta.detach().to("cpu", torch.float64).numpy()
-        value, grad = prop.expectation_value_and_gradient(params)
-        ctx.save_for_backward(torch.as_tensor(np.asarray(grad, dtype=np.float64)).to(theta.device))
+        params = theta.detach().to("cpu", torch.float64).numpy().copy()
+        prop._simulator.set_gradient_checkpointing(True)
+        value = prop.expectation_value(params)
+        ctx.prop, ctx.params, ctx.device = prop, params, theta.device
         return torch.tensor(value, dtype=theta.dtype, device=theta.device)
 
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
-        (grad,) = ctx.saved_tensors
-        return grad_out.to(grad.dtype) * grad, None
+        _, grad = ctx.prop.expectation_value_and_gradient(ctx.params)
+        g = torch.as_tensor(np.asarray(grad, dtype=np.float64)).to(ctx.device)
+        return grad_out.to(g.dtype) * g, None
 
 
 def expectation(prop, theta: torch.Tensor) -> torch.Tensor:

@@ -1658,16 +1658,17 @@ __global__ __launch_bounds__(kBlock) void k_seq_sum(const double *__restrict__ v
 
 // One graph layer replayed at angle alpha (reference evolve_step, Evolution.cpp:471-512): cos-only
 // rows c *= k; rotations c_S = c_S k + (s (-ph)) c_T, c_T = c_T k + (s ph) c_S from the pre-layer values.
-// With `snap_c` / `snap_r` set, the pre-layer values are kept for the adjoint sweep.
+// With `snap_c` / `snap_r` set, the pre-layer values are kept for the adjoint sweep, indexed from the
+// segment's first cos-only record sc0 and first rotation sr0.
 __global__ __launch_bounds__(kBlock) void k_replay(const uint32_t *__restrict__ gcos, const uint2 *__restrict__ grot,
                                                    uint64_t cb, uint64_t ncos, uint64_t rb, uint64_t nrot, double k,
                                                    double s, double *__restrict__ c, double *__restrict__ snap_c,
-                                                   double *__restrict__ snap_r) {
+                                                   double *__restrict__ snap_r, uint64_t sc0, uint64_t sr0) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i < ncos) {
         const uint32_t r = gcos[cb + i];
         const double v = c[r];
-        if (snap_c) snap_c[cb + i] = v;
+        if (snap_c) snap_c[cb + i - sc0] = v;
         c[r] = v * k;
     } else if (i < ncos + nrot) {
         const uint64_t j = rb + (i - ncos);
@@ -1676,8 +1677,8 @@ __global__ __launch_bounds__(kBlock) void k_replay(const uint32_t *__restrict__ 
         const int ph = (e.y & kPhaseBit) ? -1 : 1;
         const double vS = c[S], vT = c[T];
         if (snap_r) {
-            snap_r[2 * j] = vS;
-            snap_r[2 * j + 1] = vT;
+            snap_r[2 * (j - sr0)] = vS;
+            snap_r[2 * (j - sr0) + 1] = vT;
         }
         c[S] = vS * k + s * static_cast<double>(-ph) * vT;
         c[T] = vT * k + s * static_cast<double>(ph) * vS;
@@ -1691,21 +1692,21 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(const uint32_t *__restrict__
                                                     uint64_t cb, uint64_t ncos, uint64_t rb, uint64_t nrot, double k,
                                                     double s, const double *__restrict__ snap_c,
                                                     const double *__restrict__ snap_r, double *__restrict__ lam,
-                                                    double *__restrict__ partial, uint64_t pb) {
+                                                    double *__restrict__ partial, uint64_t pb, uint64_t sc0, uint64_t sr0) {
     __shared__ double red[kBlock];
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     double d = 0.0;
     if (i < ncos) {
         const uint32_t r = gcos[cb + i];
         const double l = lam[r];
-        d = l * (-2.0 * s * snap_c[cb + i]);
+        d = l * (-2.0 * s * snap_c[cb + i - sc0]);
         lam[r] = l * k;
     } else if (i < ncos + nrot) {
         const uint64_t j = rb + (i - ncos);
         const uint2 e = grot[j];
         const uint32_t S = e.x, T = e.y & ~kPhaseBit;
         const double ph = (e.y & kPhaseBit) ? -1.0 : 1.0;
-        const double cs = snap_r[2 * j], ct = snap_r[2 * j + 1];
+        const double cs = snap_r[2 * (j - sr0)], ct = snap_r[2 * (j - sr0) + 1];
         const double ls = lam[S], lt = lam[T];
         d = ls * (-2.0 * s * cs - 2.0 * k * ph * ct);
         d += lt * (-2.0 * s * ct + 2.0 * k * ph * cs);
@@ -2034,6 +2035,15 @@ struct mpa_engine {
         snap_r{&stream}, lam{&stream}, gpartial{&stream}, goffs{&stream}, gdl{&stream};
     uint64_t base_n = 0;
     uint64_t gcos_total = 0, grot_total = 0;  // records stored
+    // Checkpointed adjoint: coefficient vectors at segment starts; the byte budget for them plus one
+    // segment's snapshots; with keep_ckpt, mpa_expectation_value runs the checkpointed forward and a
+    // following gradient at the same parameters (ckpt_params, to the bit) starts from it.
+    DevArray ckpt{&stream};
+    int64_t grad_budget = int64_t{16} << 30;
+    bool keep_ckpt = false, ckpt_valid = false;
+    int ckpt_kind = 0;
+    std::vector<double> ckpt_params;
+    double ckpt_value = 0.0;
     bool building = false;                    // seedc tracks capacity while a graph is built
 
     // Fused pipeline state: global row index; per 1024-term sweep tile a region of emitter rows (t_item)
@@ -2293,6 +2303,19 @@ struct mpa_engine {
         std::swap(a.bytes, b.bytes);
     }
 
+    // Move `a` into an array of exactly `keep` bytes (its first `keep` bytes), when that saves a tenth.
+    void shrink_keep(DevArray &a, size_t keep) {
+        if (a.bytes <= keep + keep / 10 + 4096) return;
+        DevArray b{&stream};
+        size_t want = std::max<size_t>((keep + 4095) & ~size_t{4095}, 4096);
+        b.p = BlockCache::get().take(want, stream);
+        b.bytes = want;
+        if (a.p && keep) HIP_TRY(hipMemcpyAsync(b.p, a.p, std::min(keep, a.bytes), hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::swap(a.p, b.p);
+        std::swap(a.bytes, b.bytes);
+    }
+
     void sync_counters() {
         HIP_TRY(hipMemcpyAsync(h_counters, counters.p, C_COUNT * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -2382,8 +2405,11 @@ struct mpa_engine {
             const uint64_t *occ_s = schr ? occ_dev.as<uint64_t>() : nullptr;  // Schrodinger: fresh rows scored
             GraphRec grec{nullptr, nullptr};
             if (graph && nA > 0) {
-                gcos_ub += nA;
-                grot_ub += nA;
+                // every anticommuting term becomes at most one record (cos-only, or an endpoint of a
+                // rotation), on top of the records written so far (counted on the device, read with
+                // the gate's counts above)
+                gcos_ub = static_cast<uint64_t>(h_counters[C_GCOS]) + nA;
+                grot_ub = static_cast<uint64_t>(h_counters[C_GROT]) + nA;
                 grow_keep(g_cos, gcos_ub * 4);
                 grow_keep(g_rot, grot_ub * 8);
                 grec = GraphRec{g_cos.as<uint32_t>(), g_rot.as<uint2>()};
@@ -2488,8 +2514,7 @@ struct mpa_engine {
         }
     }
 
-    // Replay the stored graph at `params` into `rep` (evolve_operator, Evolution.cpp:529-540); with
-    // `snapshots` the pre-layer values of every record are kept for the adjoint sweep.
+    // Replay the stored graph at `params` into `rep` (evolve_operator, Evolution.cpp:529-540).
     //   kOperator   (Heisenberg): the operator from the coefficients the graph started at, layers in
     //               processing order, +alpha.
     //   kState      (Schrodinger contract_partially, MonomialPropagator.inl:1069-1083): the state,
@@ -2498,7 +2523,7 @@ struct mpa_engine {
     //               coefficients, which the caller has put in `rep`, newest layer first (the order the
     //               reference keeps Schrodinger layers in, MPGraph.h:41-43), +alpha.
     enum ReplayKind { kOperator, kState, kObservable };
-    void replay(const double *params, bool snapshots, ReplayKind kind = kOperator) {
+    void replay_init(ReplayKind kind) {
         rep.reserve(std::max<uint64_t>(N, 1) * 8);
         if (kind == kOperator) {
             if (base_n) HIP_TRY(hipMemcpyAsync(rep.p, base.p, base_n * 8, hipMemcpyDeviceToDevice, stream));
@@ -2506,32 +2531,145 @@ struct mpa_engine {
         } else if (kind == kState && N) {
             HIP_TRY(hipMemcpyAsync(rep.p, coef.p, N * 8, hipMemcpyDeviceToDevice, stream));
         }
-        if (snapshots) {
-            snap_c.reserve(std::max<uint64_t>(gcos_total, 1) * 8);
-            snap_r.reserve(std::max<uint64_t>(grot_total, 1) * 16);
-        }
-        const double phase = kind == kState ? -1.0 : 1.0;
-        const size_t nl = layers.size();
-        for (size_t i = 0; i < nl; ++i) {
-            const LayerInfo &ly = layers[kind == kObservable ? nl - 1 - i : i];
-            const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
-            if (nc + nr == 0) continue;
-            const double alpha = phase * params[ly.param_index] * ly.gen_coeff;
-            double sn, cs;
-            sin_cos(2 * alpha, &sn, &cs);
-            const uint32_t blocks = static_cast<uint32_t>((nc + nr + kBlock - 1) / kBlock);
-            hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, stream, g_cos.as<uint32_t>(),
-                               g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr, cs, sn, rep.as<double>(), snapshots ? snap_c.as<double>() : nullptr,
-                               snapshots ? snap_r.as<double>() : nullptr);
-            HIP_TRY(hipGetLastError());
+    }
+    size_t replay_layer_at(ReplayKind kind, size_t pos) const {
+        return kind == kObservable ? layers.size() - 1 - pos : pos;
+    }
+    // One layer of the replay on `rep`; with `snaps` its pre-layer values go to the snapshot buffers
+    // (indexed from the records sc0 / sr0).
+    void replay_layer(const LayerInfo &ly, double alpha, bool snaps, uint64_t sc0, uint64_t sr0) {
+        const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
+        if (nc + nr == 0) return;
+        double sn, cs;
+        sin_cos(2 * alpha, &sn, &cs);
+        const uint32_t blocks = static_cast<uint32_t>((nc + nr + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, stream, g_cos.as<uint32_t>(), g_rot.as<uint2>(),
+                           ly.cos_begin, nc, ly.rot_begin, nr, cs, sn, rep.as<double>(),
+                           snaps ? snap_c.as<double>() : nullptr, snaps ? snap_r.as<double>() : nullptr, sc0, sr0);
+        HIP_TRY(hipGetLastError());
+    }
+    double replay_alpha(ReplayKind kind, const double *params, const LayerInfo &ly) const {
+        return (kind == kState ? -1.0 : 1.0) * params[ly.param_index] * ly.gen_coeff;
+    }
+    void replay(const double *params, ReplayKind kind = kOperator) {
+        ckpt_valid = false;  // rep no longer holds a checkpointed forward
+        replay_init(kind);
+        for (size_t i = 0; i < layers.size(); ++i) {
+            const LayerInfo &ly = layers[replay_layer_at(kind, i)];
+            replay_layer(ly, replay_alpha(kind, params, ly), false, 0, 0);
         }
     }
 
-    // d<O>/d params: adjoint sweep over the layers in the reverse of the replay's order (after
-    // replay(params, true, kOperator) in the Heisenberg picture, kObservable in the Schrodinger
-    // picture, whose adjoint starts from the state instead of the diagonal phases).
+    // Checkpointed adjoint.  The replay order is cut into segments of about equal snapshot bytes
+    // (8 B per cos-only record, 16 B per rotation).  The forward replay keeps the coefficient vector at
+    // the start of every segment but the last (ckpt) and the last segment's pre-layer values; the
+    // adjoint sweeps the segments last to first, replaying each earlier one again from its checkpoint
+    // to regain its pre-layer values.  Every replayed value is the forward's to the bit, so the gradient
+    // does not depend on the segmentation.  The segment count is the smallest whose checkpoints plus
+    // one segment's snapshots fit grad_budget bytes (else the count that needs the least memory; a
+    // zero budget asks for the most segments, one per layer up to 256).
+    std::vector<size_t> seg_start;  // positions in the replay order, nseg + 1 entries
+    std::vector<uint64_t> seg_sc0, seg_sr0;
+    uint64_t seg_snap_c = 0, seg_snap_r = 0;  // largest segment's records
+    void plan_segments(ReplayKind kind) {
+        const size_t nl = layers.size();
+        std::vector<double> cum(nl + 1, 0.0);
+        for (size_t i = 0; i < nl; ++i) {
+            const LayerInfo &ly = layers[replay_layer_at(kind, i)];
+            cum[i + 1] = cum[i] + 8.0 * static_cast<double>(ly.cos_end - ly.cos_begin) +
+                         16.0 * static_cast<double>(ly.rot_end - ly.rot_begin);
+        }
+        const double vec = 8.0 * static_cast<double>(std::max<uint64_t>(N, 1));
+        auto cut = [&](size_t S, std::vector<size_t> &st) {  // greedy split, returns the largest segment
+            st.assign(1, 0);
+            double best = 0.0;
+            size_t from = 0;
+            for (size_t k = 1; k < S; ++k) {
+                const double target = cum[nl] * static_cast<double>(k) / static_cast<double>(S);
+                size_t p = static_cast<size_t>(std::lower_bound(cum.begin(), cum.end(), target) - cum.begin());
+                p = std::min(std::max(p, from + 1), nl);
+                if (p >= nl) break;
+                best = std::max(best, cum[p] - cum[from]);
+                st.push_back(p);
+                from = p;
+            }
+            st.push_back(nl);
+            return std::max(best, cum[nl] - cum[from]);
+        };
+        std::vector<size_t> st, best_st;
+        double best_mem = -1.0;
+        const size_t smax = std::max<size_t>(std::min<size_t>(nl, 256), 1);
+        if (grad_budget == 0) {  // the most segments: one per layer, up to 256
+            (void)cut(smax, st);
+            best_st = st;
+            best_mem = 0.0;
+        }
+        for (size_t S = 1; S <= smax && grad_budget != 0; ++S) {
+            const double seg = cut(S, st);
+            const double mem = static_cast<double>(st.size() - 2) * vec + seg;
+            if (best_mem < 0 || mem < best_mem) {
+                best_mem = mem;
+                best_st = st;
+            }
+            if (mem <= static_cast<double>(grad_budget)) {
+                best_st = st;
+                break;
+            }
+        }
+        seg_start = best_st;
+        const size_t ns = seg_start.size() - 1;
+        seg_sc0.assign(ns, 0);
+        seg_sr0.assign(ns, 0);
+        seg_snap_c = seg_snap_r = 0;
+        for (size_t sg = 0; sg < ns; ++sg) {
+            uint64_t c0 = UINT64_MAX, r0 = UINT64_MAX, c1 = 0, r1 = 0;
+            for (size_t i = seg_start[sg]; i < seg_start[sg + 1]; ++i) {
+                const LayerInfo &ly = layers[replay_layer_at(kind, i)];
+                c0 = std::min(c0, ly.cos_begin);
+                r0 = std::min(r0, ly.rot_begin);
+                c1 = std::max(c1, ly.cos_end);
+                r1 = std::max(r1, ly.rot_end);
+            }
+            if (c0 == UINT64_MAX) c0 = c1 = 0;
+            if (r0 == UINT64_MAX) r0 = r1 = 0;
+            seg_sc0[sg] = c0;
+            seg_sr0[sg] = r0;
+            seg_snap_c = std::max(seg_snap_c, c1 - c0);
+            seg_snap_r = std::max(seg_snap_r, r1 - r0);
+        }
+    }
+    void replay_segment(ReplayKind kind, const double *params, size_t sg, bool snaps) {
+        for (size_t i = seg_start[sg]; i < seg_start[sg + 1]; ++i) {
+            const LayerInfo &ly = layers[replay_layer_at(kind, i)];
+            replay_layer(ly, replay_alpha(kind, params, ly), snaps, seg_sc0[sg], seg_sr0[sg]);
+        }
+    }
+    // Forward replay with checkpoints (rep must hold the starting vector); afterwards rep holds the
+    // evolved vector and the last segment's snapshots are kept.
+    void forward_checkpointed(ReplayKind kind, const double *params, int64_t n_params) {
+        plan_segments(kind);
+        const size_t ns = seg_start.size() - 1;
+        snap_c.reserve(std::max<uint64_t>(seg_snap_c, 1) * 8);
+        snap_r.reserve(std::max<uint64_t>(seg_snap_r, 1) * 16);
+        if (ns > 1) ckpt.reserve((ns - 1) * std::max<uint64_t>(N, 1) * 8);
+        for (size_t sg = 0; sg < ns; ++sg) {
+            if (sg + 1 < ns && N)
+                HIP_TRY(hipMemcpyAsync(ckpt.as<double>() + sg * N, rep.p, N * 8, hipMemcpyDeviceToDevice, stream));
+            replay_segment(kind, params, sg, sg + 1 == ns);
+        }
+        ckpt_kind = kind;
+        ckpt_params.assign(params, params + n_params);
+        ckpt_valid = true;
+    }
+
+    // d<O>/d params: adjoint sweep over the layers in the reverse of the replay's order, after
+    // forward_checkpointed (kOperator in the Heisenberg picture, kObservable in the Schrodinger
+    // picture, whose adjoint starts from the state instead of the diagonal phases).  Consumes the
+    // checkpointed forward (rep is overwritten by the segment replays).
     template <int NW, int PAULI>
     void gradient(const double *params, int64_t n_params, double *grad) {
+        const ReplayKind kind = static_cast<ReplayKind>(ckpt_kind);
+        ckpt_valid = false;
         lam.reserve(std::max<uint64_t>(N, 1) * 8);
         if (N && schr) {
             HIP_TRY(hipMemcpyAsync(lam.p, coef.p, N * 8, hipMemcpyDeviceToDevice, stream));
@@ -2550,19 +2688,26 @@ struct mpa_engine {
         gpartial.reserve(std::max<uint64_t>(offs[nl], 1) * 8);
         goffs.reserve((nl + 1) * 8);
         gdl.reserve(std::max<size_t>(nl, 1) * 8);
-        for (size_t i = 0; i < nl; ++i) {
-            const size_t l = schr ? i : nl - 1 - i;
-            const LayerInfo &ly = layers[l];
-            const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
-            if (nc + nr == 0) continue;
-            const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
-            double sn, cs;
-            sin_cos(2 * alpha, &sn, &cs);
-            hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(offs[l + 1] - offs[l])), dim3(kBlock), 0, stream,
-                               g_cos.as<uint32_t>(), g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr,
-                               cs, sn, snap_c.as<double>(), snap_r.as<double>(),
-                               lam.as<double>(), gpartial.as<double>(), offs[l]);
-            HIP_TRY(hipGetLastError());
+        const size_t ns = seg_start.size() - 1;
+        for (size_t sg = ns; sg-- > 0;) {
+            if (sg + 1 < ns) {  // regain this segment's pre-layer values from its checkpoint
+                if (N) HIP_TRY(hipMemcpyAsync(rep.p, ckpt.as<double>() + sg * N, N * 8, hipMemcpyDeviceToDevice, stream));
+                replay_segment(kind, params, sg, true);
+            }
+            for (size_t i = seg_start[sg + 1]; i-- > seg_start[sg];) {
+                const size_t l = replay_layer_at(kind, i);
+                const LayerInfo &ly = layers[l];
+                const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
+                if (nc + nr == 0) continue;
+                const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
+                double sn, cs;
+                sin_cos(2 * alpha, &sn, &cs);
+                hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(offs[l + 1] - offs[l])), dim3(kBlock), 0,
+                                   stream, g_cos.as<uint32_t>(), g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr,
+                                   cs, sn, snap_c.as<double>(), snap_r.as<double>(), lam.as<double>(),
+                                   gpartial.as<double>(), offs[l], seg_sc0[sg], seg_sr0[sg]);
+                HIP_TRY(hipGetLastError());
+            }
         }
         std::vector<double> dl(nl, 0.0);
         if (nl) {
@@ -3176,7 +3321,6 @@ struct RunSchrExpect {
     mpa_engine *e;
     const double *params;
     int64_t n_params;
-    bool snapshots;
     double *out;
     template <int NW, int PAULI>
     void operator()() {
@@ -3192,8 +3336,37 @@ struct RunSchrExpect {
         std::vector<int64_t> hr;
         std::vector<double> hv;
         e->match_observable<NW>(e->coef.as<double>(), hr, hv, e->rep.as<double>());
-        e->replay(params, snapshots, mpa_engine::kObservable);
+        e->replay(params, mpa_engine::kObservable);
         *out = e->dot_core<NW, PAULI>(e->coef.as<double>(), e->rep.as<double>());
+    }
+};
+
+// The checkpointed forward replay of the stored graph at params and its <O> (the reference's
+// ev / ev_and_grad value, MPFunctions.cpp:196-228): Heisenberg, the operator from the graph's start
+// vector; Schrodinger, the observable newest layer first, dotted with the state.
+struct RunForwardCheckpointed {
+    mpa_engine *e;
+    const double *params;
+    int64_t n_params;
+    double *out;
+    template <int NW, int PAULI>
+    void operator()() {
+        if (n_params != e->graph_nparams)
+            throw ArgError("Expected " + std::to_string(e->graph_nparams) + " parameter value(s), got " +
+                           std::to_string(n_params) + ".");
+        if (e->schr) {
+            e->rep.reserve(std::max<uint64_t>(e->N, 1) * 8);
+            std::vector<int64_t> hr;
+            std::vector<double> hv;
+            e->match_observable<NW>(e->coef.as<double>(), hr, hv, e->rep.as<double>());
+            e->forward_checkpointed(mpa_engine::kObservable, params, n_params);
+            *out = e->dot_core<NW, PAULI>(e->coef.as<double>(), e->rep.as<double>());
+        } else {
+            e->replay_init(mpa_engine::kOperator);
+            e->forward_checkpointed(mpa_engine::kOperator, params, n_params);
+            *out = e->expect_impl<NW, PAULI>(e->rep.as<double>());
+        }
+        e->ckpt_value = *out;
     }
 };
 
@@ -3280,7 +3453,7 @@ void make_gates(const mpa_engine *e, const int64_t *gen_offsets, const int32_t *
 // The coefficient vector of the current picture at params: the replay when a graph is stored (n_params
 // must cover it; the Schrodinger picture replays the state), else the contracted coefficients
 // (n_params must be 0).
-const double *evaluated_coeffs(mpa_engine *e, const double *params, int64_t n_params, bool snapshots) {
+const double *evaluated_coeffs(mpa_engine *e, const double *params, int64_t n_params) {
     if (e->layers.empty()) {
         if (n_params != 0) throw ArgError("Expected 0 parameter value(s), got " + std::to_string(n_params) + ".");
         return e->coef.as<double>();
@@ -3288,7 +3461,7 @@ const double *evaluated_coeffs(mpa_engine *e, const double *params, int64_t n_pa
     if (n_params != e->graph_nparams)
         throw ArgError("Expected " + std::to_string(e->graph_nparams) + " parameter value(s), got " +
                        std::to_string(n_params) + ".");
-    e->replay(params, snapshots, e->schr ? mpa_engine::kState : mpa_engine::kOperator);
+    e->replay(params, e->schr ? mpa_engine::kState : mpa_engine::kOperator);
     return e->rep.as<double>();
 }
 
@@ -3588,6 +3761,7 @@ int mpa_update_initial_operator(mpa_engine *e, const int64_t *term_offsets, cons
                                 const double *coeff_re, const double *coeff_im, int64_t n_terms) {
     return guarded([&] {
         require_single(e);
+        e->ckpt_valid = false;
         std::vector<uint64_t> rows;
         std::vector<double> coefs;
         // the core term changes as soon as it is read, before any term is looked up (MonomialPropagator.inl:363-376)
@@ -3642,6 +3816,7 @@ int mpa_update_initial_operator(mpa_engine *e, const int64_t *term_offsets, cons
 int mpa_set_parameter_mapping(mpa_engine *e, const int64_t *param_index, int64_t n_layers) {
     return guarded([&] {
         require_single(e);
+        e->ckpt_valid = false;
         if (n_layers != static_cast<int64_t>(e->layers.size()))
             throw RunError("parameter_mapping has " + std::to_string(n_layers) + " entries; expected " +
                            std::to_string(e->layers.size()) + " (per graph layer).");
@@ -4016,6 +4191,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
     return guarded([&] {
         require_single(e);
         if (n_layers == 0) return;
+        e->ckpt_valid = false;
         check_params_length(param_map, n_layers, params, n_params);
         std::vector<GateDesc> gds;
         std::vector<int32_t> folds;
@@ -4031,7 +4207,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
             if (n_params < e->graph_nparams)
                 throw ArgError("Coefficient-informed build_graph() needs at least " +
                                std::to_string(e->graph_nparams) + " parameter value(s) to replay the existing graph.");
-            e->replay(params, false, e->schr ? mpa_engine::kState : mpa_engine::kOperator);
+            e->replay(params, e->schr ? mpa_engine::kState : mpa_engine::kOperator);
             if (e->N) HIP_TRY(hipMemcpyAsync(e->seedc.p, e->rep.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
         } else if (e->N) {
             HIP_TRY(hipMemcpyAsync(e->seedc.p, e->coef.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
@@ -4053,18 +4229,25 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
         }
         e->building = false;
         for (int64_t i = 0; i < n_layers; ++i) e->graph_nparams = std::max(e->graph_nparams, param_map[i] + 1);
+        e->shrink_keep(e->g_cos, e->gcos_total * 4);  // the records' arrays at their exact sizes
+        e->shrink_keep(e->g_rot, e->grot_total * 8);
     });
 }
 
 int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params, double *out) {
     return guarded([&] {
         require_single(e);
+        if (e->keep_ckpt && !e->layers.empty()) {  // the forward a following gradient starts from
+            RunForwardCheckpointed fw{e, params, n_params, out};
+            e->dispatch(fw);
+            return;
+        }
         if (e->schr) {
-            RunSchrExpect ev{e, params, n_params, false, out};
+            RunSchrExpect ev{e, params, n_params, out};
             e->dispatch(ev);
             return;
         }
-        const double *c = evaluated_coeffs(e, params, n_params, false);
+        const double *c = evaluated_coeffs(e, params, n_params);
         RunExpect ev{e, c, out};
         e->dispatch(ev);
     });
@@ -4087,25 +4270,52 @@ int mpa_expectation_value_and_gradient(mpa_engine *e, const double *params, int6
                                        double *grad) {
     return guarded([&] {
         require_single(e);
-        if (e->schr) {
-            RunSchrExpect ev{e, params, n_params, true, value};
-            e->dispatch(ev);
+        if (e->layers.empty()) {  // no graph: the contracted value, no parameters to differentiate
+            if (e->schr) {
+                RunSchrExpect ev{e, params, n_params, value};
+                e->dispatch(ev);
+            } else {
+                const double *c = evaluated_coeffs(e, params, n_params);
+                RunExpect ev{e, c, value};
+                e->dispatch(ev);
+            }
+            return;
+        }
+        const bool cached = e->ckpt_valid && n_params == static_cast<int64_t>(e->ckpt_params.size()) &&
+                            std::memcmp(params, e->ckpt_params.data(), static_cast<size_t>(n_params) * 8) == 0;
+        if (cached) {
+            *value = e->ckpt_value;
         } else {
-            const double *c = evaluated_coeffs(e, params, n_params, true);
-            RunExpect ev{e, c, value};
-            e->dispatch(ev);
+            RunForwardCheckpointed fw{e, params, n_params, value};
+            e->dispatch(fw);
         }
-        if (!e->layers.empty()) {
-            RunGradient rg{e, params, n_params, grad};
-            e->dispatch(rg);
-        }
+        RunGradient rg{e, params, n_params, grad};
+        e->dispatch(rg);
+    });
+}
+
+int mpa_set_gradient_checkpointing(mpa_engine *e, int keep, int64_t budget_bytes) {
+    return guarded([&] {
+        if (budget_bytes < 0) throw ArgError("budget_bytes must be non-negative");
+        e->keep_ckpt = keep != 0;
+        e->grad_budget = budget_bytes;
+        e->ckpt_valid = false;
+    });
+}
+
+int mpa_gradient_segments(mpa_engine *e, int64_t *segments, int64_t *checkpoint_bytes, int64_t *snapshot_bytes) {
+    return guarded([&] {
+        const size_t ns = e->seg_start.empty() ? 0 : e->seg_start.size() - 1;
+        *segments = static_cast<int64_t>(ns);
+        *checkpoint_bytes = static_cast<int64_t>(e->ckpt.bytes);
+        *snapshot_bytes = static_cast<int64_t>(e->snap_c.bytes + e->snap_r.bytes);
     });
 }
 
 int mpa_contract(mpa_engine *e, const double *params, int64_t n_params, int inplace, double *out) {
     return guarded([&] {
         require_single(e);
-        const double *c = evaluated_coeffs(e, params, n_params, false);
+        const double *c = evaluated_coeffs(e, params, n_params);
         if (out && e->N) {
             HIP_TRY(hipMemcpyAsync(out, c, e->N * 8, hipMemcpyDeviceToHost, e->stream));
         }
@@ -4154,7 +4364,7 @@ int mpa_export_terms(mpa_engine *e, const double *params, int64_t n_params, doub
                      int64_t *n_slots, int64_t *offsets, int32_t *slots, double *re, double *im) {
     return guarded([&] {
         require_single(e);
-        const double *cdev = evaluated_coeffs(e, params, n_params, false);
+        const double *cdev = evaluated_coeffs(e, params, n_params);
         std::vector<uint64_t> w(e->N * e->nw);
         std::vector<double> c(e->N);
         if (e->N) {
