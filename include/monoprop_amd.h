@@ -94,10 +94,12 @@ int mpa_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_
                   const double *params, int64_t n_params, int32_t only_rotate_len_k);
 
 /* Replaces MonomialPropagator::build_graph (MonomialPropagator.inl:620-690, binder.h:102-110).
- * params == NULL builds structurally; otherwise the build is coefficient-seeded at params. */
+ * params == NULL builds structurally; otherwise the build is coefficient-seeded at params.
+ * only_rotate_len_k < 0: no orbital cap; else only terms of at most k slots rotate (their layers'
+ * cos sets and emitters), as in mpa_propagate. */
 int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots,
                     int64_t n_layers, const int64_t *param_map, const double *gen_coeffs,
-                    const double *params, int64_t n_params);
+                    const double *params, int64_t n_params, int32_t only_rotate_len_k);
 
 /* Replaces MonomialPropagator::expectation_value (MonomialPropagator.inl:1034-1041,
  * binder.h:127-130): replays the stored graph at params (n_params must equal the graph's

@@ -51,7 +51,7 @@ _SIGNATURES = {
     "mpa_memory_breakdown": (C.c_int, [_P, _i64p, C.c_int32]),
     "mpa_destroy": (C.c_int, [_P]),
     "mpa_propagate": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
-    "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64]),
+    "mpa_build_graph": (C.c_int, [_P, _i64p, _i32p, C.c_int64, _i64p, _f64p, _f64p, C.c_int64, C.c_int32]),
     "mpa_expectation_value": (C.c_int, [_P, _f64p, C.c_int64, _f64p]),
     "mpa_expectation_value_noalloc": (C.c_int, [_P, _f64p]),
     "mpa_set_gradient_checkpointing": (C.c_int, [_P, C.c_int, C.c_int64]),
@@ -339,16 +339,15 @@ class DeviceSimulator(_EngineHandle):
                     only_rotate_len_k=None) -> None:
         if not len(majoranas):
             return
-        if only_rotate_len_k is not None:
-            raise ValueError("only_rotate_len_k is supported by propagate() only.")
         offs, slots, pm, gc = self._layers(majoranas, parameter_mapping, gen_coeffs)
         if parameters is None:
             th_ptr, n = None, 0
         else:
             th = _f64(parameters)
             th_ptr, n = _ptr(th, C.c_double), len(list(parameters))
+        k = -1 if only_rotate_len_k is None else int(only_rotate_len_k)
         _check(load_library().mpa_build_graph(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(majoranas),
-                                              _ptr(pm, C.c_int64), _ptr(gc, C.c_double), th_ptr, n))
+                                              _ptr(pm, C.c_int64), _ptr(gc, C.c_double), th_ptr, n, k))
         gates = list(gate_indices) if gate_indices is not None else list(range(len(majoranas)))
         offset = self._n_gates
         self._n_gates = offset + (max(gates) + 1 if gates else 0)

@@ -4187,9 +4187,12 @@ int mpa_batch_row_members(mpa_engine *e, int32_t *out) {
 }
 
 int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
-                    const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params) {
+                    const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
+                    int32_t only_rotate_len_k) {
     return guarded([&] {
         require_single(e);
+        if (only_rotate_len_k == 0 || only_rotate_len_k > 2 * e->n_modes)
+            throw ArgError("only_rotate_len_k is out of range; must be 0 < k <= 2*num_modes");
         if (n_layers == 0) return;
         e->ckpt_valid = false;
         check_params_length(param_map, n_layers, params, n_params);
@@ -4213,8 +4216,11 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
             HIP_TRY(hipMemcpyAsync(e->seedc.p, e->coef.p, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
         }
         HIP_TRY(hipStreamSynchronize(e->stream));
+        // The orbital cap gates emission and the seed's cosine (Scan.h:419-433); the records keep every
+        // anticommuting row in the cosine set, since a stored layer's replay recomputes that set over all
+        // rows before its scaled_count (CosineRecompute.h:17-20, MonomialPropagator.inl:784-790).
         Trunc trl = e->tr;
-        trl.only_rot = -1;
+        trl.only_rot = only_rotate_len_k < 0 ? -1 : only_rotate_len_k;
         if (params == nullptr) {  // structural build: no coefficient checks (Scan.h:81-95)
             trl.check_lower = 0;
             trl.check_upper = 0;

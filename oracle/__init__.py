@@ -55,7 +55,7 @@ def _lib():
         lib.orc_propagate.restype = C.c_int
         lib.orc_propagate.argtypes = [P, i64p, i32p, C.c_int64, i64p, f64p, f64p, C.c_int64, C.c_int]
         lib.orc_build_graph.restype = C.c_int
-        lib.orc_build_graph.argtypes = [P, i64p, i32p, C.c_int64, i64p, f64p, f64p, C.c_int64]
+        lib.orc_build_graph.argtypes = [P, i64p, i32p, C.c_int64, i64p, f64p, f64p, C.c_int64, C.c_int]
         lib.orc_graph_layers.restype = C.c_int64
         lib.orc_graph_layers.argtypes = [P]
         lib.orc_expectation.restype = C.c_double
@@ -169,7 +169,7 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(_lib().orc_last_error().decode())
 
-    def build_graph(self, majoranas, parameter_mapping, gen_coeffs, parameters=None):
+    def build_graph(self, majoranas, parameter_mapping, gen_coeffs, parameters=None, only_rotate_len_k=None):
         offs, slots, pm, gc = self._layers(majoranas, parameter_mapping, gen_coeffs)
         if parameters is None:
             th_ptr, n = None, 0
@@ -178,7 +178,8 @@ class Oracle:
             th_ptr, n = _ptr(th, C.c_double), len(list(parameters))
             self._keep = th
         rc = _lib().orc_build_graph(self._h, _ptr(offs, C.c_int64), _ptr(slots, C.c_int32), len(majoranas),
-                                    _ptr(pm, C.c_int64), _ptr(gc, C.c_double), th_ptr, n)
+                                    _ptr(pm, C.c_int64), _ptr(gc, C.c_double), th_ptr, n,
+                                    -1 if only_rotate_len_k is None else int(only_rotate_len_k))
         if rc != 0:
             raise RuntimeError(_lib().orc_last_error().decode())
 

@@ -924,9 +924,12 @@ int orc_propagate(void *h, const int64_t *gen_offsets, const int32_t *gen_slots,
     return 0;
 }
 
-// Graph build (MonomialPropagator.inl:537-592); seeded when params != nullptr.
+// Graph build (MonomialPropagator.inl:537-592); seeded when params != nullptr.  The orbital cap
+// (only_rot >= 0) gates emission and the seed's cosine set (Scan.h:419-433); the stored layer's cosine
+// set stays "every anticommuting row before scaled_count" (layer_cos, CosineRecompute.h:17-20).
 int orc_build_graph(void *h, const int64_t *gen_offsets, const int32_t *gen_slots, int64_t n_layers,
-                    const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params) {
+                    const int64_t *param_map, const double *gen_coeffs, const double *params, int64_t n_params,
+                    int only_rot) {
     auto *e = static_cast<Engine *>(h);
     std::vector<double> seed;
     if (params) seed = e->coeffs;  // current picture (no stored graph yet in our use)
@@ -945,10 +948,10 @@ int orc_build_graph(void *h, const int64_t *gen_offsets, const int32_t *gen_slot
             if (param_map[idx] < 0 || param_map[idx] >= n_params) return fail("parameter index out of range");
             const double alpha = 1.0 * params[param_map[idx]] * gen_coeffs[idx];
             const double apply = e->schr ? -alpha : alpha;
-            e->build_layer(g.data(), true, alpha, apply, -1, false, &seed, &ly, &cos);
+            e->build_layer(g.data(), true, alpha, apply, only_rot, false, &seed, &ly, &cos);
             e->evolve_step(seed, ly, cos, apply);
         } else {
-            e->build_layer(g.data(), false, 0.0, 0.0, -1, false, nullptr, &ly, &cos);
+            e->build_layer(g.data(), false, 0.0, 0.0, only_rot, false, nullptr, &ly, &cos);
         }
         e->graph.push_back(std::move(ly));
     }

@@ -217,3 +217,60 @@ def test_statevector_c1_short(oracle_mod):
     orc = _evolve_oracle(oracle_mod, "pauli", n, obs, gates, thetas)
     want = dense.statevector_expectation(n, [({6: "Z"}, 1.0)], sv_gates, thetas)
     assert abs(orc.expectation() - want) < 1e-12
+
+
+def _graph_operator(o, theta):
+    """The oracle's replayed graph coefficients as a Majorana operator dict (store coefficient r of a
+    weight-w row is the operator coefficient r i^{C(w,2)})."""
+    w, _ = o.rows()
+    c = o.graph_contract(theta)
+    out = {}
+    for row, v in zip(w, c):
+        slots = tuple(64 * j + b for j in range(len(row)) for b in range(64) if (int(row[j]) >> b) & 1)
+        if v != 0.0:
+            out[slots] = v * (1j) ** (len(slots) * (len(slots) - 1) // 2)
+    return out
+
+
+@pytest.mark.parametrize("k", [None, 2, 6])
+def test_capped_graph_layer_vs_dense(oracle_mod, k):
+    """One orbital layer G = i m1 m2 built as a graph with the cap only_rotate_len_k = k, replayed at an
+    angle.  The stored layer's cosine set is every anticommuting row (CosineRecompute.h:17-20) while
+    only pairs with a member of at most k slots rotate (Scan.h:419-433), so: the rotating pairs follow
+    U^dag O U exactly, an anticommuting pair with no member within the cap keeps only its cos(2a) share,
+    and commuting terms stay.  k = 6 (every length) is the uncapped graph."""
+    n, theta = 3, 0.41
+    ms = dense.majorana_matrices(n)
+    gmono = (1, 2)
+    H = 1j * dense.majorana_product(gmono, ms)
+    gate = ExpGate(MajoranaOperator({gmono: 1j}, n))
+    # (1,) -> partner (2,) and (2, 3) -> partner (1, 3) lie within k = 2; (0, 1, 3, 4, 5) and (1, 3, 4, 5)
+    # anticommute with partners longer than 2; (0,) and (1, 2, 3) commute with G
+    store = {(0,): 0.3, (1,): -0.7, (2, 3): 0.45, (0, 1, 3, 4, 5): 0.9, (1, 3, 4, 5): -0.25, (1, 2, 3): 0.6}
+    terms = {m: r * (1j) ** (len(m) * (len(m) - 1) // 2) for m, r in store.items()}
+    circ = Circuit([gate], n, [theta])
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    o = oracle_mod.Oracle(terms, n, basis="majorana", cutoff=2 * n, cutoff_type="length")
+    o.build_graph(monos, pm, gcs, None, only_rotate_len_k=k)
+    got = _graph_operator(o, [theta])
+
+    def rotated(ts):
+        if not ts:
+            return {}
+        O = sum(c * dense.majorana_product(m, ms) for m, c in ts.items())
+        return dense.majorana_decompose(dense.heisenberg(O, [H], [theta]), n)
+
+    cap = 2 * n if k is None else k
+    rot = {m: c for m, c in terms.items() if len(m) <= cap or len(set(m) ^ set(gmono)) <= cap}
+    want = rotated(rot)
+    for m, c in terms.items():
+        if m in rot:
+            continue
+        anti = len(set(m) & set(gmono)) % 2 == 1
+        # the cos(2a) share of a lone anticommuting term is the diagonal of its own exact rotation
+        want[m] = rotated({m: c}).get(m, 0.0) if anti else c
+        assert not anti or abs(want[m] - c * math.cos(2 * theta)) <= 1e-12
+    _dicts_close(got, want)
+    if k == 2:
+        anti_rot = [m for m in rot if len(set(m) & set(gmono)) % 2 == 1]
+        assert sorted(anti_rot) == [(1,), (2, 3)] and o.size() == len(store) + 2  # (2,), (1, 3) inserted
