@@ -767,14 +767,17 @@ __device__ __forceinline__ bool gidx_insert(uint64_t *__restrict__ gidx, uint64_
     return false;
 }
 
-// Row of (`key`, `member`) among rows [0, limit), or -1; *val receives the row's coefficient.
-// member_of null: every row is member 0.
+// Row of (`key`, `member`) among rows [0, limit), or -1; *val receives the row's coefficient and, when
+// `flags` is given, *flag_word the word of `flags` (one bit per row) that holds the row's bit, loaded
+// in the same round trip.  member_of null: every row is member 0.
 template <int NW>
 __device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ gidx, uint64_t gslots,
                                                    const uint64_t *__restrict__ rows,
                                                    const uint16_t *__restrict__ member_of,
                                                    const double *__restrict__ coef, const uint64_t *key,
-                                                   uint64_t limit, uint32_t member, double *val) {
+                                                   uint64_t limit, uint32_t member, double *val,
+                                                   const uint64_t *__restrict__ flags = nullptr,
+                                                   uint64_t *flag_word = nullptr) {
     const uint64_t h = gidx_hash<NW>(key, member);
     const uint32_t fp = static_cast<uint32_t>(h);
     const uint64_t end = gslots + kIdxPad;
@@ -791,11 +794,13 @@ __device__ __forceinline__ int64_t gidx_find_below(const uint64_t *__restrict__ 
             load_row<NW>(rows, static_cast<uint32_t>(r), m);
             const double c = coef[r];
             const uint32_t q = member_of ? member_of[r] : 0u;
+            const uint64_t fw = flags ? flags[r >> 6] : 0;
             bool eq = q == member;
 #pragma unroll
             for (int w = 0; w < NW; ++w) eq = eq && (m[w] == key[w]);
             if (eq) {
                 *val = c;
+                if (flags) *flag_word = fw;
                 return static_cast<int64_t>(r);
             }
         }
@@ -1184,7 +1189,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
                                                   uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
                                                   int group_log2, uint64_t *__restrict__ counters,
                                                   const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig,
-                                                  uint32_t nbatch) {
+                                                  uint32_t nbatch, uint64_t cap_rows) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
@@ -1204,7 +1209,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
         __syncthreads();
     }
     if (stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_TK_RES] = 0;
+    // a gate may double the live count: stop before it when the store could not take that (the host
+    // grows the store and resumes here)
+    if (2 * N > cap_rows) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_STOP] = static_cast<uint64_t>(gi) + 1;
+        return;
+    }
+    // the next gate's live count starts at N; the resolve's chunks add their misses to it
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[C_TK_RES] = 0;
+        counters[C_NB + ((gi & 1) ^ 1)] = N;
+    }
     const uint64_t nwords = (N + 63) / 64;
     if (tile0 * kWTileWords >= nwords) return;  // whole block idle
     uint64_t grp = 0;
@@ -1235,9 +1250,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
 //              the row index, and their column bits are gathered per word in LDS before one OR per
 //              word.  A block appends a chunk one iteration late: it resolves its next chunk first
 //              (misses staged in LDS), so the look-back rarely waits on chunks still resolving
-// The chunk holding the last emitter publishes N + I for the next gate and stops the device before a
-// gate whose worst case could outgrow the store or the index.  Every block also clears its share of
-// the other gate parity's group counts.
+// The next gate's live count N + I is the sweep's N plus every chunk's misses (atomic adds: a chunk
+// that creates no term neither looks back nor waits); the next gate's sweep stops the device when its
+// worst case could outgrow the store or the index.  Every block also clears its share of the other
+// gate parity's group counts.
 // With BATCH, an emitter's member (member[S]) selects its rotation constants (gtrig[member]) and keys its
 // index lookup; the partner it creates joins the same member.
 template <int NW, int PAULI, int BATCH, int SCHR>
@@ -1251,7 +1267,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
                                                     uint64_t gslots, uint64_t *__restrict__ desc_m,
                                                     uint64_t *__restrict__ counters, int64_t *__restrict__ stats,
-                                                    uint64_t cap_rows, uint32_t max_groups,
+                                                    uint32_t max_groups,
                                                     int group_log2, uint16_t *__restrict__ member,
                                                     const Trig *__restrict__ gtrig,
                                                     const uint64_t *__restrict__ occ_dev) {
@@ -1314,7 +1330,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
         st[1] = static_cast<int64_t>(nA);
         st[2] = static_cast<int64_t>(nEL) + static_cast<int64_t>(nEF);
         st[4] = static_cast<int64_t>(nEL);
-        if (nchunks == 0) counters[C_NB + nxt] = N;
     }
     if (blockIdx.x >= nchunks) return;  // only blocks with work take chunks
     // Chunks in order: block b takes chunk b first (blocks are dispatched in index order, so every
@@ -1443,8 +1458,10 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
 #pragma unroll
                 for (int k = 0; k < NW; ++k) t[k] = m[k] ^ gd.g[k];
                 double wT = 0.0;
-                const int64_t T = gidx_find_below<NW>(gidx, gslots, rows, BATCH ? member : nullptr, coef, t, N, q, &wT);
-                const bool skip = T >= 0 && !leader && ((elead[T >> 6] >> (T & 63)) & 1u);
+                uint64_t lw = 0;  // followers: the partner's word of emitting-leader bits, with its coefficient
+                const int64_t T = gidx_find_below<NW>(gidx, gslots, rows, BATCH ? member : nullptr, coef, t, N, q, &wT,
+                                                      leader ? nullptr : elead, &lw);
+                const bool skip = T >= 0 && !leader && ((lw >> (T & 63)) & 1u);
                 if (!skip) {
                     const double gk = BATCH ? gtrig[q].k : gd.k;
                     const double gs = BATCH ? gtrig[q].s : gd.s;
@@ -1473,11 +1490,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
             if (threadIdx.x == 0) {
                 desc_publish(desc_m + chunk, ep, chunk == 0 ? kDescIncl : kDescAgg, n_miss);
                 if (n_hit) atomicAdd(reinterpret_cast<unsigned long long *>(st + 5), static_cast<unsigned long long>(n_hit));
+                if (n_miss) {  // the next gate's live count and this gate's insert count (no order needed)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NB + nxt),
+                              static_cast<unsigned long long>(n_miss));
+                    atomicAdd(reinterpret_cast<unsigned long long *>(st + 3), static_cast<unsigned long long>(n_miss));
+                }
             }
         }
-        // A chunk that creates no term has nothing to place: its aggregate (0) is all its successors need,
-        // so only the last chunk (which publishes the gate's total) looks back without misses.
-        if (pend != 0xffffffffu && (pend_miss != 0 || pend == nchunks - 1)) {
+        // A chunk that creates no term has nothing to place: its aggregate (0) is all its successors need.
+        if (pend != 0xffffffffu && pend_miss != 0) {
             // ---- append the previous chunk: its misses land at N + (misses of earlier chunks) + rank
             const uint32_t pb = buf ^ 1u;
             // the emitters' rows are requested before the look-back, so both round trips overlap
@@ -1494,15 +1515,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                     excl = look_back(desc_m, pend, ep, counters);
                     if (threadIdx.x == 0) desc_publish(desc_m + pend, ep, kDescIncl, excl + pend_miss);
                 }
-                if (threadIdx.x == 0) {
-                    s_base = excl;
-                    if (pend == nchunks - 1) {
-                        const uint64_t I = excl + pend_miss, n = N + I;
-                        st[3] = static_cast<int64_t>(I);
-                        counters[C_NB + nxt] = n;
-                        if (2 * n > cap_rows) counters[C_STOP] = static_cast<uint64_t>(gi) + 2;
-                    }
-                }
+                if (threadIdx.x == 0) s_base = excl;
             }
             for (int k = threadIdx.x; k < kColWords * NC; k += kBlock) lcol[k] = 0;
             __syncthreads();
@@ -3038,7 +3051,7 @@ struct mpa_engine {
                                    cols.as<uint64_t>(), colstride(), sc, fold.as<int32_t>() + gd.fold_off,
                                    rows.as<uint64_t>(), coef.as<double>(), t_val.as<double>(), t_item.as<uint32_t>(),
                                    f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur,
-                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch);
+                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch, cap);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
@@ -3047,7 +3060,7 @@ struct mpa_engine {
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
                                    gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gidx_slots,
-                                   desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(), cap,
+                                   desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(),
                                    static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig,
                                    schr ? occ_dev.as<uint64_t>() : static_cast<uint64_t *>(nullptr));
                 HIP_TRY(hipGetLastError());
