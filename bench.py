@@ -3,15 +3,18 @@
 Workload (one "step"): the 10x10 open-lattice kicked-Ising circuit (180 ZZ bonds, row-major), 20
 Trotter steps of ZZ rotations theta_J = -pi/4 then X rotations theta_h, propagated once for each
 theta_h in {0.1, 0.4, 0.8}; observable (1/100) sum_i Z_i, support cutoff 10, lower_atol 1e-5,
-FP64 coefficients, two u64 limbs per plane.  Inputs are synthetic circuits (no data files).  The three
-propagations run as members of one batched store (each member's terms and coefficients are those of
-its own propagation); `unbatched` times the same steps one propagation at a time.
+FP64 coefficients, two u64 limbs per plane.  Inputs are synthetic circuits (no data files).  By
+default the three propagations run at once, each on its own engine (own HIP stream, own host thread);
+`schedules` times the same steps as the three members of one batched store (each member's terms and
+coefficients are those of its own propagation) and as one propagation after another.
 
 Metric: term-gate applications per second, T / t with T = sum over gates of the live terms before
 the gate (counted on the device); t covers engine creation, propagation and <O> for the three
-circuits (their generator lists are built once, before timing).  The roofline figure is the dominant
-kernel's algorithmic bytes over its HIP-event-timed device duration, from one extra step of the same
-workload run with the per-launch profiler on (kept out of the timed steps).  Algorithmic bytes per gate
+circuits (their generator lists are built once, before timing).  The roofline figure is one gate's two
+kernels (sweep and resolve) together: their algorithmic bytes over their HIP-event-timed device
+duration, from one extra step of the same
+workload run with the per-launch profiler on (kept out of the timed steps; its propagations run one
+after another, so that each launch's duration is its own).  Algorithmic bytes per gate
 are the path's measurement definition: the sweep reads 16 B of x/z limbs per live term (per limb pair
 that meets the generator), reads and writes each anticommuting coefficient, reads the anticommuting
 terms' other limbs and writes each emitted partner record (row + value); the resolve reads those records
@@ -26,7 +29,7 @@ budget, its stores and <O> compared bit for bit with the GPU step's (the serial 
 one propagation at a time, default theta_h 0.8: at 0.1 and 0.4 the 100-qubit state outgrows the GPU).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--trotter-steps 20] [--no-cpu-baseline]
-       [--no-gradient] [--batch-sets 4] [--unbatched] [--schrodinger Q] [--theta-h 0.1 0.4 0.8]
+       [--no-gradient] [--batch-sets 4] [--batched | --unbatched] [--schrodinger Q] [--theta-h 0.1 0.4 0.8]
 """
 
 from __future__ import annotations
@@ -66,9 +69,12 @@ def parse_args():
                     help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
     ap.add_argument("--traffic-out", default=str(TRAFFIC_FILE))
     ap.add_argument("--cpu-sample-steps", type=int, default=4, help="Trotter steps of the serial CPU sample")
+    ap.add_argument("--batched", action="store_true",
+                    help="time the step as one batched store (the theta_h as its members)")
     ap.add_argument("--unbatched", action="store_true",
-                    help="time one propagation per theta_h (default: the three as members of one store)")
-    ap.add_argument("--no-compare", action="store_true", help="skip timing the unbatched steps for comparison")
+                    help="time the step as one propagation per theta_h, one after another (default: the three "
+                         "at once, each on its own engine and stream)")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the other step schedules for comparison")
     ap.add_argument("--schrodinger", type=int, default=None, metavar="Q",
                     help="Schrodinger picture, initial state terms of at most Q qubits (one propagation at a time)")
     ap.add_argument("--theta-h", type=float, nargs="+", default=None, help="X angles of the step's propagations")
@@ -117,26 +123,37 @@ def run_gpu_step_batched(L, inputs, profile):
     return total_t, [float(v) for v in sim.expectation_values()], [sim]
 
 
-def run_gpu_step(L, inputs, profile):
-    """One step: a propagation per theta_h; returns (term-gate applications, <O> list, simulators)."""
+def _lone_propagation(n, inp, profile):
+    """One theta_h propagated by its own engine: (term-gate applications, <O>, simulator)."""
     from monoprop_amd._engine import DeviceSimulator
 
-    n = L * L
-    total_t = 0
-    evs = []
-    sims = []
-    for monos, gcs, pm, params in inputs:
-        schr = None if SCHRODINGER_QUBITS is None else 2 * SCHRODINGER_QUBITS
-        sim = DeviceSimulator(observable_terms(n), 10, [], lower_atol=1e-5, cutoff_type="support",
-                              logical_num_modes=n, basis="pauli", schrodinger_cutoff=schr)
-        if profile:
-            sim.set_profiling(True)
-        sim.propagate(monos, pm, gcs, params)
-        stats = sim.gate_stats()
-        total_t += int(stats[0].sum())
-        evs.append(sim.expectation_value([]))
-        sims.append(sim)
-    return total_t, evs, sims
+    monos, gcs, pm, params = inp
+    schr = None if SCHRODINGER_QUBITS is None else 2 * SCHRODINGER_QUBITS
+    sim = DeviceSimulator(observable_terms(n), 10, [], lower_atol=1e-5, cutoff_type="support",
+                          logical_num_modes=n, basis="pauli", schrodinger_cutoff=schr)
+    if profile:
+        sim.set_profiling(True)
+    sim.propagate(monos, pm, gcs, params)
+    return int(sim.gate_stats()[0].sum()), sim.expectation_value([]), sim
+
+
+def run_gpu_step(L, inputs, profile):
+    """One step: a propagation per theta_h, one after another; returns (term-gate applications, <O> list,
+    simulators)."""
+    res = [_lone_propagation(L * L, inp, profile) for inp in inputs]
+    return sum(r[0] for r in res), [r[1] for r in res], [r[2] for r in res]
+
+
+def run_gpu_step_concurrent(L, inputs, profile):
+    """One step: a propagation per theta_h, each by its own engine (own HIP stream) on its own host
+    thread, all at once (monoprop_amd.concurrent); returns (term-gate applications, <O> list, simulators).
+    Profiled steps run one after another, so that each launch's event-timed duration is its own."""
+    if profile:
+        return run_gpu_step(L, inputs, profile)
+    from monoprop_amd.concurrent import run_concurrently
+
+    res = run_concurrently([lambda inp=inp: _lone_propagation(L * L, inp, False) for inp in inputs])
+    return sum(r[0] for r in res), [r[1] for r in res], [r[2] for r in res]
 
 
 def _cpu_model() -> str:
@@ -164,14 +181,19 @@ def _digests(words, coeffs) -> tuple[str, str]:
             hashlib.sha256(np.ascontiguousarray(coeffs, dtype=np.float64).tobytes()).hexdigest())
 
 
-def gpu_member_results(L, inputs):
-    """One batched step of the timed workload; per theta_h member its store digests and <O> bits."""
-    _, evs, sims = run_gpu_step_batched(L, inputs, profile=False)
-    w, c, m = sims[0].rows()
+def gpu_member_results(L, inputs, step):
+    """One step of the timed workload; per theta_h its store digests and <O> bits (a batched step's
+    members are cut out of the shared store)."""
+    _, evs, sims = step(L, inputs, profile=False)
     out = []
-    for q, ev in enumerate(evs):
-        sel = m == q
-        out.append((*_digests(w[sel], c[sel]), float(ev).hex()))
+    if step is run_gpu_step_batched:
+        w, c, m = sims[0].rows()
+        for q, ev in enumerate(evs):
+            sel = m == q
+            out.append((*_digests(w[sel], c[sel]), float(ev).hex()))
+    else:
+        for sim, ev in zip(sims, evs):
+            out.append((*_digests(*sim.rows()), float(ev).hex()))
     return out
 
 
@@ -409,7 +431,7 @@ def main():
     if args.schrodinger is not None:
         SCHRODINGER_QUBITS = args.schrodinger
         THETA_H = (0.8,)
-        args.unbatched = True  # the batched store is Heisenberg only
+        args.unbatched, args.batched = True, False  # one propagation: the batched store is Heisenberg only
     if args.theta_h:
         THETA_H = tuple(args.theta_h)
     if args.copy_calibration:
@@ -431,7 +453,10 @@ def main():
         print(json.dumps({"batch": batch_measurement(args.batch_sets, solo=not args.batch_no_solo)}))
         return
     inputs = build_inputs(args.lattice, args.trotter_steps)
-    step = run_gpu_step if args.unbatched else run_gpu_step_batched
+    schedules = {"concurrent": run_gpu_step_concurrent, "batched": run_gpu_step_batched,
+                 "sequential": run_gpu_step}
+    mode = "sequential" if args.unbatched else "batched" if args.batched else "concurrent"
+    step = schedules[mode]
 
     def timed(step_fn):
         torch.cuda.synchronize()
@@ -463,12 +488,22 @@ def main():
     del sims
     prof = {k: v for k, v in prof.items() if v["launches"] > 0}
 
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
-    dname, d = dom
+    # The roofline unit: one gate of the fused pipeline, its propagate (k_sweep) and merge (k_resolve)
+    # launches together.  The path's algorithmic bytes are defined per gate over both, and the two share
+    # the device time about evenly; without them (other pipelines) the kernel with the most device time.
+    # The PMC traffic file covers the default (Heisenberg) workload only.
+    if "k_sweep" in prof and "k_resolve" in prof:
+        dname = "k_sweep + k_resolve"
+        parts = [prof["k_sweep"], prof["k_resolve"]]
+        d = {"ms": sum(x["ms"] for x in parts), "bytes": sum(x["bytes"] for x in parts),
+             "launches": prof["k_sweep"]["launches"]}
+        tr = [measured_traffic(k) for k in ("k_sweep", "k_resolve")]
+        traffic = sum(tr) if SCHRODINGER_QUBITS is None and all(t is not None for t in tr) else None
+    else:
+        dname, d = max(prof.items(), key=lambda kv: kv[1]["ms"])
+        traffic = measured_traffic(dname) if SCHRODINGER_QUBITS is None else None
     avg_ms = d["ms"] / max(d["launches"], 1)
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    # the PMC traffic file covers the default (Heisenberg) workload only
-    traffic = measured_traffic(dname) if SCHRODINGER_QUBITS is None else None
     kern_ms = sum(v["ms"] for v in prof.values())
     kern_bytes = sum(v["bytes"] for v in prof.values())
 
@@ -495,7 +530,7 @@ def main():
             "gates_per_propagation": (2 * args.lattice * (args.lattice - 1) + args.lattice ** 2) * args.trotter_steps,
             "term_gate_applications_per_step": total_t // args.steps,
             "expectation": evs,
-            "batched": not args.unbatched,
+            "schedule": mode,
         },
         "roofline": {
             "bound": "hbm",
@@ -505,11 +540,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "avg_launch_ms": avg_ms,
+            "avg_launch_ms": avg_ms,  # per gate: the sweep's and the resolve's launch together
             "launches": d["launches"],
-            # what the kernel actually moves through HBM (PMC traffic over the event-timed launch); the
+            # what the kernels actually move through HBM (PMC traffic over the event-timed launches); the
             # column index needs far fewer bytes than the limb reads of the algorithmic figure
             "hbm_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+            "hbm_frac": (traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
         },
         "kernels": {
             k: {"ms": round(v["ms"], 3), "GB/s": round(v["bytes"] / (v["ms"] * 1e6), 1) if v["ms"] > 0 else 0.0,
@@ -523,16 +559,21 @@ def main():
         "all_kernels_GBps": kern_bytes / (kern_ms * 1e6) if kern_ms > 0 else 0.0,
         "profiled_step": {"propagate_ms": round(prof_wall_ms, 2), "kernel_ms": round(kern_ms, 2)},
     }
-    if not args.unbatched and not args.no_compare:  # the same steps, one propagation per theta_h, for comparison
-        t_u, evs_u, el_u = timed(run_gpu_step)
-        line["unbatched"] = {"value": t_u / el_u, "ms_per_step": el_u * 1e3 / args.steps,
-                             "max_abs_diff": max(abs(a - b) for a, b in zip(evs, evs_u))}
+    if not args.no_compare and SCHRODINGER_QUBITS is None:  # the same steps under the other schedules
+        line["schedules"] = {}
+        for other, fn in schedules.items():
+            if other == mode:
+                continue
+            t_u, evs_u, el_u = timed(fn)
+            line["schedules"][other] = {"value": t_u / el_u, "ms_per_step": el_u * 1e3 / args.steps,
+                                        "same_expectation_bits": [float(a).hex() == float(b).hex()
+                                                                  for a, b in zip(evs, evs_u)]}
     if not args.no_gradient:
         line["gradient"] = gradient_measurement()
     if args.batch_sets > 0:
         line["batch"] = batch_measurement(args.batch_sets, solo=not args.batch_no_solo)
     if not args.no_cpu_baseline:
-        gpu_res = None if args.unbatched else gpu_member_results(args.lattice, inputs)
+        gpu_res = None if SCHRODINGER_QUBITS is not None else gpu_member_results(args.lattice, inputs, step)
         line["cpu_baseline"] = cpu_baseline(args.lattice, args.trotter_steps, args.cpu_sample_steps, gpu_res)
     print(json.dumps(line))
 

@@ -115,8 +115,8 @@ int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params,
  * at the same parameters (bit for bit) starts from it instead of replaying again. */
 int mpa_set_gradient_checkpointing(mpa_engine *e, int keep, int64_t budget_bytes);
 
-/* No counterpart in the reference: the last gradient's segment count and the bytes its checkpoints
- * and snapshots hold. */
+/* No counterpart in the reference: the last gradient's segment count and the device bytes its plan
+ * needs for the checkpoints and for one segment's snapshots. */
 int mpa_gradient_segments(mpa_engine *e, int64_t *segments, int64_t *checkpoint_bytes,
                           int64_t *snapshot_bytes);
 

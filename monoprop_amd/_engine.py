@@ -16,6 +16,7 @@ import ctypes as C
 import json
 import math
 import os
+import threading
 import weakref
 from collections.abc import Mapping, Sequence
 from pathlib import Path
@@ -92,12 +93,15 @@ EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 # before the C runtime's exit handlers (where the HIP runtime and any profiler attached to it shut
 # down); a destructor that runs after the hook, during interpreter teardown, makes no HIP call.
 _LIVE: weakref.WeakSet = weakref.WeakSet()
+_LIVE_LOCK = threading.Lock()  # engines may be created on several host threads at once
 _SHUTDOWN = False
 
 
 def _destroy_live_engines() -> None:
     global _SHUTDOWN
-    for obj in list(_LIVE):
+    with _LIVE_LOCK:
+        live = list(_LIVE)
+    for obj in live:
         obj._destroy()
     _SHUTDOWN = True
 
@@ -107,7 +111,8 @@ class _EngineHandle:
 
     def _own(self, h) -> None:
         self._h = h
-        _LIVE.add(self)
+        with _LIVE_LOCK:
+            _LIVE.add(self)
 
     def _destroy(self) -> None:
         h = getattr(self, "_h", None)
@@ -125,6 +130,11 @@ def library_path() -> Path:
 
 def load_library():
     """Load the HIP engine; raise ``ImportError`` if it has not been built."""
+    with _LIVE_LOCK:
+        return _load_library_locked()
+
+
+def _load_library_locked():
     global _LIB
     if _LIB is None:
         if not _LIB_PATH.exists():

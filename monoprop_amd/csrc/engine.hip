@@ -952,13 +952,15 @@ constexpr int kWTileWords = 16;                          // sweep: one wavefront
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
 // Count groups: 2^group_log2 tiles (64 tiles = 65536 terms, 16 atomics per word, up to 1024 tiles for
 // the largest stores) so that at most kMaxGroups prefixes sit in LDS.
-constexpr int kMinGroupLog2 = 4, kMaxGroupLog2 = 10;
+constexpr int kMinGroupLog2 = 3, kMaxGroupLog2 = 10;
 constexpr uint32_t kMaxGroups = 4096;
 // per-group count word: leaders | followers << 21 | anticommuting << 42 (each <= 2^20)
 constexpr uint32_t kGrpBits = 21;
 constexpr uint64_t kGrpMask = (uint64_t{1} << kGrpBits) - 1;
 constexpr int kResChunk = kBlock;                         // resolve: one emitter per thread
-constexpr uint32_t kMaxGrid = 1280;                       // resolve blocks that take chunks (5 per CU)
+// resolve blocks: each draws one arrival ticket (one atomic on one address), so a grid much larger than
+// a gate's chunk count costs more than it gives; chunks past the grid go by further tickets
+constexpr uint32_t kMaxGrid = 128;
 constexpr int kBsumBatch = 16;                            // resolve: sweep-block sums read per round
 // per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
 constexpr uint32_t kCntBits = 16;
@@ -1277,13 +1279,20 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
     extern __shared__ uint32_t dyn_lds[];  // group prefixes: leaders [0, max_groups], followers after
     __shared__ uint32_t lds[kWaves];
     __shared__ uint64_t lds64[kWaves];
-    __shared__ uint32_t s_ticket, s_base;
+    __shared__ uint32_t s_ticket, s_base, s_first;
     __shared__ uint32_t st_row[2][kResChunk];  // a chunk's misses: emitter row, new coefficient
     __shared__ double st_ins[2][kResChunk];
     uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
-    // The stop flag, the live count and every group word of the capacity (groups past this gate's are
-    // zero) are all requested before anything waits on them.
+    // The first chunk goes by a ticket drawn on arrival: a chunk's predecessors then belong to blocks
+    // that arrived earlier and are running, so the look-back always ends.  Block index order would not
+    // do: blocks are dispatched per XCD, and an XCD busy with other work (kernels of other streams) can
+    // hold back a lower-indexed block while a higher-indexed one already waits on it.  The ticket, the
+    // stop flag, the live count and every group word of the capacity (groups past this gate's are zero)
+    // are all requested before anything waits on them.
+    uint64_t first_ticket = 0;
+    if (threadIdx.x == 0)
+        first_ticket = atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull);
     const uint64_t stop = counters[C_STOP];
     const uint64_t N = counters[C_NB + (gi & 1)];
     const uint32_t gper = (max_groups + kBlock - 1) / kBlock;  // <= kMaxGroups / kBlock
@@ -1308,8 +1317,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
         mine += (gv[i] & kGrpMask) | (((gv[i] >> kGrpBits) & kGrpMask) << 32);
         mine_a += static_cast<uint32_t>(gv[i] >> (2 * kGrpBits));
     }
+    if (threadIdx.x == 0) s_first = static_cast<uint32_t>(first_ticket);
     uint64_t tot;
-    uint64_t x = block_excl_scan64(mine, lds64, &tot);
+    uint64_t x = block_excl_scan64(mine, lds64, &tot);  // (its barriers also publish s_first)
     const uint32_t nA = block_sum(mine_a, lds);
     const uint32_t nEL = static_cast<uint32_t>(tot), nEF = static_cast<uint32_t>(tot >> 32);
 #pragma unroll
@@ -1331,22 +1341,23 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
         st[2] = static_cast<int64_t>(nEL) + static_cast<int64_t>(nEF);
         st[4] = static_cast<int64_t>(nEL);
     }
-    if (blockIdx.x >= nchunks) return;  // only blocks with work take chunks
-    // Chunks in order: block b takes chunk b first (blocks are dispatched in index order, so every
-    // earlier chunk's block has started), then work tickets after the first min(nchunks, grid) chunks.
-    // One chunk in flight behind the current one: chunk c is resolved and its misses staged (LDS) while
-    // the previous chunk of this block, whose look-back has had chunk c's time to settle, is appended.
-    const uint32_t first_round = nchunks < gridDim.x ? nchunks : gridDim.x;
+    const uint32_t ticket0 = s_first;
+    if (ticket0 >= nchunks) return;  // only blocks with work take chunks
+    // Chunk c is the one of ticket c: a block's arrival ticket first, then (only when there are more
+    // chunks than blocks: every block draws one arrival ticket, so otherwise those cover all chunks)
+    // further tickets from the same counter.  Every chunk publishes its aggregate in the iteration that
+    // draws it, before its block waits on anything.  One chunk in flight behind the current one: chunk
+    // c is resolved and its misses staged (LDS) while the previous chunk of this block, whose look-back
+    // has had chunk c's time to settle, is appended.
+    const bool more = nchunks > gridDim.x;
     uint32_t pend = 0xffffffffu, pend_miss = 0, buf = 0;
     for (bool first = true;; first = false) {
         __syncthreads();  // LDS of the previous iteration fully consumed (and the group prefixes written)
-        // (when every chunk was handed out in the first round, no ticket is needed)
         if (threadIdx.x == 0)
-            s_ticket = first ? blockIdx.x
-                       : first_round == nchunks
-                           ? 0xffffffffu
-                           : first_round + static_cast<uint32_t>(atomicAdd(
-                                               reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
+            s_ticket = first ? ticket0
+                       : !more ? 0xffffffffu
+                               : static_cast<uint32_t>(atomicAdd(
+                                     reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
         __syncthreads();
         const uint32_t chunk = s_ticket;
         const bool have = chunk < nchunks;
@@ -4324,10 +4335,12 @@ int mpa_set_gradient_checkpointing(mpa_engine *e, int keep, int64_t budget_bytes
 
 int mpa_gradient_segments(mpa_engine *e, int64_t *segments, int64_t *checkpoint_bytes, int64_t *snapshot_bytes) {
     return guarded([&] {
+        // what the last plan needs (its arrays may be larger: they only grow, and cached device blocks
+        // can be larger than asked for)
         const size_t ns = e->seg_start.empty() ? 0 : e->seg_start.size() - 1;
         *segments = static_cast<int64_t>(ns);
-        *checkpoint_bytes = static_cast<int64_t>(e->ckpt.bytes);
-        *snapshot_bytes = static_cast<int64_t>(e->snap_c.bytes + e->snap_r.bytes);
+        *checkpoint_bytes = ns > 1 ? static_cast<int64_t>((ns - 1) * std::max<uint64_t>(e->N, 1) * 8) : 0;
+        *snapshot_bytes = ns ? static_cast<int64_t>(e->seg_snap_c * 8 + e->seg_snap_r * 16) : 0;
     });
 }
 
