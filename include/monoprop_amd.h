@@ -226,9 +226,10 @@ int mpa_batch_create(mpa_engine **out, int basis, int num_modes, uint32_t cutoff
  * params[q * n_params + param_map[i]] * gen_coeffs[i].  Runs the fused pipeline for every member
  * (-2 if some member has cos 2a == 0 at a gate: propagate that member alone).
  * max_capacity bounds the store: propagation stops before the first layer that would grow it past
- * max_capacity rows (0: no bound; < 0: the largest power of two whose growth fits in 3/4 of the
- * device).  *layers_done receives the layers applied: the last *layers_done layers in authoring
- * order; the first n_layers - *layers_done are still to apply. */
+ * max_capacity rows (0: no bound; < 0: automatic, the largest multiple of 16384 rows whose growth,
+ * the old and the new store at once, fits in 3/4 of the device memory available now, divided among
+ * -max_capacity stores growing at once).  *layers_done receives the layers applied: the last
+ * *layers_done layers in authoring order; the first n_layers - *layers_done are still to apply. */
 int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t *gen_slots,
                         int64_t n_layers, const int64_t *param_map, const double *gen_coeffs,
                         const double *params, int64_t n_params, int64_t max_capacity,

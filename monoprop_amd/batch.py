@@ -166,7 +166,8 @@ def _halves(sizes: np.ndarray) -> np.ndarray:
 def expectation_grid(circuit: Circuit, operators: Sequence[Operator], parameter_sets, *, cutoff: int,
                      lower_atol: float | None = None, upper_atol: float | None = None,
                      cutoff_type: str | None = None, max_members: int = BatchSimulator.max_members,
-                     max_capacity: int = -1, stats: list | None = None, progress=None) -> np.ndarray:
+                     max_capacity: int = -1, stats: list | None = None, progress=None,
+                     workers: int = 1) -> np.ndarray:
     """``<O_j>(theta_i)`` for every parameter set ``i`` and observable ``j``: shape
     ``(len(parameter_sets), len(operators))``.
 
@@ -178,10 +179,28 @@ def expectation_grid(circuit: Circuit, operators: Sequence[Operator], parameter_
     member is never bounded.  ``stats``, when given, receives one ``(members, stored terms, device
     ms, split, term-gate applications)`` tuple per propagate call, ``split`` true when the call
     stopped at the bound; ``progress``, when given, is called with each such tuple.
+
+    ``workers`` > 1 cuts the parameter sets into that many contiguous groups and runs one driver per
+    group at once (``monoprop_amd.concurrent``: own engines, own streams, own host threads); the
+    automatic bound then shares the available memory among them.  Every value is the same to the bit
+    whatever the grouping (a member's propagation does not depend on its batch).
     """
     params = np.asarray(parameter_sets, dtype=np.float64)
     if params.ndim != 2:
         raise ValueError("parameter_sets must be a 2-D array (sets x circuit parameters)")
+    if workers < 1:
+        raise ValueError("workers must be at least 1")
+    if workers > 1 and params.shape[0] > 1:
+        from .concurrent import run_concurrently
+
+        groups = [g for g in np.array_split(np.arange(params.shape[0]), min(workers, params.shape[0])) if len(g)]
+        cap = -len(groups) if max_capacity < 0 else max_capacity
+        parts = run_concurrently([
+            lambda g=g: expectation_grid(circuit, operators, params[g], cutoff=cutoff, lower_atol=lower_atol,
+                                         upper_atol=upper_atol, cutoff_type=cutoff_type, max_members=max_members,
+                                         max_capacity=cap, stats=stats, progress=progress)
+            for g in groups])
+        return np.concatenate(parts, axis=0)
     n_sets, n_ops = params.shape[0], len(operators)
     pairs = [(i, j) for i in range(n_sets) for j in range(n_ops)]
     out = np.empty((n_sets, n_ops), dtype=np.float64)

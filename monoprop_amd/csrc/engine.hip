@@ -3937,7 +3937,7 @@ int mpa_batch_propagate(mpa_engine *e, const int64_t *gen_offsets, const int32_t
                                  static_cast<double>(BlockCache::get().cached_bytes(e->device)) +
                                  e->bytes_per_row() * static_cast<double>(e->cap);
             // the last growth holds the old store and the new one (1.5 times larger) at once
-            const double rows = 0.75 * avail / (2.5 * e->bytes_per_row());
+            const double rows = 0.75 * avail / (2.5 * e->bytes_per_row()) / static_cast<double>(-max_capacity);
             e->cap_limit = std::max<uint64_t>(static_cast<uint64_t>(rows) / mpa_engine::kGranule, 1) * mpa_engine::kGranule;
         } else {
             e->cap_limit = static_cast<uint64_t>(max_capacity);

@@ -5,8 +5,8 @@
   oracle's golden digests and its <O> equals the oracle's to the bit (the lone propagations are held
   to the same digests by test_gpu_golden.py);
 * the parameter sweep's own inputs (its eight observables at its first two parameter sets, cutoff 8,
-  lower_atol 1e-5, 3 Trotter steps): one batch of 16 members, and the sweep driver forced to split
-  the batch on the device, both against the oracle's golden digests and <O> bits;
+  lower_atol 1e-5, 3 Trotter steps): one batch of 16 members, the sweep driver forced to split the
+  batch on the device, and two drivers at once, against the oracle's golden digests and <O> bits;
 * 64-qubit brickwork ring, all 12 layers (about 2300 angles), the gradient of the graph built at the
   seeded angles: <O> of the replay against the oracle's propagation (1e-10 relative: the replay
   applies the rotations without the fused sweep's (v k)(1/k) recovery), the adjoint gradient against
@@ -91,6 +91,19 @@ def test_sweep_inputs_with_device_splits():
     grid = expectation_grid(circ, obs, workloads.kicked_ising_rows(circ, sets), cutoff=8, lower_atol=1e-5,
                             max_members=16, max_capacity=1 << 21, stats=stats)
     assert any(s[3] for s in stats), "the bound should have forced at least one split"
+    for i in range(2):
+        for j in range(len(obs)):
+            assert float(grid[i, j]).hex() == VECTORS[f"sweep_set{i}_obs{j}"]["expectation_hex"], (i, j)
+
+
+@pytest.mark.parametrize("max_capacity", [-1, 1 << 21])
+def test_sweep_inputs_concurrent_drivers(max_capacity):
+    """Two sweep drivers at once (workers=2, each set on its own engines and stream; the automatic bound
+    shares the memory, the small one forces splits): every <O> keeps the oracle's bits."""
+    print(f"case: sweep driver, 2 workers, max_capacity={max_capacity}", flush=True)
+    circ, obs, sets = _sweep_inputs()
+    grid = expectation_grid(circ, obs, workloads.kicked_ising_rows(circ, sets), cutoff=8, lower_atol=1e-5,
+                            max_members=16, max_capacity=max_capacity, workers=2)
     for i in range(2):
         for j in range(len(obs)):
             assert float(grid[i, j]).hex() == VECTORS[f"sweep_set{i}_obs{j}"]["expectation_hex"], (i, j)
