@@ -445,11 +445,11 @@ __device__ __forceinline__ void rotate_pair(const GateDesc &gd, double vS, doubl
     *cT = wT + gd.s * static_cast<double>(ph) * vS;
 }
 
-// Graph recording (build_graph): cos-only rows and rotation pairs of the layer, appended in any order
-// (each row is touched by exactly one record, so replay order inside a layer is irrelevant).
+// Graph recording (build_graph): the rotation pairs of the layer, appended in any order (each row is
+// touched by at most one pair, so replay order inside a layer is irrelevant).  Cos-only rows are only
+// counted (C_GCOS): the replay recomputes them from the column index.
 struct GraphRec {
-    uint32_t *cos;  // null when not recording
-    uint2 *rot;     // {source row, target row | phase bit}
+    uint2 *rot;  // {source row, target row | phase bit}; null when not recording
 };
 
 template <int NW, int PAULI>
@@ -468,7 +468,6 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
     bool emit = false, paired = false;
     // graph records of this thread: up to two cos-only rows, or one rotation
     bool rc0 = false, rc1 = false, rr = false;
-    uint32_t c0 = 0, c1 = 0;
     uint2 rot = {0u, 0u};
     if (f < nF) {
         const uint32_t item = nL + f;
@@ -519,8 +518,6 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
                 cL = cosL ? vL * gd.k : vL;
                 cF = in_cos ? v * gd.k : v;
                 rc0 = rc1 = true;
-                c0 = rl;
-                c1 = r;
             }
             coef[rl] = cL;
             coef[r] = cF;
@@ -534,7 +531,6 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
             } else {
                 coef[r] = w;
                 rc0 = true;
-                c0 = r;
             }
         }
         miss[item] = is_miss;
@@ -546,11 +542,10 @@ __global__ __launch_bounds__(kBlock) void k_followers(const GateDesc *__restrict
     const uint64_t pball = __ballot(paired);
     if (lane_id() == 0 && pball) atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_NPAIR),
                                            static_cast<unsigned long long>(__popcll(pball)));
-    if (grec.cos != nullptr) {
-        const uint64_t p0 = wave_append(rc0, counters + C_GCOS);
-        if (rc0) grec.cos[p0] = c0;
-        const uint64_t p1 = wave_append(rc1, counters + C_GCOS);
-        if (rc1) grec.cos[p1] = c1;
+    if (grec.rot != nullptr) {
+        const uint64_t nc = __popcll(__ballot(rc0)) + __popcll(__ballot(rc1));
+        if (lane_id() == 0 && nc)
+            atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_GCOS), static_cast<unsigned long long>(nc));
         const uint64_t pr = wave_append(rr, counters + C_GROT);
         if (rr) grec.rot[pr] = rot;
     }
@@ -570,7 +565,6 @@ __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__
     const GateDesc &gd = gdesc[gi];
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     bool rc = false;
-    uint32_t row = 0;
     if (b < nL) {
         if (match[b]) {
             miss[b] = 0;
@@ -580,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__
             const int ph = (lc & 2u) ? -1 : 1;
             const double v = lv[b];
             const double w = in_cos ? v * gd.k : v;
-            row = AL[b];
+            const uint32_t row = AL[b];
             if (e) {
                 double vT0 = 0.0;
                 if (occ_dev) {
@@ -600,10 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_leaders_unmatched(const GateDesc *__
             }
         }
     }
-    if (grec.cos != nullptr) {
-        const uint64_t p = wave_append(rc, counters + C_GCOS);
-        if (rc) grec.cos[p] = row;
-    }
+    if (grec.rot != nullptr) (void)wave_append(rc, counters + C_GCOS);
 }
 
 // Per-block counts of miss flags over the combined item range [0, nA).
@@ -650,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void k_insert(const GateDesc *__restrict__ 
         coef[dest] = ins[i];
         if (zero_fill != nullptr) zero_fill[dest] = occ_dev ? fresh_score<NW, PAULI>(t, occ_dev) : 0.0;
     }
-    if (grec.cos != nullptr) {
+    if (grec.rot != nullptr) {
         const uint64_t p = wave_append(v != 0, counters + C_GROT);
         if (v) grec.rot[p] = make_uint2(src, static_cast<uint32_t>(dest) | ((flag & 2u) ? kPhaseBit : 0u));
     }
@@ -1680,53 +1671,181 @@ __global__ __launch_bounds__(kBlock) void k_seq_sum(const double *__restrict__ v
     if (owner) out[q] = r;
 }
 
-// One graph layer replayed at angle alpha (reference evolve_step, Evolution.cpp:471-512): cos-only
-// rows c *= k; rotations c_S = c_S k + (s (-ph)) c_T, c_T = c_T k + (s ph) c_S from the pre-layer values.
-// With `snap_c` / `snap_r` set, the pre-layer values are kept for the adjoint sweep, indexed from the
-// segment's first cos-only record sc0 and first rotation sr0.
-__global__ __launch_bounds__(kBlock) void k_replay(const uint32_t *__restrict__ gcos, const uint2 *__restrict__ grot,
-                                                   uint64_t cb, uint64_t ncos, uint64_t rb, uint64_t nrot, double k,
-                                                   double s, double *__restrict__ c, double *__restrict__ snap_c,
-                                                   double *__restrict__ snap_r, uint64_t sc0, uint64_t sr0) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (i < ncos) {
-        const uint32_t r = gcos[cb + i];
-        const double v = c[r];
-        if (snap_c) snap_c[cb + i - sc0] = v;
-        c[r] = v * k;
-    } else if (i < ncos + nrot) {
-        const uint64_t j = rb + (i - ncos);
-        const uint2 e = grot[j];
-        const uint32_t S = e.x, T = e.y & ~kPhaseBit;
-        const int ph = (e.y & kPhaseBit) ? -1 : 1;
-        const double vS = c[S], vT = c[T];
-        if (snap_r) {
-            snap_r[2 * (j - sr0)] = vS;
-            snap_r[2 * (j - sr0) + 1] = vT;
-        }
-        c[S] = vS * k + s * static_cast<double>(-ph) * vT;
-        c[T] = vT * k + s * static_cast<double>(ph) * vS;
-    }
+// Graph layers (reference evolve_step, Evolution.cpp:471-512).  A layer stores only its rotation
+// records; its cos-only rows are recomputed from the column index (CosineRecompute.h:17-20): the rows
+// below the layer's live count n_before that anticommute with G (XOR of G's fold columns) and belong to
+// no rotation record.  Every anticommuting row of the layer is in exactly one of the two sets, so the
+// rotation kernel marks its rows below n_before in `pmark` (one byte per row, plain stores: two
+// records never share a row) and the cosine kernel, launched after it, skips the marked rows and
+// clears their marks, leaving `pmark` zero.
+//
+// The cosine kernels run one wavefront per 64 words (4096 rows): lane l folds G's columns for word l,
+// the set bits are listed in row order in LDS (12-bit offsets in the chunk), and the list is walked
+// densely, 64 rows per round and two rounds in flight, each lane loading a row's mark and coefficient
+// together.  A wave's cos-only rows are numbered in row order from its chunk offset `cofs[chunk]`
+// (computed when the graph was built), which places the pre-layer values of the adjoint's snapshots.
+constexpr int kCosChunkWords = 64;
+constexpr int kCosChunkRows = kCosChunkWords * 64;
+
+__device__ __forceinline__ void pmark_pair(uint8_t *pmark, uint32_t S, uint32_t T, uint64_t n_before) {
+    pmark[S] = 1;
+    if (T < n_before) pmark[T] = 1;
 }
 
-// Adjoint of one layer: d<O>/d alpha from the pre-layer snapshots, then lambda <- R^T lambda.
-// cos-only: -2 s c_i; pair (S,T,ph): -2s c_S - 2k ph c_T and -2s c_T + 2k ph c_S.  Per-block sums of the
-// derivative go to partial[pb + block].
-__global__ __launch_bounds__(kBlock) void k_adjoint(const uint32_t *__restrict__ gcos, const uint2 *__restrict__ grot,
-                                                    uint64_t cb, uint64_t ncos, uint64_t rb, uint64_t nrot, double k,
-                                                    double s, const double *__restrict__ snap_c,
-                                                    const double *__restrict__ snap_r, double *__restrict__ lam,
-                                                    double *__restrict__ partial, uint64_t pb, uint64_t sc0, uint64_t sr0) {
+// G's fold columns of one graph layer, passed by value so that no load precedes the column loads
+// (generators with more than kLayerCols columns read the rest from `fold` in memory).
+constexpr int kLayerCols = 8;
+struct LayerCols {
+    int32_t col[kLayerCols];
+    int32_t n;
+};
+
+// G's anticommuting bits of word w among rows [0, n_before) (0 past them).
+__device__ __forceinline__ uint64_t layer_fold_bits(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                    const LayerCols &lc, const int32_t *__restrict__ fold,
+                                                    uint64_t n_before, uint64_t w) {
+    const uint64_t nwords = (n_before + 63) / 64;
+    if (w >= nwords) return 0;
+    uint64_t y[kLayerCols];
+#pragma unroll
+    for (int f = 0; f < kLayerCols; ++f) y[f] = f < lc.n ? cols[static_cast<uint64_t>(lc.col[f]) * colstride + w] : 0;
+    uint64_t bits = 0;
+#pragma unroll
+    for (int f = 0; f < kLayerCols; ++f) bits ^= y[f];
+    for (int f = kLayerCols; f < lc.n; ++f) bits ^= cols[static_cast<uint64_t>(fold[f]) * colstride + w];
+    if (w == nwords - 1 && (n_before & 63)) bits &= (uint64_t{1} << (n_before & 63)) - 1;
+    return bits;
+}
+
+// The cos-only rows of this wave's chunk in row order, rank counted from `rank0`; marked rows have their
+// mark cleared.  MODE 0: replay (c *= k, pre-layer value to snap[rank] when snap is set), 1: adjoint
+// (d += lam (-2 s snap[rank]), lam *= k), 2: count only.  `list` is the wave's LDS list.  Returns the
+// number of cos-only rows.
+template <int MODE>
+__device__ __forceinline__ uint64_t layer_cos_chunk(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                    const LayerCols &lc, const int32_t *__restrict__ fold,
+                                                    uint64_t n_before,
+                                                    uint8_t *__restrict__ pmark, uint64_t chunk, double k, double s,
+                                                    double *__restrict__ c, double *__restrict__ snap,
+                                                    uint64_t rank0, double *__restrict__ lam, double *d_out,
+                                                    uint16_t *list) {
+    const uint32_t lane = lane_id();
+    const uint64_t row0 = chunk * kCosChunkRows;
+    const uint64_t mine = layer_fold_bits(cols, colstride, lc, fold, n_before, chunk * kCosChunkWords + lane);
+    const uint32_t cnt = static_cast<uint32_t>(__popcll(mine));
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total == 0) return 0;
+    {
+        uint32_t pos = incl - cnt;
+        for (uint64_t b = mine; b; b &= b - 1)
+            list[pos++] = static_cast<uint16_t>(lane * 64 + static_cast<uint32_t>(__builtin_ctzll(b)));
+    }
+    wave_lds_sync();
+    const uint64_t below = (uint64_t{1} << lane) - 1;
+    uint64_t rank = rank0;
+    double d = 0.0;
+    for (uint32_t e0 = 0; e0 < total; e0 += 128) {
+        uint64_t r[2];
+        uint8_t pm[2];
+        double v[2];
+        bool in[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t e = e0 + q * 64 + lane;
+            in[q] = e < total;
+            r[q] = in[q] ? row0 + list[e] : 0;
+            pm[q] = 0;
+            v[q] = 0.0;
+            if (in[q]) {
+                pm[q] = pmark[r[q]];
+                if (MODE == 0) v[q] = c[r[q]];
+                if (MODE == 1) v[q] = lam[r[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const bool cos_row = in[q] && pm[q] == 0;
+            const uint64_t cm = __ballot(cos_row);
+            if (in[q] && pm[q]) pmark[r[q]] = 0;
+            if (cos_row) {
+                const uint64_t at = rank + static_cast<uint64_t>(__popcll(cm & below));
+                if (MODE == 0) {
+                    if (snap) snap[at] = v[q];
+                    c[r[q]] = v[q] * k;
+                } else if (MODE == 1) {
+                    d += v[q] * (-2.0 * s * snap[at]);
+                    lam[r[q]] = v[q] * k;
+                }
+            }
+            rank += static_cast<uint64_t>(__popcll(cm));
+        }
+    }
+    wave_lds_sync();  // the list is reused by the wave's next caller only after every lane has read it
+    if (MODE == 1) *d_out = d;
+    return rank - rank0;
+}
+
+// Rotations of one layer at angle alpha: c_S = c_S k + (s (-ph)) c_T, c_T = c_T k + (s ph) c_S from the
+// pre-layer values (kept in snap_r, indexed from the segment's first rotation sr0, when set); with
+// `pmark` set, the pair's rows below n_before are marked for the cosine kernel.
+__global__ __launch_bounds__(kBlock) void k_layer_rot(const uint2 *__restrict__ grot, uint64_t rb, uint64_t nrot,
+                                                      double k, double s, double *__restrict__ c,
+                                                      double *__restrict__ snap_r, uint64_t sr0,
+                                                      uint8_t *__restrict__ pmark, uint64_t n_before) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= nrot) return;
+    const uint64_t j = rb + i;
+    const uint2 e = grot[j];
+    const uint32_t S = e.x, T = e.y & ~kPhaseBit;
+    const int ph = (e.y & kPhaseBit) ? -1 : 1;
+    const double vS = c[S], vT = c[T];
+    if (snap_r) {
+        snap_r[2 * (j - sr0)] = vS;
+        snap_r[2 * (j - sr0) + 1] = vT;
+    }
+    c[S] = vS * k + s * static_cast<double>(-ph) * vT;
+    c[T] = vT * k + s * static_cast<double>(ph) * vS;
+    if (pmark) pmark_pair(pmark, S, T, n_before);
+}
+
+// Cosine rows of one layer: c *= k (pre-layer values to snap_c from snap_base when set).
+__global__ __launch_bounds__(kBlock) void k_layer_cos(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                      LayerCols lc, const int32_t *__restrict__ fold, uint64_t n_before,
+                                                      double k, double *__restrict__ c, uint8_t *__restrict__ pmark,
+                                                      double *__restrict__ snap_c, uint64_t snap_base,
+                                                      const uint32_t *__restrict__ cofs, uint64_t nchunks) {
+    __shared__ uint16_t list[kWaves][kCosChunkRows];
+    const uint64_t chunk = static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    if (chunk >= nchunks) return;
+    const uint64_t rank0 = snap_c ? snap_base + cofs[chunk] : 0;
+    (void)layer_cos_chunk<0>(cols, colstride, lc, fold, n_before, pmark, chunk, k, 0.0, c, snap_c, rank0, nullptr,
+                             nullptr, list[threadIdx.x >> 6]);
+}
+
+// Sum of one block's derivative terms into partial[blockIdx.x] (fixed tree order).
+__device__ __forceinline__ void block_partial(double d, double *red, double *__restrict__ partial) {
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(st)) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adjoint of one layer's rotations: d<O>/d alpha from the pre-layer snapshots, then lambda <- R^T lambda;
+// pair (S,T,ph): -2s c_S - 2k ph c_T and -2s c_T + 2k ph c_S.  Marks the pair's rows like k_layer_rot.
+__global__ __launch_bounds__(kBlock) void k_layer_rot_adj(const uint2 *__restrict__ grot, uint64_t rb, uint64_t nrot,
+                                                          double k, double s, const double *__restrict__ snap_r,
+                                                          uint64_t sr0, double *__restrict__ lam,
+                                                          double *__restrict__ partial,
+                                                          uint8_t *__restrict__ pmark, uint64_t n_before) {
     __shared__ double red[kBlock];
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     double d = 0.0;
-    if (i < ncos) {
-        const uint32_t r = gcos[cb + i];
-        const double l = lam[r];
-        d = l * (-2.0 * s * snap_c[cb + i - sc0]);
-        lam[r] = l * k;
-    } else if (i < ncos + nrot) {
-        const uint64_t j = rb + (i - ncos);
+    if (i < nrot) {
+        const uint64_t j = rb + i;
         const uint2 e = grot[j];
         const uint32_t S = e.x, T = e.y & ~kPhaseBit;
         const double ph = (e.y & kPhaseBit) ? -1.0 : 1.0;
@@ -1736,14 +1855,50 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(const uint32_t *__restrict__
         d += lt * (-2.0 * s * ct + 2.0 * k * ph * cs);
         lam[S] = k * ls + s * ph * lt;
         lam[T] = k * lt - s * ph * ls;
+        if (pmark) pmark_pair(pmark, S, T, n_before);
     }
-    red[threadIdx.x] = d;
-    __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < static_cast<unsigned>(st)) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[pb + blockIdx.x] = red[0];
+    block_partial(d, red, partial);
+}
+
+// Adjoint of one layer's cosine rows: -2 s c_i lambda_i, then lambda_i *= k.
+__global__ __launch_bounds__(kBlock) void k_layer_cos_adj(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                          LayerCols lc, const int32_t *__restrict__ fold,
+                                                          uint64_t n_before, double k, double s,
+                                                          uint8_t *__restrict__ pmark,
+                                                          const double *__restrict__ snap_c, uint64_t snap_base,
+                                                          const uint32_t *__restrict__ cofs, uint64_t nchunks,
+                                                          double *__restrict__ lam, double *__restrict__ partial) {
+    __shared__ double red[kBlock];
+    __shared__ uint16_t list[kWaves][kCosChunkRows];
+    const uint64_t chunk = static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    double d = 0.0;
+    if (chunk < nchunks)
+        (void)layer_cos_chunk<1>(cols, colstride, lc, fold, n_before, pmark, chunk, k, s, nullptr,
+                                 const_cast<double *>(snap_c), snap_base + cofs[chunk], lam, &d,
+                                 list[threadIdx.x >> 6]);
+    block_partial(d, red, partial);
+}
+
+// Graph set-up: marks of one layer's rotation rows (below n_before).
+__global__ __launch_bounds__(kBlock) void k_layer_mark(const uint2 *__restrict__ grot, uint64_t rb, uint64_t nrot,
+                                                       uint8_t *__restrict__ pmark, uint64_t n_before) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= nrot) return;
+    const uint2 e = grot[rb + i];
+    pmark_pair(pmark, e.x, e.y & ~kPhaseBit, n_before);
+}
+
+// Graph set-up: cos-only rows of one layer per wave chunk (out[chunk]).
+__global__ __launch_bounds__(kBlock) void k_layer_cos_count(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                            LayerCols lc, const int32_t *__restrict__ fold,
+                                                            uint64_t n_before, uint8_t *__restrict__ pmark,
+                                                            uint32_t *__restrict__ out, uint64_t nchunks) {
+    __shared__ uint16_t list[kWaves][kCosChunkRows];
+    const uint64_t chunk = static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    if (chunk >= nchunks) return;
+    const uint64_t n = layer_cos_chunk<2>(cols, colstride, lc, fold, n_before, pmark, chunk, 1.0, 0.0, nullptr,
+                                          nullptr, 0, nullptr, nullptr, list[threadIdx.x >> 6]);
+    if (lane_id() == 0) out[chunk] = static_cast<uint32_t>(n);
 }
 
 // lambda_i = <ref| M_i |ref>: the diagonal phase of fully paired rows, 0 elsewhere.
@@ -2046,19 +2201,27 @@ struct mpa_engine {
         tr.bchg = bchg_dev.as<uint64_t>();
     }
 
-    // Graph (build_graph): per-layer cos-only rows and rotation records, the coefficient vector the
-    // seeded build evolves, the operator coefficients the graph starts from, and replay scratch.
+    // Graph (build_graph): per layer the rotation records and what recomputes its cos-only rows (the
+    // live count before it, G's fold columns, per 256-word block the cos-only rows before the block),
+    // the coefficient vector the seeded build evolves, the operator coefficients the graph starts
+    // from, and replay scratch.  cos_begin / cos_end number the layer's cos-only rows in build order
+    // (they place its pre-layer values in a segment's snapshot).
     struct LayerInfo {
         uint64_t cos_begin, cos_end, rot_begin, rot_end;
         int64_t param_index;
         double gen_coeff;
+        uint64_t n_before;
+        int32_t fold_off, fold_cnt;  // in graph_folds / g_fold
+        uint64_t cofs_off;           // first block offset in g_cofs
     };
     std::vector<LayerInfo> layers;
+    std::vector<int32_t> graph_folds;
     int64_t graph_nparams = 0;
-    DevArray g_cos{&stream}, g_rot{&stream}, seedc{&stream}, base{&stream}, rep{&stream}, snap_c{&stream},
-        snap_r{&stream}, lam{&stream}, gpartial{&stream}, goffs{&stream}, gdl{&stream};
+    DevArray g_rot{&stream}, g_fold{&stream}, g_cofs{&stream}, pmark{&stream}, seedc{&stream}, base{&stream},
+        rep{&stream}, snap_c{&stream}, snap_r{&stream}, lam{&stream}, gpartial{&stream}, goffs{&stream}, gdl{&stream};
     uint64_t base_n = 0;
-    uint64_t gcos_total = 0, grot_total = 0;  // records stored
+    uint64_t gcos_total = 0, grot_total = 0;  // cos-only rows counted, rotation records stored
+    uint64_t cofs_total = 0;                  // block offsets stored
     // Checkpointed adjoint: coefficient vectors at segment starts; the byte budget for them plus one
     // segment's snapshots; with keep_ckpt, mpa_expectation_value runs the checkpointed forward and a
     // following gradient at the same parameters (ckpt_params, to the bit) starts from it.
@@ -2375,7 +2538,7 @@ struct mpa_engine {
         HIP_TRY(hipEventRecord(ev[0], stream));
         std::vector<int32_t> fold_plus_piv(ng, 0);
         // in a graph build the seed vector is evolved and the operator's own coefficients get zeros
-        uint64_t gcos_ub = gcos_total, grot_ub = grot_total;
+        uint64_t grot_ub = grot_total;
         if (graph) {
             const int64_t gc[2] = {static_cast<int64_t>(gcos_total), static_cast<int64_t>(grot_total)};
             HIP_TRY(hipMemcpyAsync(counters.as<uint64_t>() + C_GCOS, gc, 16, hipMemcpyHostToDevice, stream));
@@ -2427,16 +2590,13 @@ struct mpa_engine {
             ensure_items(nL, nA);
             double *cv = graph ? seedc.as<double>() : coef.as<double>();
             const uint64_t *occ_s = schr ? occ_dev.as<uint64_t>() : nullptr;  // Schrodinger: fresh rows scored
-            GraphRec grec{nullptr, nullptr};
+            GraphRec grec{nullptr};
             if (graph && nA > 0) {
-                // every anticommuting term becomes at most one record (cos-only, or an endpoint of a
-                // rotation), on top of the records written so far (counted on the device, read with
-                // the gate's counts above)
-                gcos_ub = static_cast<uint64_t>(h_counters[C_GCOS]) + nA;
+                // every anticommuting term is cos-only or an endpoint of at most one rotation, on top
+                // of the records written so far (counted on the device, read with the gate's counts)
                 grot_ub = static_cast<uint64_t>(h_counters[C_GROT]) + nA;
-                grow_keep(g_cos, gcos_ub * 4);
                 grow_keep(g_rot, grot_ub * 8);
-                grec = GraphRec{g_cos.as<uint32_t>(), g_rot.as<uint2>()};
+                grec = GraphRec{g_rot.as<uint2>()};
             }
             if (nA > 0) {
                 ++epoch;
@@ -2529,7 +2689,14 @@ struct mpa_engine {
             for (size_t g = 0; g < ng; ++g) {
                 const uint64_t ce = static_cast<uint64_t>(st[kStatW * g + 6]);
                 const uint64_t re = static_cast<uint64_t>(st[kStatW * g + 7]);
-                layers.push_back({cb, ce, rb, re, gds[g].param_index, gds[g].gen_coeff});
+                LayerInfo ly{cb, ce, rb, re, gds[g].param_index, gds[g].gen_coeff,
+                             static_cast<uint64_t>(st[kStatW * g]), static_cast<int32_t>(graph_folds.size()), 0, 0};
+                if (ce > cb) {  // the fold columns that recompute its cos-only rows
+                    graph_folds.insert(graph_folds.end(), folds.begin() + gds[g].fold_off,
+                                       folds.begin() + gds[g].fold_off + gds[g].fold_cnt);
+                    ly.fold_cnt = gds[g].fold_cnt;
+                }
+                layers.push_back(ly);
                 cb = ce;
                 rb = re;
             }
@@ -2559,18 +2726,106 @@ struct mpa_engine {
     size_t replay_layer_at(ReplayKind kind, size_t pos) const {
         return kind == kObservable ? layers.size() - 1 - pos : pos;
     }
-    // One layer of the replay on `rep`; with `snaps` its pre-layer values go to the snapshot buffers
-    // (indexed from the records sc0 / sr0).
+    // The pair-row marks of the cosine recompute: one byte per row, all zero between layers.
+    void ensure_pmark() {
+        const uint64_t need = std::max<uint64_t>(N, 1);
+        if (pmark.bytes >= need) return;
+        pmark.release();
+        pmark.reserve(need);
+        HIP_TRY(hipMemsetAsync(pmark.p, 0, pmark.bytes, stream));  // all of it: later sizes reuse the block
+    }
+    // 64-word chunks (one wavefront each) of a layer's cosine kernels, their blocks, its rotation blocks.
+    static uint64_t cos_chunks(const LayerInfo &ly) {
+        return ly.cos_end > ly.cos_begin ? ((ly.n_before + 63) / 64 + kCosChunkWords - 1) / kCosChunkWords : 0;
+    }
+    static uint32_t cos_blocks(const LayerInfo &ly) {
+        return static_cast<uint32_t>((cos_chunks(ly) + kWaves - 1) / kWaves);
+    }
+    LayerCols layer_cols(const LayerInfo &ly) const {
+        LayerCols lc{};
+        lc.n = ly.fold_cnt;
+        for (int f = 0; f < std::min(ly.fold_cnt, kLayerCols); ++f) lc.col[f] = graph_folds[static_cast<size_t>(ly.fold_off + f)];
+        return lc;
+    }
+    static uint32_t rot_blocks(const LayerInfo &ly) {
+        return static_cast<uint32_t>((ly.rot_end - ly.rot_begin + kBlock - 1) / kBlock);
+    }
+    // One layer of the replay on `rep`: its rotations, then its recomputed cos-only rows; with `snaps`
+    // the pre-layer values go to the snapshot buffers (indexed from the segment's first cos-only row
+    // sc0 and first rotation sr0).
     void replay_layer(const LayerInfo &ly, double alpha, bool snaps, uint64_t sc0, uint64_t sr0) {
         const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
         if (nc + nr == 0) return;
+        if (nc) ensure_pmark();
         double sn, cs;
         sin_cos(2 * alpha, &sn, &cs);
-        const uint32_t blocks = static_cast<uint32_t>((nc + nr + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, stream, g_cos.as<uint32_t>(), g_rot.as<uint2>(),
-                           ly.cos_begin, nc, ly.rot_begin, nr, cs, sn, rep.as<double>(),
-                           snaps ? snap_c.as<double>() : nullptr, snaps ? snap_r.as<double>() : nullptr, sc0, sr0);
-        HIP_TRY(hipGetLastError());
+        if (nr) {
+            hipLaunchKernelGGL(k_layer_rot, dim3(rot_blocks(ly)), dim3(kBlock), 0, stream, g_rot.as<uint2>(),
+                               ly.rot_begin, nr, cs, sn, rep.as<double>(), snaps ? snap_r.as<double>() : nullptr, sr0,
+                               nc ? pmark.as<uint8_t>() : nullptr, ly.n_before);
+            HIP_TRY(hipGetLastError());
+        }
+        if (nc) {
+            hipLaunchKernelGGL(k_layer_cos, dim3(cos_blocks(ly)), dim3(kBlock), 0, stream, cols.as<uint64_t>(),
+                               colstride(), layer_cols(ly), g_fold.as<int32_t>() + ly.fold_off, ly.n_before, cs,
+                               rep.as<double>(), pmark.as<uint8_t>(), snaps ? snap_c.as<double>() : nullptr,
+                               ly.cos_begin - sc0, g_cofs.as<uint32_t>() + ly.cofs_off, cos_chunks(ly));
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    // After a build: the new layers' fold columns on the device and, per layer and 64-word chunk, the
+    // cos-only rows before the chunk (the snapshot positions of the adjoint).  The recomputed count
+    // of every layer must equal the count its build recorded.
+    void graph_finalize(size_t from) {
+        if (!graph_folds.empty()) {
+            g_fold.release();
+            g_fold.reserve(graph_folds.size() * 4);
+            HIP_TRY(hipMemcpyAsync(g_fold.p, graph_folds.data(), graph_folds.size() * 4, hipMemcpyHostToDevice, stream));
+        }
+        uint64_t total = cofs_total;
+        for (size_t l = from; l < layers.size(); ++l) {
+            layers[l].cofs_off = total;
+            total += cos_chunks(layers[l]);
+        }
+        if (total == cofs_total) return;
+        grow_keep(g_cofs, total * 4);
+        ensure_pmark();
+        for (size_t l = from; l < layers.size(); ++l) {
+            const LayerInfo &ly = layers[l];
+            if (cos_chunks(ly) == 0) continue;
+            const uint64_t nr = ly.rot_end - ly.rot_begin;
+            if (nr) {
+                hipLaunchKernelGGL(k_layer_mark, dim3(rot_blocks(ly)), dim3(kBlock), 0, stream, g_rot.as<uint2>(),
+                                   ly.rot_begin, nr, pmark.as<uint8_t>(), ly.n_before);
+                HIP_TRY(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_layer_cos_count, dim3(cos_blocks(ly)), dim3(kBlock), 0, stream, cols.as<uint64_t>(),
+                               colstride(), layer_cols(ly), g_fold.as<int32_t>() + ly.fold_off, ly.n_before,
+                               pmark.as<uint8_t>(), g_cofs.as<uint32_t>() + ly.cofs_off, cos_chunks(ly));
+            HIP_TRY(hipGetLastError());
+        }
+        std::vector<uint32_t> cnt(total - cofs_total);
+        HIP_TRY(hipMemcpyAsync(cnt.data(), g_cofs.as<uint32_t>() + cofs_total, cnt.size() * 4, hipMemcpyDeviceToHost,
+                               stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (size_t l = from; l < layers.size(); ++l) {
+            const LayerInfo &ly = layers[l];
+            uint64_t acc = 0;
+            for (uint64_t b = 0; b < cos_chunks(ly); ++b) {
+                uint32_t &v = cnt[ly.cofs_off - cofs_total + b];
+                const uint32_t c = v;
+                v = static_cast<uint32_t>(acc);
+                acc += c;
+            }
+            if (acc != ly.cos_end - ly.cos_begin)
+                throw RunError("graph layer " + std::to_string(l) + ": " + std::to_string(acc) +
+                               " recomputed cos-only rows, the build recorded " +
+                               std::to_string(ly.cos_end - ly.cos_begin));
+        }
+        HIP_TRY(hipMemcpyAsync(g_cofs.as<uint32_t>() + cofs_total, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice,
+                               stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        cofs_total = total;
     }
     double replay_alpha(ReplayKind kind, const double *params, const LayerInfo &ly) const {
         return (kind == kState ? -1.0 : 1.0) * params[ly.param_index] * ly.gen_coeff;
@@ -2704,11 +2959,9 @@ struct mpa_engine {
             HIP_TRY(hipGetLastError());
         }
         const size_t nl = layers.size();
-        std::vector<uint64_t> offs(nl + 1, 0);
-        for (size_t l = 0; l < nl; ++l) {
-            const uint64_t items = (layers[l].cos_end - layers[l].cos_begin) + (layers[l].rot_end - layers[l].rot_begin);
-            offs[l + 1] = offs[l] + (items + kBlock - 1) / kBlock;
-        }
+        std::vector<uint64_t> offs(nl + 1, 0);  // per layer: its rotation blocks' partials, then its cosine blocks'
+        for (size_t l = 0; l < nl; ++l) offs[l + 1] = offs[l] + rot_blocks(layers[l]) + cos_blocks(layers[l]);
+        ensure_pmark();
         gpartial.reserve(std::max<uint64_t>(offs[nl], 1) * 8);
         goffs.reserve((nl + 1) * 8);
         gdl.reserve(std::max<size_t>(nl, 1) * 8);
@@ -2726,11 +2979,23 @@ struct mpa_engine {
                 const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
                 double sn, cs;
                 sin_cos(2 * alpha, &sn, &cs);
-                hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(offs[l + 1] - offs[l])), dim3(kBlock), 0,
-                                   stream, g_cos.as<uint32_t>(), g_rot.as<uint2>(), ly.cos_begin, nc, ly.rot_begin, nr,
-                                   cs, sn, snap_c.as<double>(), snap_r.as<double>(), lam.as<double>(),
-                                   gpartial.as<double>(), offs[l], seg_sc0[sg], seg_sr0[sg]);
-                HIP_TRY(hipGetLastError());
+                if (nr) {
+                    hipLaunchKernelGGL(k_layer_rot_adj, dim3(rot_blocks(ly)), dim3(kBlock), 0, stream,
+                                       g_rot.as<uint2>(), ly.rot_begin, nr, cs, sn, snap_r.as<double>(), seg_sr0[sg],
+                                       lam.as<double>(), gpartial.as<double>() + offs[l],
+                                       nc ? pmark.as<uint8_t>() : nullptr, ly.n_before);
+                    HIP_TRY(hipGetLastError());
+                }
+                if (nc) {
+                    hipLaunchKernelGGL(k_layer_cos_adj, dim3(cos_blocks(ly)), dim3(kBlock), 0, stream,
+                                       cols.as<uint64_t>(), colstride(), layer_cols(ly),
+                                       g_fold.as<int32_t>() + ly.fold_off, ly.n_before, cs, sn, pmark.as<uint8_t>(),
+                                       snap_c.as<double>(),
+                                       ly.cos_begin - seg_sc0[sg], g_cofs.as<uint32_t>() + ly.cofs_off,
+                                       cos_chunks(ly), lam.as<double>(),
+                                       gpartial.as<double>() + offs[l] + rot_blocks(ly));
+                    HIP_TRY(hipGetLastError());
+                }
             }
         }
         std::vector<double> dl(nl, 0.0);
@@ -4126,10 +4391,17 @@ int mpa_clone(mpa_engine *e, mpa_engine **out) {
         o->graph_nparams = e->graph_nparams;
         o->gcos_total = e->gcos_total;
         o->grot_total = e->grot_total;
+        o->cofs_total = e->cofs_total;
+        o->graph_folds = e->graph_folds;
         o->base_n = e->base_n;
-        if (e->gcos_total) {
-            o->g_cos.reserve(e->gcos_total * 4);
-            HIP_TRY(hipMemcpyAsync(o->g_cos.p, e->g_cos.p, e->gcos_total * 4, hipMemcpyDeviceToDevice, o->stream));
+        if (!e->graph_folds.empty()) {
+            o->g_fold.reserve(e->graph_folds.size() * 4);
+            HIP_TRY(hipMemcpyAsync(o->g_fold.p, e->graph_folds.data(), e->graph_folds.size() * 4, hipMemcpyHostToDevice,
+                                   o->stream));
+        }
+        if (e->cofs_total) {
+            o->g_cofs.reserve(e->cofs_total * 4);
+            HIP_TRY(hipMemcpyAsync(o->g_cofs.p, e->g_cofs.p, e->cofs_total * 4, hipMemcpyDeviceToDevice, o->stream));
         }
         if (e->grot_total) {
             o->g_rot.reserve(e->grot_total * 8);
@@ -4166,11 +4438,11 @@ int mpa_memory_breakdown(mpa_engine *e, int64_t *out, int32_t n) {
         f[6] = static_cast<int64_t>(e->cols.bytes);                       // inverted_index_bytes (bit columns)
         f[7] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6];            // total_bytes
         f[8] = static_cast<int64_t>((e->cap - std::min(e->cap, e->N)) * e->nw * 8);  // d_terms_slack_bytes
-        f[9] = static_cast<int64_t>(e->g_cos.bytes + e->g_rot.bytes + e->base.bytes);  // graph bytes
+        f[9] = static_cast<int64_t>(e->g_rot.bytes + e->g_fold.bytes + e->g_cofs.bytes + e->base.bytes);  // graph bytes
         size_t work = 0;  // propagate / evaluation scratch the engine keeps between calls
         for (const DevArray *a : {&e->f_elead, &e->t_val, &e->t_item, &e->t_cnt, &e->t_bsum, &e->t_gsum, &e->desc_m,
                                   &e->AL, &e->lead, &e->foll, &e->blk_a, &e->blk_b, &e->rec, &e->lv, &e->match,
-                                  &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam,
+                                  &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam, &e->pmark,
                                   &e->gpartial, &e->sum_blk, &e->sum_v, &e->sum_m, &e->sum_out})
             work += a->bytes;
         f[10] = static_cast<int64_t>(work);
@@ -4249,6 +4521,7 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
             trl.check_lower = 0;
             trl.check_upper = 0;
         }
+        const size_t first_layer = e->layers.size();
         e->building = true;
         try {
             RunPropagate run{e, gds, folds, true, trl, false};
@@ -4259,8 +4532,8 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
         }
         e->building = false;
         for (int64_t i = 0; i < n_layers; ++i) e->graph_nparams = std::max(e->graph_nparams, param_map[i] + 1);
-        e->shrink_keep(e->g_cos, e->gcos_total * 4);  // the records' arrays at their exact sizes
-        e->shrink_keep(e->g_rot, e->grot_total * 8);
+        e->shrink_keep(e->g_rot, e->grot_total * 8);  // the records' array at its exact size
+        e->graph_finalize(first_layer);
     });
 }
 
@@ -4354,8 +4627,9 @@ int mpa_contract(mpa_engine *e, const double *params, int64_t n_params, int inpl
         if (inplace && !e->layers.empty()) {
             if (e->N) HIP_TRY(hipMemcpyAsync(e->coef.p, c, e->N * 8, hipMemcpyDeviceToDevice, e->stream));
             e->layers.clear();
+            e->graph_folds.clear();
             e->graph_nparams = 0;
-            e->gcos_total = e->grot_total = 0;
+            e->gcos_total = e->grot_total = e->cofs_total = 0;
             e->base_n = 0;
         }
         HIP_TRY(hipStreamSynchronize(e->stream));
