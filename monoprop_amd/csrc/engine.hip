@@ -1679,12 +1679,13 @@ __global__ __launch_bounds__(kBlock) void k_seq_sum(const double *__restrict__ v
 // records never share a row) and the cosine kernel, launched after it, skips the marked rows and
 // clears their marks, leaving `pmark` zero.
 //
-// The cosine kernels run one wavefront per 64 words (4096 rows): lane l folds G's columns for word l,
-// the set bits are listed in row order in LDS (12-bit offsets in the chunk), and the list is walked
-// densely, 64 rows per round and two rounds in flight, each lane loading a row's mark and coefficient
-// together.  A wave's cos-only rows are numbered in row order from its chunk offset `cofs[chunk]`
-// (computed when the graph was built), which places the pre-layer values of the adjoint's snapshots.
-constexpr int kCosChunkWords = 64;
+// The cosine kernels run one wavefront per 16 words (1024 rows): lanes (column slot, word) load G's
+// columns for the 16 words at once and XOR them across lanes, the set bits are listed in row order in
+// LDS (10-bit offsets in the chunk), and the list is walked densely, 64 rows per round and two rounds in
+// flight, each lane loading a row's mark and coefficient together.  A wave's cos-only rows are numbered
+// in row order from its chunk offset `cofs[chunk]` (computed when the graph was built), which places
+// the pre-layer values of the adjoint's snapshots.
+constexpr int kCosChunkWords = 16;
 constexpr int kCosChunkRows = kCosChunkWords * 64;
 
 __device__ __forceinline__ void pmark_pair(uint8_t *pmark, uint32_t S, uint32_t T, uint64_t n_before) {
@@ -1700,19 +1701,31 @@ struct LayerCols {
     int32_t n;
 };
 
-// G's anticommuting bits of word w among rows [0, n_before) (0 past them).
+// G's anticommuting bits among rows [0, n_before) of word w0 + (lane % 16) of the chunk starting at
+// word w0 (0 past them): lane = column slot * 16 + word, four slots per pass, XOR across the slots.
 __device__ __forceinline__ uint64_t layer_fold_bits(const uint64_t *__restrict__ cols, uint64_t colstride,
                                                     const LayerCols &lc, const int32_t *__restrict__ fold,
-                                                    uint64_t n_before, uint64_t w) {
+                                                    uint64_t n_before, uint64_t w0) {
+    const uint32_t lane = lane_id();
     const uint64_t nwords = (n_before + 63) / 64;
-    if (w >= nwords) return 0;
-    uint64_t y[kLayerCols];
-#pragma unroll
-    for (int f = 0; f < kLayerCols; ++f) y[f] = f < lc.n ? cols[static_cast<uint64_t>(lc.col[f]) * colstride + w] : 0;
+    const uint64_t w = w0 + (lane & (kCosChunkWords - 1));
+    const int sl = static_cast<int>(lane / kCosChunkWords);
+    constexpr int kSlots = 64 / kCosChunkWords;
     uint64_t bits = 0;
+    if (w < nwords) {
+        uint64_t y[kLayerCols / kSlots];
 #pragma unroll
-    for (int f = 0; f < kLayerCols; ++f) bits ^= y[f];
-    for (int f = kLayerCols; f < lc.n; ++f) bits ^= cols[static_cast<uint64_t>(fold[f]) * colstride + w];
+        for (int pass = 0; pass < kLayerCols / kSlots; ++pass) {
+            const int f = pass * kSlots + sl;
+            y[pass] = f < lc.n ? cols[static_cast<uint64_t>(lc.col[f]) * colstride + w] : 0;
+        }
+#pragma unroll
+        for (int pass = 0; pass < kLayerCols / kSlots; ++pass) bits ^= y[pass];
+        for (int f = kLayerCols + sl; f < lc.n; f += kSlots) bits ^= cols[static_cast<uint64_t>(fold[f]) * colstride + w];
+    }
+#pragma unroll
+    for (int d = kCosChunkWords; d < 64; d <<= 1) bits ^= __shfl_xor(bits, d, 64);
+    if (w >= nwords) return 0;
     if (w == nwords - 1 && (n_before & 63)) bits &= (uint64_t{1} << (n_before & 63)) - 1;
     return bits;
 }
@@ -1731,14 +1744,14 @@ __device__ __forceinline__ uint64_t layer_cos_chunk(const uint64_t *__restrict__
                                                     uint16_t *list) {
     const uint32_t lane = lane_id();
     const uint64_t row0 = chunk * kCosChunkRows;
-    const uint64_t mine = layer_fold_bits(cols, colstride, lc, fold, n_before, chunk * kCosChunkWords + lane);
-    const uint32_t cnt = static_cast<uint32_t>(__popcll(mine));
+    const uint64_t mine = layer_fold_bits(cols, colstride, lc, fold, n_before, chunk * kCosChunkWords);
+    const uint32_t cnt = lane < kCosChunkWords ? static_cast<uint32_t>(__popcll(mine)) : 0u;  // lanes >= 16 repeat
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = __shfl(incl, 63, 64);
     if (total == 0) return 0;
     {
         uint32_t pos = incl - cnt;
-        for (uint64_t b = mine; b; b &= b - 1)
+        for (uint64_t b = cnt ? mine : 0; b; b &= b - 1)
             list[pos++] = static_cast<uint16_t>(lane * 64 + static_cast<uint32_t>(__builtin_ctzll(b)));
     }
     wave_lds_sync();
@@ -1786,97 +1799,97 @@ __device__ __forceinline__ uint64_t layer_cos_chunk(const uint64_t *__restrict__
     return rank - rank0;
 }
 
-// Rotations of one layer at angle alpha: c_S = c_S k + (s (-ph)) c_T, c_T = c_T k + (s ph) c_S from the
-// pre-layer values (kept in snap_r, indexed from the segment's first rotation sr0, when set); with
-// `pmark` set, the pair's rows below n_before are marked for the cosine kernel.
-__global__ __launch_bounds__(kBlock) void k_layer_rot(const uint2 *__restrict__ grot, uint64_t rb, uint64_t nrot,
-                                                      double k, double s, double *__restrict__ c,
-                                                      double *__restrict__ snap_r, uint64_t sr0,
-                                                      uint8_t *__restrict__ pmark, uint64_t n_before) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= nrot) return;
-    const uint64_t j = rb + i;
-    const uint2 e = grot[j];
-    const uint32_t S = e.x, T = e.y & ~kPhaseBit;
-    const int ph = (e.y & kPhaseBit) ? -1 : 1;
-    const double vS = c[S], vT = c[T];
-    if (snap_r) {
-        snap_r[2 * (j - sr0)] = vS;
-        snap_r[2 * (j - sr0) + 1] = vT;
-    }
-    c[S] = vS * k + s * static_cast<double>(-ph) * vT;
-    c[T] = vT * k + s * static_cast<double>(ph) * vS;
-    if (pmark) pmark_pair(pmark, S, T, n_before);
-}
-
-// Cosine rows of one layer: c *= k (pre-layer values to snap_c from snap_base when set).
-__global__ __launch_bounds__(kBlock) void k_layer_cos(const uint64_t *__restrict__ cols, uint64_t colstride,
-                                                      LayerCols lc, const int32_t *__restrict__ fold, uint64_t n_before,
-                                                      double k, double *__restrict__ c, uint8_t *__restrict__ pmark,
-                                                      double *__restrict__ snap_c, uint64_t snap_base,
-                                                      const uint32_t *__restrict__ cofs, uint64_t nchunks) {
-    __shared__ uint16_t list[kWaves][kCosChunkRows];
-    const uint64_t chunk = static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
-    if (chunk >= nchunks) return;
-    const uint64_t rank0 = snap_c ? snap_base + cofs[chunk] : 0;
-    (void)layer_cos_chunk<0>(cols, colstride, lc, fold, n_before, pmark, chunk, k, 0.0, c, snap_c, rank0, nullptr,
-                             nullptr, list[threadIdx.x >> 6]);
-}
-
-// Sum of one block's derivative terms into partial[blockIdx.x] (fixed tree order).
+// Sum of one block's derivative terms into partial[blockIdx.x] (a fixed butterfly per wave, then the
+// wave sums in wave order).
 __device__ __forceinline__ void block_partial(double d, double *red, double *__restrict__ partial) {
-    red[threadIdx.x] = d;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = d;
     __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < static_cast<unsigned>(st)) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < kWaves; ++w) t += red[w];
+        partial[blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// Adjoint of one layer's rotations: d<O>/d alpha from the pre-layer snapshots, then lambda <- R^T lambda;
-// pair (S,T,ph): -2s c_S - 2k ph c_T and -2s c_T + 2k ph c_S.  Marks the pair's rows like k_layer_rot.
-__global__ __launch_bounds__(kBlock) void k_layer_rot_adj(const uint2 *__restrict__ grot, uint64_t rb, uint64_t nrot,
-                                                          double k, double s, const double *__restrict__ snap_r,
-                                                          uint64_t sr0, double *__restrict__ lam,
-                                                          double *__restrict__ partial,
-                                                          uint8_t *__restrict__ pmark, uint64_t n_before) {
-    __shared__ double red[kBlock];
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    double d = 0.0;
-    if (i < nrot) {
-        const uint64_t j = rb + i;
-        const uint2 e = grot[j];
-        const uint32_t S = e.x, T = e.y & ~kPhaseBit;
-        const double ph = (e.y & kPhaseBit) ? -1.0 : 1.0;
-        const double cs = snap_r[2 * (j - sr0)], ct = snap_r[2 * (j - sr0) + 1];
-        const double ls = lam[S], lt = lam[T];
-        d = ls * (-2.0 * s * cs - 2.0 * k * ph * ct);
-        d += lt * (-2.0 * s * ct + 2.0 * k * ph * cs);
-        lam[S] = k * ls + s * ph * lt;
-        lam[T] = k * lt - s * ph * ls;
-        if (pmark) pmark_pair(pmark, S, T, n_before);
-    }
-    block_partial(d, red, partial);
-}
+// One graph layer in one launch, by block range:
+//   [0, rot_blocks)                rotations: c_S = c_S k + (s (-ph)) c_T, c_T = c_T k + (s ph) c_S from the
+//                                  pre-layer values (kept in snap_r from the segment's first rotation sr0
+//                                  when set); ADJ: d<O>/d alpha from snap_r, then lambda <- R^T lambda
+//                                  (pair: -2s c_S - 2k ph c_T and -2s c_T + 2k ph c_S)
+//   [rot_blocks, + cos_blocks)     cos-only rows (marks of this layer in pm_cur, cleared): c *= k, pre-layer
+//                                  values to snap_c when set; ADJ: -2 s c_i lambda_i, lambda_i *= k
+//   [.., + mark_blocks)            marks of the next layer of the pass into pm_next (the other buffer)
+// ADJ writes one derivative partial per rotation and cosine block (partial[blockIdx.x]).
+struct LayerArgs {
+    uint64_t rb, nrot, n_before, nchunks, snap_base, sr0;
+    double k, s;
+    LayerCols lc;
+    const int32_t *fold;
+    const uint32_t *cofs;
+    double *snap_c, *snap_r;
+    uint32_t rot_blocks, cos_blocks;
+    uint64_t next_rb, next_nrot, next_n_before;  // the next layer's pair rows to mark (next_nrot 0: none)
+};
 
-// Adjoint of one layer's cosine rows: -2 s c_i lambda_i, then lambda_i *= k.
-__global__ __launch_bounds__(kBlock) void k_layer_cos_adj(const uint64_t *__restrict__ cols, uint64_t colstride,
-                                                          LayerCols lc, const int32_t *__restrict__ fold,
-                                                          uint64_t n_before, double k, double s,
-                                                          uint8_t *__restrict__ pmark,
-                                                          const double *__restrict__ snap_c, uint64_t snap_base,
-                                                          const uint32_t *__restrict__ cofs, uint64_t nchunks,
-                                                          double *__restrict__ lam, double *__restrict__ partial) {
-    __shared__ double red[kBlock];
+template <int ADJ>
+__global__ __launch_bounds__(kBlock) void k_layer(const uint2 *__restrict__ grot, const uint64_t *__restrict__ cols,
+                                                  uint64_t colstride, double *__restrict__ v,
+                                                  uint8_t *__restrict__ pm_cur, uint8_t *__restrict__ pm_next,
+                                                  LayerArgs a, double *__restrict__ partial) {
     __shared__ uint16_t list[kWaves][kCosChunkRows];
-    const uint64_t chunk = static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
-    double d = 0.0;
-    if (chunk < nchunks)
-        (void)layer_cos_chunk<1>(cols, colstride, lc, fold, n_before, pmark, chunk, k, s, nullptr,
-                                 const_cast<double *>(snap_c), snap_base + cofs[chunk], lam, &d,
-                                 list[threadIdx.x >> 6]);
-    block_partial(d, red, partial);
+    __shared__ double red[kWaves];
+    const uint32_t b = blockIdx.x;
+    if (b < a.rot_blocks) {
+        const uint64_t i = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
+        double d = 0.0;
+        if (i < a.nrot) {
+            const uint64_t j = a.rb + i;
+            const uint2 e = grot[j];
+            const uint32_t S = e.x, T = e.y & ~kPhaseBit;
+            if (!ADJ) {
+                const int ph = (e.y & kPhaseBit) ? -1 : 1;
+                const double vS = v[S], vT = v[T];
+                if (a.snap_r) {
+                    a.snap_r[2 * (j - a.sr0)] = vS;
+                    a.snap_r[2 * (j - a.sr0) + 1] = vT;
+                }
+                v[S] = vS * a.k + a.s * static_cast<double>(-ph) * vT;
+                v[T] = vT * a.k + a.s * static_cast<double>(ph) * vS;
+            } else {
+                const double ph = (e.y & kPhaseBit) ? -1.0 : 1.0;
+                const double cs = a.snap_r[2 * (j - a.sr0)], ct = a.snap_r[2 * (j - a.sr0) + 1];
+                const double ls = v[S], lt = v[T];
+                d = ls * (-2.0 * a.s * cs - 2.0 * a.k * ph * ct);
+                d += lt * (-2.0 * a.s * ct + 2.0 * a.k * ph * cs);
+                v[S] = a.k * ls + a.s * ph * lt;
+                v[T] = a.k * lt - a.s * ph * ls;
+            }
+        }
+        if (ADJ) block_partial(d, red, partial);
+        return;
+    }
+    if (b < a.rot_blocks + a.cos_blocks) {
+        const uint64_t chunk = static_cast<uint64_t>(b - a.rot_blocks) * kWaves + (threadIdx.x >> 6);
+        double d = 0.0;
+        if (chunk < a.nchunks) {
+            const uint64_t rank0 = (ADJ || a.snap_c) ? a.snap_base + a.cofs[chunk] : 0;
+            if (!ADJ)
+                (void)layer_cos_chunk<0>(cols, colstride, a.lc, a.fold, a.n_before, pm_cur, chunk, a.k, 0.0, v,
+                                         a.snap_c, rank0, nullptr, nullptr, list[threadIdx.x >> 6]);
+            else
+                (void)layer_cos_chunk<1>(cols, colstride, a.lc, a.fold, a.n_before, pm_cur, chunk, a.k, a.s, nullptr,
+                                         a.snap_c, rank0, v, &d, list[threadIdx.x >> 6]);
+        }
+        if (ADJ) block_partial(d, red, partial);  // the cosine blocks' partials follow the rotation blocks'
+        return;
+    }
+    const uint64_t i = static_cast<uint64_t>(b - a.rot_blocks - a.cos_blocks) * kBlock + threadIdx.x;
+    if (i < a.next_nrot) {
+        const uint2 e = grot[a.next_rb + i];
+        pmark_pair(pm_next, e.x, e.y & ~kPhaseBit, a.next_n_before);
+    }
 }
 
 // Graph set-up: marks of one layer's rotation rows (below n_before).
@@ -2217,7 +2230,7 @@ struct mpa_engine {
     std::vector<LayerInfo> layers;
     std::vector<int32_t> graph_folds;
     int64_t graph_nparams = 0;
-    DevArray g_rot{&stream}, g_fold{&stream}, g_cofs{&stream}, pmark{&stream}, seedc{&stream}, base{&stream},
+    DevArray g_rot{&stream}, g_fold{&stream}, g_cofs{&stream}, pmark{&stream}, pmark1{&stream}, seedc{&stream}, base{&stream},
         rep{&stream}, snap_c{&stream}, snap_r{&stream}, lam{&stream}, gpartial{&stream}, goffs{&stream}, gdl{&stream};
     uint64_t base_n = 0;
     uint64_t gcos_total = 0, grot_total = 0;  // cos-only rows counted, rotation records stored
@@ -2727,14 +2740,17 @@ struct mpa_engine {
         return kind == kObservable ? layers.size() - 1 - pos : pos;
     }
     // The pair-row marks of the cosine recompute: one byte per row, all zero between layers.
+    // (two buffers: a layer's launch reads and clears one while it marks the next layer's rows in the other)
     void ensure_pmark() {
         const uint64_t need = std::max<uint64_t>(N, 1);
-        if (pmark.bytes >= need) return;
-        pmark.release();
-        pmark.reserve(need);
-        HIP_TRY(hipMemsetAsync(pmark.p, 0, pmark.bytes, stream));  // all of it: later sizes reuse the block
+        for (DevArray *m : {&pmark, &pmark1}) {
+            if (m->bytes >= need) continue;
+            m->release();
+            m->reserve(need);
+            HIP_TRY(hipMemsetAsync(m->p, 0, m->bytes, stream));  // all of it: later sizes reuse the block
+        }
     }
-    // 64-word chunks (one wavefront each) of a layer's cosine kernels, their blocks, its rotation blocks.
+    // 16-word chunks (one wavefront each) of a layer's cosine rows, their blocks, its rotation blocks.
     static uint64_t cos_chunks(const LayerInfo &ly) {
         return ly.cos_end > ly.cos_begin ? ((ly.n_before + 63) / 64 + kCosChunkWords - 1) / kCosChunkWords : 0;
     }
@@ -2750,28 +2766,70 @@ struct mpa_engine {
     static uint32_t rot_blocks(const LayerInfo &ly) {
         return static_cast<uint32_t>((ly.rot_end - ly.rot_begin + kBlock - 1) / kBlock);
     }
-    // One layer of the replay on `rep`: its rotations, then its recomputed cos-only rows; with `snaps`
-    // the pre-layer values go to the snapshot buffers (indexed from the segment's first cos-only row
-    // sc0 and first rotation sr0).
-    void replay_layer(const LayerInfo &ly, double alpha, bool snaps, uint64_t sc0, uint64_t sr0) {
-        const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
-        if (nc + nr == 0) return;
-        if (nc) ensure_pmark();
-        double sn, cs;
-        sin_cos(2 * alpha, &sn, &cs);
-        if (nr) {
-            hipLaunchKernelGGL(k_layer_rot, dim3(rot_blocks(ly)), dim3(kBlock), 0, stream, g_rot.as<uint2>(),
-                               ly.rot_begin, nr, cs, sn, rep.as<double>(), snaps ? snap_r.as<double>() : nullptr, sr0,
-                               nc ? pmark.as<uint8_t>() : nullptr, ly.n_before);
+    uint8_t *pm_buf(size_t i) { return (i & 1) ? pmark1.as<uint8_t>() : pmark.as<uint8_t>(); }
+    static bool layer_live(const LayerInfo &ly) { return ly.cos_end > ly.cos_begin || ly.rot_end > ly.rot_begin; }
+    // One pass over graph layers (`order`: layer indices in processing order), one k_layer launch per
+    // layer with records: the replay on `rep` (snapshots of the pre-layer values when `snaps`, indexed
+    // from the segment's first cos-only row sc0 and first rotation sr0), or with ADJ the adjoint on
+    // `lam` (its derivative partials from gpartial + offs[layer]).  Each launch marks the pair rows of
+    // the pass's next layer in the other mark buffer; the first layer's marks go first, on their own.
+    template <int ADJ>
+    void run_pass(ReplayKind kind, const double *params, const std::vector<size_t> &order, bool snaps, uint64_t sc0,
+                  uint64_t sr0, const uint64_t *offs) {
+        std::vector<size_t> live;
+        for (size_t l : order)
+            if (layer_live(layers[l])) live.push_back(l);
+        if (live.empty()) return;
+        ensure_pmark();
+        {
+            const LayerInfo &f = layers[live[0]];
+            const uint64_t nr = f.rot_end - f.rot_begin;
+            if (f.cos_end > f.cos_begin && nr) {
+                hipLaunchKernelGGL(k_layer_mark, dim3(rot_blocks(f)), dim3(kBlock), 0, stream, g_rot.as<uint2>(),
+                                   f.rot_begin, nr, pm_buf(0), f.n_before);
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        for (size_t i = 0; i < live.size(); ++i) {
+            const LayerInfo &ly = layers[live[i]];
+            LayerArgs a{};
+            a.rb = ly.rot_begin;
+            a.nrot = ly.rot_end - ly.rot_begin;
+            a.n_before = ly.n_before;
+            a.nchunks = cos_chunks(ly);
+            a.snap_base = ly.cos_begin - sc0;
+            a.sr0 = sr0;
+            sin_cos(2 * replay_alpha(kind, params, ly), &a.s, &a.k);
+            a.lc = layer_cols(ly);
+            a.fold = g_fold.as<int32_t>() + ly.fold_off;
+            a.cofs = g_cofs.as<uint32_t>() + ly.cofs_off;
+            a.snap_c = (ADJ || snaps) ? snap_c.as<double>() : nullptr;
+            a.snap_r = (ADJ || snaps) ? snap_r.as<double>() : nullptr;
+            a.rot_blocks = rot_blocks(ly);
+            a.cos_blocks = cos_blocks(ly);
+            uint32_t mark_blocks = 0;
+            if (i + 1 < live.size()) {
+                const LayerInfo &nx = layers[live[i + 1]];
+                if (nx.cos_end > nx.cos_begin) {
+                    a.next_rb = nx.rot_begin;
+                    a.next_nrot = nx.rot_end - nx.rot_begin;
+                    a.next_n_before = nx.n_before;
+                    mark_blocks = rot_blocks(nx);
+                }
+            }
+            const uint32_t blocks = a.rot_blocks + a.cos_blocks + mark_blocks;
+            if (blocks == 0) continue;
+            hipLaunchKernelGGL(k_layer<ADJ>, dim3(blocks), dim3(kBlock), 0, stream, g_rot.as<uint2>(),
+                               cols.as<uint64_t>(), colstride(), ADJ ? lam.as<double>() : rep.as<double>(), pm_buf(i),
+                               pm_buf(i + 1), a, ADJ ? gpartial.as<double>() + offs[live[i]] : nullptr);
             HIP_TRY(hipGetLastError());
         }
-        if (nc) {
-            hipLaunchKernelGGL(k_layer_cos, dim3(cos_blocks(ly)), dim3(kBlock), 0, stream, cols.as<uint64_t>(),
-                               colstride(), layer_cols(ly), g_fold.as<int32_t>() + ly.fold_off, ly.n_before, cs,
-                               rep.as<double>(), pmark.as<uint8_t>(), snaps ? snap_c.as<double>() : nullptr,
-                               ly.cos_begin - sc0, g_cofs.as<uint32_t>() + ly.cofs_off, cos_chunks(ly));
-            HIP_TRY(hipGetLastError());
-        }
+    }
+    std::vector<size_t> pass_order(ReplayKind kind, size_t from, size_t to, bool reverse) const {
+        std::vector<size_t> o;
+        for (size_t i = from; i < to; ++i) o.push_back(replay_layer_at(kind, i));
+        if (reverse) std::reverse(o.begin(), o.end());
+        return o;
     }
     // After a build: the new layers' fold columns on the device and, per layer and 64-word chunk, the
     // cos-only rows before the chunk (the snapshot positions of the adjoint).  The recomputed count
@@ -2833,10 +2891,7 @@ struct mpa_engine {
     void replay(const double *params, ReplayKind kind = kOperator) {
         ckpt_valid = false;  // rep no longer holds a checkpointed forward
         replay_init(kind);
-        for (size_t i = 0; i < layers.size(); ++i) {
-            const LayerInfo &ly = layers[replay_layer_at(kind, i)];
-            replay_layer(ly, replay_alpha(kind, params, ly), false, 0, 0);
-        }
+        run_pass<0>(kind, params, pass_order(kind, 0, layers.size(), false), false, 0, 0, nullptr);
     }
 
     // Checkpointed adjoint.  The replay order is cut into segments of about equal snapshot bytes
@@ -2918,10 +2973,8 @@ struct mpa_engine {
         }
     }
     void replay_segment(ReplayKind kind, const double *params, size_t sg, bool snaps) {
-        for (size_t i = seg_start[sg]; i < seg_start[sg + 1]; ++i) {
-            const LayerInfo &ly = layers[replay_layer_at(kind, i)];
-            replay_layer(ly, replay_alpha(kind, params, ly), snaps, seg_sc0[sg], seg_sr0[sg]);
-        }
+        run_pass<0>(kind, params, pass_order(kind, seg_start[sg], seg_start[sg + 1], false), snaps, seg_sc0[sg],
+                    seg_sr0[sg], nullptr);
     }
     // Forward replay with checkpoints (rep must hold the starting vector); afterwards rep holds the
     // evolved vector and the last segment's snapshots are kept.
@@ -2971,32 +3024,8 @@ struct mpa_engine {
                 if (N) HIP_TRY(hipMemcpyAsync(rep.p, ckpt.as<double>() + sg * N, N * 8, hipMemcpyDeviceToDevice, stream));
                 replay_segment(kind, params, sg, true);
             }
-            for (size_t i = seg_start[sg + 1]; i-- > seg_start[sg];) {
-                const size_t l = replay_layer_at(kind, i);
-                const LayerInfo &ly = layers[l];
-                const uint64_t nc = ly.cos_end - ly.cos_begin, nr = ly.rot_end - ly.rot_begin;
-                if (nc + nr == 0) continue;
-                const double alpha = 1.0 * params[ly.param_index] * ly.gen_coeff;
-                double sn, cs;
-                sin_cos(2 * alpha, &sn, &cs);
-                if (nr) {
-                    hipLaunchKernelGGL(k_layer_rot_adj, dim3(rot_blocks(ly)), dim3(kBlock), 0, stream,
-                                       g_rot.as<uint2>(), ly.rot_begin, nr, cs, sn, snap_r.as<double>(), seg_sr0[sg],
-                                       lam.as<double>(), gpartial.as<double>() + offs[l],
-                                       nc ? pmark.as<uint8_t>() : nullptr, ly.n_before);
-                    HIP_TRY(hipGetLastError());
-                }
-                if (nc) {
-                    hipLaunchKernelGGL(k_layer_cos_adj, dim3(cos_blocks(ly)), dim3(kBlock), 0, stream,
-                                       cols.as<uint64_t>(), colstride(), layer_cols(ly),
-                                       g_fold.as<int32_t>() + ly.fold_off, ly.n_before, cs, sn, pmark.as<uint8_t>(),
-                                       snap_c.as<double>(),
-                                       ly.cos_begin - seg_sc0[sg], g_cofs.as<uint32_t>() + ly.cofs_off,
-                                       cos_chunks(ly), lam.as<double>(),
-                                       gpartial.as<double>() + offs[l] + rot_blocks(ly));
-                    HIP_TRY(hipGetLastError());
-                }
-            }
+            run_pass<1>(kind, params, pass_order(kind, seg_start[sg], seg_start[sg + 1], true), true, seg_sc0[sg],
+                        seg_sr0[sg], offs.data());
         }
         std::vector<double> dl(nl, 0.0);
         if (nl) {
@@ -4442,7 +4471,7 @@ int mpa_memory_breakdown(mpa_engine *e, int64_t *out, int32_t n) {
         size_t work = 0;  // propagate / evaluation scratch the engine keeps between calls
         for (const DevArray *a : {&e->f_elead, &e->t_val, &e->t_item, &e->t_cnt, &e->t_bsum, &e->t_gsum, &e->desc_m,
                                   &e->AL, &e->lead, &e->foll, &e->blk_a, &e->blk_b, &e->rec, &e->lv, &e->match,
-                                  &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam, &e->pmark,
+                                  &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam, &e->pmark, &e->pmark1,
                                   &e->gpartial, &e->sum_blk, &e->sum_v, &e->sum_m, &e->sum_out})
             work += a->bytes;
         f[10] = static_cast<int64_t>(work);
