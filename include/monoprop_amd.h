@@ -108,7 +108,7 @@ int mpa_expectation_value(mpa_engine *e, const double *params, int64_t n_params,
 
 /* No counterpart in the reference (it keeps no snapshots: its adjoint un-evolves with 1/cos,
  * Evolution.cpp:355-387).  Memory of the exact adjoint: budget_bytes bounds the coefficient-vector
- * checkpoints plus one segment's pre-layer values (default 16 GiB; the gradient replays each earlier
+ * checkpoints plus one segment's pre-layer values (default 24 GiB; the gradient replays each earlier
  * segment once more, and its result does not depend on the budget; 0 asks for one segment per
  * layer, up to 256).  keep != 0 makes
  * mpa_expectation_value run that checkpointed forward, so that mpa_expectation_value_and_gradient

@@ -381,7 +381,7 @@ class DeviceSimulator(_EngineHandle):
                                                     C.byref(out)))
         return out.value
 
-    def set_gradient_checkpointing(self, keep: bool, budget_bytes: int = 16 << 30) -> None:
+    def set_gradient_checkpointing(self, keep: bool, budget_bytes: int = 24 << 30) -> None:
         """Memory of the exact adjoint (checkpoints + one segment's pre-layer values, at most
         ``budget_bytes`` when possible); with ``keep`` every expectation_value runs the checkpointed
         forward so that a gradient at the same parameters starts from it."""

@@ -2239,7 +2239,7 @@ struct mpa_engine {
     // segment's snapshots; with keep_ckpt, mpa_expectation_value runs the checkpointed forward and a
     // following gradient at the same parameters (ckpt_params, to the bit) starts from it.
     DevArray ckpt{&stream};
-    int64_t grad_budget = int64_t{16} << 30;
+    int64_t grad_budget = int64_t{24} << 30;
     bool keep_ckpt = false, ckpt_valid = false;
     int ckpt_kind = 0;
     std::vector<double> ckpt_params;
