@@ -30,6 +30,7 @@ one propagation at a time, default theta_h 0.8: at 0.1 and 0.4 the 100-qubit sta
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--trotter-steps 20] [--no-cpu-baseline]
        [--no-gradient] [--batch-sets 4] [--batched | --unbatched] [--schrodinger Q] [--theta-h 0.1 0.4 0.8]
+       [--only-gradient] [--only-batch]
 """
 
 from __future__ import annotations
@@ -65,6 +66,7 @@ def parse_args():
     ap.add_argument("--batch-no-solo", action="store_true", help="skip the one-propagation-per-pair comparison")
     ap.add_argument("--batch-workers", type=int, default=4, help="sweep drivers running at once")
     ap.add_argument("--only-batch", action="store_true", help="run and print only the batched sweep")
+    ap.add_argument("--only-gradient", action="store_true", help="run and print only the gradient measurement")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the known-size copy (counter scale)")
     ap.add_argument("--traffic-from", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "FETCH", "WRITE"),
                     help="derive per-launch HBM traffic from rocprofv3 --pmc CSVs and write the summary")
@@ -464,6 +466,9 @@ def main():
     from monoprop_amd import _engine
 
     _engine.load_library()
+    if args.only_gradient:
+        print(json.dumps({"gradient": gradient_measurement()}))
+        return
     if args.only_batch:
         print(json.dumps({"batch": batch_measurement(args.batch_sets, solo=not args.batch_no_solo,
                                                      workers=args.batch_workers)}))

@@ -129,3 +129,22 @@ def test_propagate_after_build_graph_rejected():
     prop.build_graph(c1)
     with pytest.raises(RuntimeError, match="non-empty graph"):
         prop.propagate(c2)
+
+
+@pytest.mark.parametrize("n_params", [1, 3])
+def test_propagate_parameter_length_is_the_engines_check(n_params):
+    """A parameter vector shorter or longer than max(parameter_mapping) + 1 reaches the engine's check
+    and raises the reference's RuntimeError (Validation.cpp:79-91), for propagate as for build_graph;
+    the store is left as it was."""
+    from monoprop_amd._engine import DeviceSimulator
+    from monoprop_amd.circuit import expand_monomials
+
+    print(f"case: propagate with {n_params} parameters for 2", flush=True)
+    c = _two_gates()
+    monos, gcs, pm, _ = expand_monomials(c.gates, c.resolved_mapping, 8)
+    d = DeviceSimulator(_OBS.terms, 16, [], logical_num_modes=8, basis="majorana")
+    with pytest.raises(RuntimeError, match=r"length of parameters \(%d\) must be the same as max\(parameter_mapping\)\+1 \(2\)" % n_params):
+        d.propagate(monos, pm, gcs, [0.5, 0.3, 0.1][:n_params])
+    assert d.size() == len(_OBS.terms)
+    d.propagate(monos, pm, gcs, [0.5, 0.3])
+    assert d.size() >= len(_OBS.terms)
