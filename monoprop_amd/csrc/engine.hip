@@ -952,6 +952,7 @@ constexpr int kResChunk = kBlock;                         // resolve: one emitte
 // resolve blocks: each draws one arrival ticket (one atomic on one address), so a grid much larger than
 // a gate's chunk count costs more than it gives; chunks past the grid go by further tickets
 constexpr uint32_t kMaxGrid = 128;
+constexpr uint32_t kGraphGrid = 512;  // resolve blocks of a graph build (measured: 512 beats 128, 1024, 2048)
 constexpr int kBsumBatch = 16;                            // resolve: sweep-block sums read per round
 // per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
 constexpr uint32_t kCntBits = 16;
@@ -990,6 +991,20 @@ __device__ __forceinline__ int sweep_col(const SweepCols &sc, int pass, int sl) 
     return sl == 0 ? c0 : sl == 1 ? c1 : sl == 2 ? c2 : c3;
 }
 
+// A graph build on the fused kernels (build_graph without an orbital cap): the seed vector is evolved
+// with the unfused arithmetic of GraphSink (Engine.h:69-180): the sweep leaves the anticommuting
+// coefficients unscaled, the resolve rotates every applied pair from the exact pre-gate values, records
+// it in `rot` ({source row, target row | phase bit}, appended at the C_GROT counter; at most rot_cap
+// records) and marks its rows below the live count in `pmark`, and k_build_cos then scales the unmarked
+// anticommuting rows.  New rows get the operator's own starting value in `zero_fill` (0, or the state
+// score in the Schrodinger picture).  rot == nullptr: a plain propagation.
+struct GraphArgs {
+    uint2 *rot;
+    uint8_t *pmark;
+    double *zero_fill;
+    uint64_t rot_cap;
+};
+
 // Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate.  One wavefront
 // (one 64-thread block) per 1024-term tile, no dependence between tiles:
 //   A  G's fold and pivot columns of the tile's 16 words (lane = slot * 16 + word), loaded by the kernel
@@ -1005,7 +1020,7 @@ __device__ __forceinline__ int sweep_col(const SweepCols &sc, int pass, int sl) 
 //      counts, also summed per group of tiles; leader emit masks per word
 //   BATCH  rows belong to batch members (member[row]); each term takes its member's k and |s| from the
 //          block's copy of the gate's per-member constants (s_trig: k, |s| per member, LDS)
-template <int NW, int PAULI, int BATCH>
+template <int NW, int PAULI, int BATCH, int GRAPH>
 __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
                                                uint64_t colstride, const SweepCols &sc, const int32_t *__restrict__ fold,
                                                const uint64_t *xc, const uint64_t *__restrict__ rows,
@@ -1077,7 +1092,7 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
             const uint32_t qm = BATCH ? ((q[BATCH ? j : 0] >> (16 * (lane & 1u))) & 0xffffu) : 0u;  // r & 1 == lane & 1
             const double k = BATCH ? s_trig[2 * qm] : gd.k;
             const double abs_s = BATCH ? s_trig[2 * qm + 1] : gd.abs_s;
-            coef[r] = v[j] * k;
+            if (!GRAPH) coef[r] = v[j] * k;  // a graph build scales after its resolve (k_build_cos)
             if (!(tr.check_lower && abs_s * fabs(v[j]) <= tr.lower)) {
                 cand |= 1u << j;
                 if (tr.check_upper && abs_s * fabs(v[j]) >= tr.upper) resc |= 1u << j;
@@ -1173,7 +1188,7 @@ __device__ __forceinline__ void sweep_load_cols(const uint64_t *__restrict__ col
 // the tile's column words and (BATCH) the per-member constants are all requested before anything
 // waits on any of them.  (A persistent grid of resident blocks walking the tiles, with the next
 // tiles' columns prefetched, was measured slower: more registers, and a barrier per tile set.)
-template <int NW, int PAULI, int BATCH>
+template <int NW, int PAULI, int BATCH, int GRAPH>
 __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr, const uint64_t *__restrict__ cols,
                                                   uint64_t colstride, SweepCols sc, const int32_t *__restrict__ fold,
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
@@ -1182,13 +1197,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
                                                   uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
                                                   int group_log2, uint64_t *__restrict__ counters,
                                                   const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig,
-                                                  uint32_t nbatch, uint64_t cap_rows) {
+                                                  uint32_t nbatch, uint64_t cap_rows, GraphArgs ga) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
     extern __shared__ double s_trig[];  // BATCH: k, |s| per member
     const uint64_t stop = counters[C_STOP];
     const uint64_t N = counters[C_NB + (gi & 1)];
+    const uint64_t nrec = GRAPH ? counters[C_GROT] : 0;
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t tile0 = static_cast<uint64_t>(blockIdx.x) * kWaves;
     const uint64_t tile = tile0 + wave;
@@ -1204,7 +1220,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
     if (stop) return;
     // a gate may double the live count: stop before it when the store could not take that (the host
     // grows the store and resumes here)
-    if (2 * N > cap_rows) {
+    // (a graph build also stops when this gate's records, at most one per live term, might not fit)
+    if (2 * N > cap_rows || (GRAPH && nrec + N > ga.rot_cap)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_STOP] = static_cast<uint64_t>(gi) + 1;
         return;
     }
@@ -1217,8 +1234,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
     if (tile0 * kWTileWords >= nwords) return;  // whole block idle
     uint64_t grp = 0;
     if (tile * kWTileWords < nwords)
-        grp = sweep_tile<NW, PAULI, BATCH>(gdv, tr, cols, colstride, sc, fold, xc, rows, coef, t_val, t_item, elead,
-                                           tcnt, N, tile, sem[wave], cidx[wave], member, s_trig);
+        grp = sweep_tile<NW, PAULI, BATCH, GRAPH>(gdv, tr, cols, colstride, sc, fold, xc, rows, coef, t_val, t_item,
+                                                  elead, tcnt, N, tile, sem[wave], cidx[wave], member, s_trig);
     if (lane_id() == 0) s_grp[wave] = grp;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1263,7 +1280,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                                                     uint32_t max_groups,
                                                     int group_log2, uint16_t *__restrict__ member,
                                                     const Trig *__restrict__ gtrig,
-                                                    const uint64_t *__restrict__ occ_dev) {
+                                                    const uint64_t *__restrict__ occ_dev, GraphArgs ga) {
     constexpr int NC = max_cols<NW, PAULI>();
     constexpr int kColWords = (kResChunk + 63) / 64 + 1;  // words a chunk's consecutive new terms can touch
     __shared__ uint64_t lcol[kColWords * NC];
@@ -1273,6 +1290,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
     __shared__ uint32_t s_ticket, s_base, s_first;
     __shared__ uint32_t st_row[2][kResChunk];  // a chunk's misses: emitter row, new coefficient
     __shared__ double st_ins[2][kResChunk];
+    __shared__ uint8_t st_ph[2][kResChunk];     // graph build: a miss's phase bit
+    __shared__ uint64_t s_hbase, s_rbase;        // graph build: first record of a chunk's hits / misses
     uint32_t *gpl = dyn_lds, *gpf = dyn_lds + max_groups + 1;
     const int ncols = PAULI ? 2 * n_modes : 2 * n_modes + 1;
     // The first chunk goes by a ticket drawn on arrival: a chunk's predecessors then belong to blocks
@@ -1452,7 +1471,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
             }
             // ---- resolution
             double ins = 0.0;
-            uint32_t miss = 0, hit = 0;
+            uint32_t miss = 0, hit = 0, mph = 0;
+            uint2 hrec = make_uint2(0u, 0u);
             if (valid) {
                 const uint32_t q = BATCH ? member[S] : 0u;
                 uint64_t m[NW], t[NW];
@@ -1471,23 +1491,41 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                     const int ph = mpa::emit_phase<NW, PAULI>(m, gd.g, t);
                     const double wS = vS * gk;
                     if (T >= 0) {
+                        // fused: the stored partner value was swept (k v_T), recovered as (k v_T)(1/k);
+                        // graph build: it is the pre-gate value (the gate's 1/k is 1)
                         const double vTu = wT * ginv;
                         coef[S] = wS + gs * static_cast<double>(-ph) * vTu;
-                        coef[T] = wT + gs * static_cast<double>(ph) * vS;
+                        coef[T] = (ga.rot ? wT * gk : wT) + gs * static_cast<double>(ph) * vS;
                         hit = 1;
+                        if (ga.rot) {
+                            ga.pmark[S] = 1;
+                            ga.pmark[T] = 1;
+                            hrec = make_uint2(S, static_cast<uint32_t>(T) | (ph < 0 ? kPhaseBit : 0u));
+                        }
                     } else {  // the new partner starts at vT0: 0, or its state score (SCHR)
                         const double vT0 = SCHR ? fresh_score<NW, PAULI>(t, occ_dev) : 0.0;
                         coef[S] = wS + gs * static_cast<double>(-ph) * vT0;
                         ins = gk * vT0 + gs * static_cast<double>(ph) * vS;
                         miss = 1;
+                        mph = ph < 0 ? 1u : 0u;
+                        if (ga.rot) ga.pmark[S] = 1;
                     }
                 }
             }
             const uint32_t rank = block_excl_scan(miss, lds, &n_miss);
-            const uint32_t n_hit = block_sum(hit, lds);
+            uint32_t n_hit;
+            const uint32_t hrank = block_excl_scan(hit, lds, &n_hit);
             if (miss) {  // staged in the chunk's rank order
                 st_row[buf][rank] = S;
                 st_ins[buf][rank] = ins;
+                st_ph[buf][rank] = static_cast<uint8_t>(mph);
+            }
+            if (ga.rot && n_hit) {  // the chunk's rotations of existing pairs, one counter add per chunk
+                if (threadIdx.x == 0)
+                    s_hbase = atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_GROT),
+                                        static_cast<unsigned long long>(n_hit));
+                __syncthreads();
+                if (hit) ga.rot[s_hbase + hrank] = hrec;
             }
             if (threadIdx.x == 0) {
                 desc_publish(desc_m + chunk, ep, chunk == 0 ? kDescIncl : kDescAgg, n_miss);
@@ -1517,7 +1555,12 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                     excl = look_back(desc_m, pend, ep, counters);
                     if (threadIdx.x == 0) desc_publish(desc_m + pend, ep, kDescIncl, excl + pend_miss);
                 }
-                if (threadIdx.x == 0) s_base = excl;
+                if (threadIdx.x == 0) {
+                    s_base = excl;
+                    if (ga.rot)  // the misses' rotation records
+                        s_rbase = atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_GROT),
+                                            static_cast<unsigned long long>(pend_miss));
+                }
             }
             for (int k = threadIdx.x; k < kColWords * NC; k += kBlock) lcol[k] = 0;
             __syncthreads();
@@ -1531,6 +1574,12 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                 for (int k = 0; k < NW; ++k) pd[k] = t[k];
                 coef[dest] = st_ins[pb][threadIdx.x];
                 if (BATCH) member[dest] = static_cast<uint16_t>(q);
+                if (ga.rot) {
+                    ga.zero_fill[dest] = SCHR ? fresh_score<NW, PAULI>(t, occ_dev) : 0.0;
+                    ga.rot[s_rbase + threadIdx.x] =
+                        make_uint2(st_row[pb][threadIdx.x],
+                                   static_cast<uint32_t>(dest) | (st_ph[pb][threadIdx.x] ? kPhaseBit : 0u));
+                }
                 gather_column_bits<NW, PAULI>(lcol, ncols, n_modes, t, (dest >> 6) - wlo, uint64_t{1} << (dest & 63));
                 if (!gidx_insert<NW>(gidx, gslots, t, static_cast<uint32_t>(dest), q))
                     atomicOr(reinterpret_cast<unsigned long long *>(counters + C_ERR), kErrIndex);
@@ -1890,6 +1939,23 @@ __global__ __launch_bounds__(kBlock) void k_layer(const uint2 *__restrict__ grot
         const uint2 e = grot[a.next_rb + i];
         pmark_pair(pm_next, e.x, e.y & ~kPhaseBit, a.next_n_before);
     }
+}
+
+// Graph build on the fused kernels: after a gate's resolve, its cos-only rows (the anticommuting rows
+// below the gate's live count that the resolve did not mark) take c *= k on the seed vector, and the
+// resolve's marks are cleared.  The live count is read on the device; a stopped device skips the gate.
+__global__ __launch_bounds__(kBlock) void k_build_cos(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                      LayerCols lc, const int32_t *__restrict__ fold, double k,
+                                                      double *__restrict__ c, uint8_t *__restrict__ pmark,
+                                                      const uint64_t *__restrict__ counters, int gi) {
+    __shared__ uint16_t list[kWaves][kCosChunkRows];
+    if (counters[C_STOP]) return;
+    const uint64_t n_before = counters[C_NB + (gi & 1)];
+    const uint64_t nchunks = ((n_before + 63) / 64 + kCosChunkWords - 1) / kCosChunkWords;
+    const uint64_t chunk = static_cast<uint64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    if (chunk >= nchunks) return;
+    (void)layer_cos_chunk<0>(cols, colstride, lc, fold, n_before, pmark, chunk, k, 0.0, c, nullptr, 0, nullptr,
+                             nullptr, list[threadIdx.x >> 6]);
 }
 
 // Graph set-up: marks of one layer's rotation rows (below n_before).
@@ -3280,8 +3346,16 @@ struct mpa_engine {
     // Contract-immediately propagation without host synchronisation between gates: two persistent
     // launches per gate, enqueued in chunks; the device stops before a gate whose worst case could
     // outgrow the store, the host grows it and resumes there.
+    // With `graph` (build_graph, no orbital cap): the seed vector is evolved with the unfused graph
+    // arithmetic, each gate's rotations are recorded and its cos-only rows scaled by k_build_cos, and
+    // the gates become graph layers (GraphArgs).  `trc` is the truncation of this run.
     template <int NW, int PAULI, int BATCH>
     void propagate_fused(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds) {
+        propagate_fused<NW, PAULI, BATCH>(gds, folds, tr, false);
+    }
+    template <int NW, int PAULI, int BATCH>
+    void propagate_fused(const std::vector<GateDesc> &gds, const std::vector<int32_t> &folds, const Trunc &trc,
+                         bool graph) {
         const size_t ng = gds.size();
         gdesc.reserve(std::max<size_t>(ng, 1) * sizeof(GateDesc));
         fold.reserve(std::max<size_t>(folds.size(), 1) * 4);
@@ -3291,6 +3365,12 @@ struct mpa_engine {
             HIP_TRY(hipMemcpyAsync(fold.p, folds.data(), folds.size() * 4, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemsetAsync(stats.p, 0, std::max<size_t>(ng, 1) * kStatW * 8, stream));
         reset_fused_counters(0);
+        if (graph) {  // records continue after the stored ones
+            const uint64_t g0 = grot_total;
+            HIP_TRY(hipMemcpyAsync(counters.as<uint64_t>() + C_GROT, &g0, 8, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            h_counters[C_GROT] = static_cast<int64_t>(g0);
+        }
         hipEvent_t ev[2];
         HIP_TRY(hipEventCreate(&ev[0]));
         HIP_TRY(hipEventCreate(&ev[1]));
@@ -3304,6 +3384,21 @@ struct mpa_engine {
                 ensure_capacity(need);
             }
             ensure_fused_arrays<NW, BATCH>();
+            GraphArgs ga{nullptr, nullptr, nullptr, 0};
+            if (graph) {
+                // room for at least this gate's records (one per live term at most), grown by half
+                const uint64_t nrec = static_cast<uint64_t>(h_counters[C_GROT]);
+                if (g_rot.bytes / 8 < nrec + 2 * N + 1024) grow_keep(g_rot, (nrec + 2 * N + 1024) * 8);
+                const uint64_t need_mark = std::max<uint64_t>(cap, 1);
+                for (DevArray *m : {&pmark}) {
+                    if (m->bytes >= need_mark) continue;
+                    m->release();
+                    m->reserve(need_mark);
+                    HIP_TRY(hipMemsetAsync(m->p, 0, m->bytes, stream));
+                }
+                ga = GraphArgs{g_rot.as<uint2>(), pmark.as<uint8_t>(), coef.as<double>(), g_rot.bytes / 8};
+            }
+            double *cv = graph ? seedc.as<double>() : coef.as<double>();
             // enqueue about as many gates as the store takes at the last chunk's growth rate, so that a
             // device stop leaves few empty launches behind
             size_t span = kChunk;
@@ -3315,8 +3410,10 @@ struct mpa_engine {
             const uint64_t n_start = N;
             const uint64_t tiles_ub = (cap_items + kWTileRows - 1) / kWTileRows;
             const uint32_t sgrid = static_cast<uint32_t>(std::max<uint64_t>((tiles_ub + kWaves - 1) / kWaves, 1));
+            // a graph build resolves every applied pair, typically thousands of chunks per gate: a wider grid
+            const uint64_t rcap = graph ? std::max<uint64_t>(res_grid_cap, kGraphGrid) : res_grid_cap;
             const uint32_t rgrid =
-                static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, res_grid_cap));
+                static_cast<uint32_t>(std::clamp<uint64_t>((cap_items + kResChunk - 1) / kResChunk, 1, rcap));
             const size_t rlds = 2 * (gsum_stride + 1) * sizeof(uint32_t);
             for (size_t g = gi; g < end; ++g) {
                 const GateDesc &gd = gds[g];
@@ -3351,25 +3448,38 @@ struct mpa_engine {
                     sc.ncol = c;
                 }
                 pa = prof_begin();
-                hipLaunchKernelGGL((k_sweep<NW, PAULI, BATCH>), dim3(sgrid), dim3(kBlock),
-                                   BATCH ? static_cast<size_t>(nbatch) * 16 : 0, stream, gd, ig, tr,
+                auto *ksw = (!BATCH && graph) ? k_sweep<NW, PAULI, 0, 1> : k_sweep<NW, PAULI, BATCH, 0>;
+                hipLaunchKernelGGL(ksw, dim3(sgrid), dim3(kBlock),
+                                   BATCH ? static_cast<size_t>(nbatch) * 16 : 0, stream, gd, ig, trc,
                                    cols.as<uint64_t>(), colstride(), sc, fold.as<int32_t>() + gd.fold_off,
-                                   rows.as<uint64_t>(), coef.as<double>(), t_val.as<double>(), t_item.as<uint32_t>(),
+                                   rows.as<uint64_t>(), cv, t_val.as<double>(), t_item.as<uint32_t>(),
                                    f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur,
-                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch, cap);
+                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch, cap, ga);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
                 auto *kres = (!BATCH && schr) ? k_resolve<NW, PAULI, 0, 1> : k_resolve<NW, PAULI, BATCH, 0>;
-                hipLaunchKernelGGL(kres, dim3(rgrid), dim3(kBlock), rlds, stream, gd, ig, ep, rows.as<uint64_t>(), coef.as<double>(), cols.as<uint64_t>(), colstride(),
+                hipLaunchKernelGGL(kres, dim3(rgrid), dim3(kBlock), rlds, stream, gd, ig, ep, rows.as<uint64_t>(), cv, cols.as<uint64_t>(), colstride(),
                                    n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
                                    gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gidx_slots,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(),
                                    static_cast<uint32_t>(gsum_stride), group_log2, mem, gtrig,
-                                   schr ? occ_dev.as<uint64_t>() : static_cast<uint64_t *>(nullptr));
+                                   schr ? occ_dev.as<uint64_t>() : static_cast<uint64_t *>(nullptr), ga);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_RESOLVE, pa);
+                if (graph) {
+                    LayerCols lc{};
+                    lc.n = gd.fold_cnt;
+                    for (int f = 0; f < std::min(gd.fold_cnt, kLayerCols); ++f)
+                        lc.col[f] = folds[static_cast<size_t>(gd.fold_off + f)];
+                    const uint64_t cchunks = (cap / 64 + kCosChunkWords - 1) / kCosChunkWords;
+                    hipLaunchKernelGGL(k_build_cos, dim3(static_cast<uint32_t>((cchunks + kWaves - 1) / kWaves)),
+                                       dim3(kBlock), 0, stream, cols.as<uint64_t>(), colstride(), lc,
+                                       fold.as<int32_t>() + gd.fold_off, gd.k, cv, pmark.as<uint8_t>(),
+                                       counters.as<uint64_t>(), ig);
+                    HIP_TRY(hipGetLastError());
+                }
             }
             sync_counters();
             prof_resolve();
@@ -3411,6 +3521,33 @@ struct mpa_engine {
             stat_p[g] = st[kStatW * g + 5];
         }
         if (prof_on) add_algorithmic_bytes_fused(gds);
+        if (graph) {
+            // per gate: rotations = hits + inserts; cos-only rows = anticommuting - rows in rotations
+            // below the live count (2 per hit, 1 per insert)
+            uint64_t cb = gcos_total, rb = grot_total;
+            for (size_t g = 0; g < ng; ++g) {
+                const uint64_t a = static_cast<uint64_t>(stat_a[g]), h = static_cast<uint64_t>(stat_p[g]);
+                const uint64_t ins = static_cast<uint64_t>(stat_i[g]);
+                if (a < 2 * h + ins) throw RunError("graph build: inconsistent gate counts at gate " + std::to_string(g));
+                const uint64_t ce = cb + (a - 2 * h - ins), re = rb + h + ins;
+                LayerInfo ly{cb, ce, rb, re, gds[g].param_index, gds[g].gen_coeff, static_cast<uint64_t>(stat_n[g]),
+                             static_cast<int32_t>(graph_folds.size()), 0, 0};
+                if (ce > cb) {
+                    graph_folds.insert(graph_folds.end(), folds.begin() + gds[g].fold_off,
+                                       folds.begin() + gds[g].fold_off + gds[g].fold_cnt);
+                    ly.fold_cnt = gds[g].fold_cnt;
+                }
+                layers.push_back(ly);
+                cb = ce;
+                rb = re;
+            }
+            sync_counters();
+            if (static_cast<uint64_t>(h_counters[C_GROT]) != rb)
+                throw RunError("graph build: " + std::to_string(h_counters[C_GROT]) + " rotation records written, " +
+                               std::to_string(rb) + " counted");
+            gcos_total = cb;
+            grot_total = rb;
+        }
     }
 
     // <O> of every batch member (its identity coefficient plus its paired rows), into out[nbatch].
@@ -3584,7 +3721,7 @@ struct RunPropagate {
     template <int NW, int PAULI>
     void operator()() {
         if (fused) {
-            e->propagate_fused<NW, PAULI, 0>(gds, folds);
+            e->propagate_fused<NW, PAULI, 0>(gds, folds, graph ? trl : e->tr, graph);
         } else {
             e->gidx_rows = static_cast<uint64_t>(-1);  // the unfused pipeline does not maintain the index
             e->propagate_impl<NW, PAULI>(gds, folds, graph, trl);
@@ -4551,9 +4688,12 @@ int mpa_build_graph(mpa_engine *e, const int64_t *gen_offsets, const int32_t *ge
             trl.check_upper = 0;
         }
         const size_t first_layer = e->layers.size();
+        // without an orbital cap the build runs on the fused kernels (GraphArgs); the cap's two-pass
+        // cosine arm, and mpa_set_pipeline(e, 0), take the per-gate pipeline
+        const bool fused = only_rotate_len_k < 0 && e->fused_enabled;
         e->building = true;
         try {
-            RunPropagate run{e, gds, folds, true, trl, false};
+            RunPropagate run{e, gds, folds, true, trl, fused};
             e->dispatch(run);
         } catch (...) {
             e->building = false;

@@ -170,3 +170,59 @@ def test_checkpointed_gradient_independent_of_segments(oracle_mod, schr):
     vo, go = o.graph_grad(np.asarray(x))
     assert vo == v1
     assert np.allclose(g1, go, rtol=1e-10, atol=1e-12)
+
+
+def _build(basis, n, terms, circ, pipeline_fused, *, cutoff, cutoff_type, lower, state=(), schr=None, seeded=True):
+    monos, gcs, pm, gidx = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    d = DeviceSimulator(terms, cutoff, list(state), lower_atol=lower, cutoff_type=cutoff_type, logical_num_modes=n,
+                        basis=basis, schrodinger_cutoff=schr)
+    d.set_pipeline(pipeline_fused)
+    d.build_graph(monos, pm, gcs, gidx, list(circ.parameters) if seeded else None)
+    return d
+
+
+@pytest.mark.parametrize("case", ["ring64", "lattice5", "hubbard3", "ring8_schr", "ring64_structural"])
+def test_graph_build_pipelines_agree(oracle_mod, case):
+    """A graph built on the fused kernels (the default without an orbital cap) and one built by the
+    per-gate pipeline hold the same store, the same layer records (rotation and recomputed cos-only
+    counts), replay to the same coefficient vector bit for bit, and give the oracle's replayed <O>; their
+    gradients agree to rounding (the records' order inside a layer differs)."""
+    print(f"case: graph build pipelines {case}", flush=True)
+    schr, seeded = None, True
+    if case == "ring64":
+        circ, n, basis, ct, cut, low = workloads.brickwork_ring(64, 3, seed=5), 64, "pauli", "support", 8, 1e-6
+        terms, state = workloads.ring_zz_mean(64).get_local_operator().terms, []
+    elif case == "ring64_structural":
+        circ, n, basis, ct, cut, low = workloads.brickwork_ring(64, 1, seed=2), 64, "pauli", "support", 4, None
+        terms, state, seeded = workloads.ring_zz_mean(64).get_local_operator().terms, [], False
+    elif case == "lattice5":
+        circ, n, basis, ct, cut, low = workloads.kicked_ising_lattice(5, 3, theta_h=0.4), 25, "pauli", "support", 8, 1e-5
+        terms, state = workloads.mean_z(25).get_local_operator().terms, []
+    elif case == "hubbard3":
+        circ, n = workloads.hubbard_lattice(3, 2)
+        basis, ct, cut, low = "majorana", "length", 6, 1e-6
+        terms, state = workloads.number_operator(0, n).terms, list(circ.initial_state)
+    else:
+        circ, n, basis, ct, cut, low = workloads.brickwork_ring(8, 2, seed=1), 8, "pauli", "support", 4, 1e-8
+        terms, state, schr = workloads.ring_zz_mean(8).get_local_operator().terms, [0, 5], 4
+    a = _build(basis, n, terms, circ, True, cutoff=cut, cutoff_type=ct, lower=low, state=state, schr=schr,
+               seeded=seeded)
+    b = _build(basis, n, terms, circ, False, cutoff=cut, cutoff_type=ct, lower=low, state=state, schr=schr,
+               seeded=seeded)
+    wa, ca = a.rows()
+    wb, cb = b.rows()
+    assert np.array_equal(wa, wb) and np.array_equal(ca, cb)
+    assert a.graph_size() == b.graph_size() and a.graph_layers() == b.graph_layers()
+    theta = np.asarray(circ.parameters, dtype=np.float64) * 0.9 + 0.05
+    assert np.array_equal(np.asarray(a.contract_partially(theta, False)), np.asarray(b.contract_partially(theta, False)))
+    va, ga = a.expectation_value_and_gradient(theta)
+    vb, gb = b.expectation_value_and_gradient(theta)
+    assert va == vb
+    assert np.allclose(ga, gb, rtol=1e-10, atol=1e-14)
+    if schr is None:
+        monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+        o = oracle_mod.Oracle(terms, n, basis=basis, cutoff=cut, cutoff_type=ct, lower_atol=low, initial_state=state)
+        o.build_graph(monos, pm, gcs, circ.parameters if seeded else None)
+        vo, go = o.graph_grad(theta)
+        assert vo == va
+        assert np.allclose(ga, go, rtol=1e-9, atol=1e-13)
