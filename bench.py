@@ -442,6 +442,21 @@ def propagate_merge_fractions(prof):
     return out
 
 
+def step_rate(prof, ms_per_step):
+    """Device-wide rate of one timed step: the sweep and resolve launches' algorithmic bytes and PMC
+    traffic per step over the wall time per step."""
+    ks = [k for k in ("k_sweep", "k_resolve") if k in prof]
+    if not ks or ms_per_step <= 0:
+        return None
+    alg = sum(prof[k]["bytes"] for k in ks) / (ms_per_step * 1e-3) / 1e9
+    out = {"algorithmic_GBps": round(alg, 1), "algorithmic_frac": round(alg / HBM_PEAK_GBS, 4)}
+    traffic = [measured_traffic(k) for k in ks] if SCHRODINGER_QUBITS is None else [None]
+    if all(t is not None for t in traffic):
+        hbm = sum(t * prof[k]["launches"] for t, k in zip(traffic, ks)) / (ms_per_step * 1e-3) / 1e9
+        out.update({"hbm_GBps": round(hbm, 1), "hbm_frac": round(hbm / HBM_PEAK_GBS, 4)})
+    return out
+
+
 def main():
     global THETA_H, SCHRODINGER_QUBITS
     args = parse_args()
@@ -577,6 +592,10 @@ def main():
         },
         "kernel_ms_per_step": round(kern_ms, 3),
         "propagate_merge": propagate_merge_fractions(prof),
+        # the whole device over the timed step: one step's algorithmic bytes and PMC traffic of the sweep
+        # and resolve launches (from the profiled step: the same gates) over the timed wall time per step,
+        # which with the concurrent schedule covers three propagations overlapping on the device
+        "step_rate": step_rate(prof, elapsed * 1e3 / args.steps),
         "all_kernels_GBps": kern_bytes / (kern_ms * 1e6) if kern_ms > 0 else 0.0,
         "profiled_step": {"propagate_ms": round(prof_wall_ms, 2), "kernel_ms": round(kern_ms, 2)},
     }

@@ -190,6 +190,13 @@ int mpa_set_pipeline(mpa_engine *e, int fused);
  * them. */
 int mpa_set_launch(mpa_engine *e, int32_t resolve_grid, int32_t min_group_log2);
 
+/* No counterpart in the reference (its store grows until host memory runs out): the largest term store
+ * a lone propagator may hold, in rows (0: as much as device memory takes).  When the store cannot grow
+ * to the worst case of the next gate (twice the live terms) within the limit or device memory, each
+ * gate is bounded by its own anticommuting terms instead, counted on the device; a gate that does not
+ * fit even so raises -2.  Results do not depend on it. */
+int mpa_set_store_limit(mpa_engine *e, int64_t rows);
+
 /* Device time of the last propagate call from first to last launch (HIP events), in ms. */
 int mpa_last_propagate_ms(mpa_engine *e, double *ms);
 

@@ -68,6 +68,7 @@ _SIGNATURES = {
     "mpa_gate_stats": (C.c_int, [_P, _i64p, _i64p, _i64p, _i64p, _i64p]),
     "mpa_set_pipeline": (C.c_int, [_P, C.c_int]),
     "mpa_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "mpa_set_store_limit": (C.c_int, [_P, C.c_int64]),
     "mpa_set_profiling": (C.c_int, [_P, C.c_int]),
     "mpa_kernel_profile": (C.c_int, [_P, C.c_int, C.POINTER(C.c_char_p), _f64p, _f64p, _i64p]),
     "mpa_last_propagate_ms": (C.c_int, [_P, _f64p]),
@@ -524,6 +525,12 @@ class DeviceSimulator(_EngineHandle):
     def set_launch(self, resolve_grid: int, min_group_log2: int) -> None:
         """Launch parameters of the fused pipeline (results do not depend on them)."""
         _check(load_library().mpa_set_launch(self._h, int(resolve_grid), int(min_group_log2)))
+
+    def set_store_limit(self, rows: int) -> None:
+        """Largest term store in rows (0: device memory decides).  Past the 2N worst case of a gate the
+        store is bounded gate by gate by the gate's anticommuting terms (results do not depend on it); a
+        gate that does not fit raises RuntimeError."""
+        _check(load_library().mpa_set_store_limit(self._h, int(rows)))
 
     def set_profiling(self, on: bool) -> None:
         """Bracket every kernel launch with HIP events (resets the accumulators)."""

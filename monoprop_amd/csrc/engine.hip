@@ -63,6 +63,8 @@ enum : int {
     // gate parity (C_NB + (gate & 1))
     C_TK_RES = 11,
     C_NB = 16,
+    // fused pipeline on a tight store: anticommuting rows of the gate, per gate parity (C_ACNT + (gate & 1))
+    C_ACNT = 18,
     C_COUNT = 24
 };
 // per-gate stats: N, nA, emitted, inserted, leaders, pairs, graph cos-record end, graph rotation end
@@ -1005,6 +1007,34 @@ struct GraphArgs {
     uint64_t rot_cap;
 };
 
+// Store bound of a gate on a tight store (the store could not grow to the 2N worst case): its
+// anticommuting rows among the live ones, which bound the rows the gate can append, added into
+// counters[C_ACNT + (gi & 1)] (zeroed by the previous gate).  Grid-stride over the column words.
+__global__ __launch_bounds__(kBlock) void k_count_anti(const uint64_t *__restrict__ cols, uint64_t colstride,
+                                                       SweepCols sc, const int32_t *__restrict__ fold,
+                                                       uint64_t *__restrict__ counters, int gi) {
+    __shared__ uint32_t lds[kWaves];
+    if (counters[C_STOP]) return;
+    const uint64_t N = counters[C_NB + (gi & 1)];
+    const uint64_t nwords = (N + 63) / 64;
+    uint32_t cnt = 0;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; w < nwords;
+         w += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        uint64_t y[kColArgs];
+#pragma unroll
+        for (int f = 0; f < kColArgs; ++f) y[f] = f < sc.nfold ? cols[static_cast<uint64_t>(sc.col[f]) * colstride + w] : 0;
+        uint64_t a = 0;
+#pragma unroll
+        for (int f = 0; f < kColArgs; ++f) a ^= y[f];
+        for (int f = kColArgs; f < sc.nfold; ++f) a ^= cols[static_cast<uint64_t>(fold[f]) * colstride + w];
+        if (w == nwords - 1 && (N & 63)) a &= (uint64_t{1} << (N & 63)) - 1;
+        cnt += static_cast<uint32_t>(__popcll(a));
+    }
+    const uint32_t tot = block_sum(cnt, lds);
+    if (threadIdx.x == 0 && tot)
+        atomicAdd(reinterpret_cast<unsigned long long *>(counters + C_ACNT + (gi & 1)), static_cast<unsigned long long>(tot));
+}
+
 // Anticommutation + fused cos sweep + emission gate (Scan.h:384-401) over one gate.  One wavefront
 // (one 64-thread block) per 1024-term tile, no dependence between tiles:
 //   A  G's fold and pivot columns of the tile's 16 words (lane = slot * 16 + word), loaded by the kernel
@@ -1197,7 +1227,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
                                                   uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
                                                   int group_log2, uint64_t *__restrict__ counters,
                                                   const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig,
-                                                  uint32_t nbatch, uint64_t cap_rows, GraphArgs ga) {
+                                                  uint32_t nbatch, uint64_t cap_rows, GraphArgs ga, int tight) {
     __shared__ uint64_t sem[kWaves][kWTileWords];
     __shared__ uint16_t cidx[kWaves][kWTileRows];
     __shared__ uint64_t s_grp[kWaves];
@@ -1205,6 +1235,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
     const uint64_t stop = counters[C_STOP];
     const uint64_t N = counters[C_NB + (gi & 1)];
     const uint64_t nrec = GRAPH ? counters[C_GROT] : 0;
+    const uint64_t acnt = counters[C_ACNT + (gi & 1)];
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t tile0 = static_cast<uint64_t>(blockIdx.x) * kWaves;
     const uint64_t tile = tile0 + wave;
@@ -1218,10 +1249,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
         __syncthreads();
     }
     if (stop) return;
-    // a gate may double the live count: stop before it when the store could not take that (the host
-    // grows the store and resumes here)
-    // (a graph build also stops when this gate's records, at most one per live term, might not fit)
-    if (2 * N > cap_rows || (GRAPH && nrec + N > ga.rot_cap)) {
+    // a gate may double the live count (on a tight store: add at most its anticommuting rows, counted by
+    // k_count_anti): stop before it when the store could not take that (the host grows the store and
+    // resumes here); a graph build also stops when this gate's records (at most one per live term) might
+    // not fit
+    if ((tight ? N + acnt : 2 * N) > cap_rows || (GRAPH && nrec + N > ga.rot_cap)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_STOP] = static_cast<uint64_t>(gi) + 1;
         return;
     }
@@ -1229,6 +1261,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counters[C_TK_RES] = 0;
         counters[C_NB + ((gi & 1) ^ 1)] = N;
+        counters[C_ACNT + ((gi & 1) ^ 1)] = 0;
     }
     const uint64_t nwords = (N + 63) / 64;
     if (tile0 * kWTileWords >= nwords) return;  // whole block idle
@@ -1609,6 +1642,7 @@ __global__ void k_advance_fused(uint64_t *__restrict__ counters, int64_t *__rest
     int64_t *s = stats + static_cast<int64_t>(gi) * kStatW;
     s[0] = static_cast<int64_t>(counters[C_NB + (gi & 1)]);
     counters[C_NB + ((gi & 1) ^ 1)] = counters[C_NB + (gi & 1)];
+    counters[C_ACNT + ((gi & 1) ^ 1)] = 0;
 }
 
 
@@ -2340,6 +2374,7 @@ struct mpa_engine {
     uint32_t nbatch = 0;
     std::vector<double> bcore;
     uint64_t cap_limit = 0;  // fused pipeline: stop before a gate that would grow the store past this (0: none)
+    uint64_t store_limit = 0;  // lone propagation: rows the store may not outgrow (0: device memory decides)
     size_t fused_done = 0;   // gates the last fused propagate applied
     DevArray bmember{&stream}, btrig{&stream}, bout{&stream};
 
@@ -2441,10 +2476,10 @@ struct mpa_engine {
     uint64_t colstride() const { return cap / 64; }
 
     // Grow the store to hold at least `need` rows, preserving rows, coefficients and columns.
-    void ensure_capacity(uint64_t need) {
+    void ensure_capacity(uint64_t need, bool exact = false) {
         if (need <= cap) return;
         try {
-            grow_store(need);
+            grow_store(need, exact);
         } catch (const HipFailure &f) {
             throw RunError("out of device memory growing the term store to " + std::to_string(need) + " rows (" +
                            std::to_string(N) + " live terms): " + f.what);
@@ -2462,6 +2497,20 @@ struct mpa_engine {
         cap = c;
         HIP_TRY(hipMemsetAsync(cols.p, 0, static_cast<size_t>(ncols) * (c / 64) * 8, stream));
         if (nbatch) bmember.reserve(c * 2);
+    }
+
+    // Grows the store to `need` rows (exact: the granule above it, else by at least half), first giving
+    // back the capacity-sized work arrays when `release` (they are rebuilt); false when the device memory
+    // does not hold it (the store is then unchanged).
+    bool try_grow(uint64_t need, bool exact, bool release) {
+        if (need <= cap) return true;
+        if (release) release_work_arrays();
+        try {
+            ensure_capacity(need, exact);
+        } catch (const RunError &) {
+            return false;
+        }
+        return true;
     }
 
     // Gives back the arrays sized by the capacity other than the store itself (the fused pipeline's,
@@ -2500,8 +2549,8 @@ struct mpa_engine {
         blk_b.reserve(nblk * 4);
     }
 
-    void grow_store(uint64_t need) {
-        const uint64_t nc = grown_capacity(need);
+    void grow_store(uint64_t need, bool exact = false) {
+        const uint64_t nc = exact ? (need + kGranule - 1) / kGranule * kGranule : grown_capacity(need);
         DevArray r2{&stream}, c2{&stream}, k2{&stream};
         r2.reserve(nc * nw * 8);
         c2.reserve(nc * 8);
@@ -3253,10 +3302,12 @@ struct mpa_engine {
 
     // Arrays of the fused pipeline sized to the store (a gate runs only while N <= cap / 2), and a global
     // index with >= 2 slots per row of capacity covering every stored row.
+    // Per-tile arrays cover the rows a gate can sweep: half the capacity normally (a gate starts at
+    // N <= cap / 2), all of it on a tight store.
     template <int NW, int BATCH>
-    void ensure_fused_arrays() {
-        if (fused_cap != cap) {
-            cap_items = cap / 2;
+    void ensure_fused_arrays(bool tight = false) {
+        if (fused_cap != cap || cap_items != (tight ? cap : cap / 2)) {
+            cap_items = tight ? cap : cap / 2;
             const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
             t_item.reserve(tiles * kWTileRows * 4);
             t_cnt.reserve(tiles * 8);
@@ -3339,6 +3390,7 @@ struct mpa_engine {
         h[C_N] = N;
         h[C_STOP] = 0;
         h[C_TK_RES] = 0;
+        h[C_ACNT] = h[C_ACNT + 1] = 0;
         HIP_TRY(hipMemcpyAsync(counters.p, h, C_COUNT * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
     }
@@ -3377,13 +3429,47 @@ struct mpa_engine {
         HIP_TRY(hipEventRecord(ev[0], stream));
         size_t gi = 0;
         double rate = 1.0;  // per-gate growth of the live count over the last chunk
+        // A gate may at most double the live count, so the store is grown to more than 2N before a gate
+        // whose worst case it could not take.  When that cannot be had (out of device memory, or past a
+        // batch's cap_limit) the store turns tight: each gate's anticommuting rows are counted first
+        // (k_count_anti) and bound what it can append, and the store grows only to what a stopped gate
+        // needs.
+        bool tight = false;
+        uint64_t tight_need = 0;   // live + anticommuting rows of the gate a tight store stopped before
+        uint64_t no_grow_cap = 0;  // capacity at which growing past the 2N worst case failed
+        // a batch's cap_limit stops the batch (its driver splits it); a lone store's limit is an error
+        const uint64_t limit = cap_limit ? cap_limit : store_limit;
         while (gi < ng) {
-            if (2 * N > cap) {  // a gate may at most double the live count
+            if (2 * N > cap) {
                 const uint64_t need = 2 * N + N / 4;
-                if (cap_limit && grown_capacity(need) > cap_limit) break;  // stop before gate gi
-                ensure_capacity(need);
+                const bool allowed = !(limit && grown_capacity(need) > limit);
+                if (allowed && no_grow_cap != cap && try_grow(need, false, false)) {
+                    tight = false;
+                } else {
+                    no_grow_cap = cap;
+                    tight = true;
+                    if (tight_need > cap) {
+                        const uint64_t lim = limit ? std::max<uint64_t>(limit / kGranule * kGranule, cap) : UINT64_MAX;
+                        if (tight_need > lim) {
+                            if (cap_limit) break;  // a batch stops before gate gi (the driver splits it)
+                            throw RunError("the term store limit of " + std::to_string(store_limit) +
+                                           " rows cannot take gate " + std::to_string(gi) + " (" + std::to_string(N) +
+                                           " live terms, at most " + std::to_string(tight_need - N) + " more)");
+                        }
+                        const uint64_t want = std::min(grown_capacity(tight_need), lim);
+                        if (!try_grow(want, true, true) && !try_grow(tight_need, true, true))
+                            throw RunError("out of device memory growing the term store to " +
+                                           std::to_string(tight_need) + " rows (" + std::to_string(N) +
+                                           " live terms, at most " + std::to_string(tight_need - N) +
+                                           " more this gate)");
+                        no_grow_cap = cap;
+                    }
+                }
+            } else {
+                tight = false;
             }
-            ensure_fused_arrays<NW, BATCH>();
+            tight_need = 0;
+            ensure_fused_arrays<NW, BATCH>(tight);
             GraphArgs ga{nullptr, nullptr, nullptr, 0};
             if (graph) {
                 // room for at least this gate's records (one per live term at most), grown by half
@@ -3448,13 +3534,19 @@ struct mpa_engine {
                     sc.ncol = c;
                 }
                 pa = prof_begin();
+                if (tight) {
+                    hipLaunchKernelGGL(k_count_anti, dim3(std::min<uint32_t>(sgrid, 256)), dim3(kBlock), 0, stream,
+                                       cols.as<uint64_t>(), colstride(), sc, fold.as<int32_t>() + gd.fold_off,
+                                       counters.as<uint64_t>(), ig);
+                    HIP_TRY(hipGetLastError());
+                }
                 auto *ksw = (!BATCH && graph) ? k_sweep<NW, PAULI, 0, 1> : k_sweep<NW, PAULI, BATCH, 0>;
                 hipLaunchKernelGGL(ksw, dim3(sgrid), dim3(kBlock),
                                    BATCH ? static_cast<size_t>(nbatch) * 16 : 0, stream, gd, ig, trc,
                                    cols.as<uint64_t>(), colstride(), sc, fold.as<int32_t>() + gd.fold_off,
                                    rows.as<uint64_t>(), cv, t_val.as<double>(), t_item.as<uint32_t>(),
                                    f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur,
-                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch, cap, ga);
+                                   group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch, cap, ga, tight ? 1 : 0);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
@@ -3491,6 +3583,7 @@ struct mpa_engine {
             const uint64_t stop = static_cast<uint64_t>(h_counters[C_STOP]);
             const size_t next = stop ? static_cast<size_t>(stop - 1) : end;
             N = static_cast<uint64_t>(h_counters[C_NB + (next & 1)]);
+            if (stop && tight) tight_need = N + static_cast<uint64_t>(h_counters[C_ACNT + (next & 1)]);
             if (next > gi && n_start > 0 && N > n_start)
                 rate = std::pow(static_cast<double>(N) / static_cast<double>(n_start), 1.0 / static_cast<double>(next - gi));
             else
@@ -4907,6 +5000,13 @@ int mpa_set_launch(mpa_engine *e, int32_t resolve_grid, int32_t min_group_log2) 
             e->min_group_log2 = min_group_log2;
             e->fused_cap = 0;  // the count groups are laid out again before the next fused gate
         }
+    });
+}
+
+int mpa_set_store_limit(mpa_engine *e, int64_t rows) {
+    return guarded([&] {
+        if (rows < 0) throw ArgError("rows must be non-negative");
+        e->store_limit = static_cast<uint64_t>(rows);
     });
 }
 
