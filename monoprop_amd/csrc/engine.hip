@@ -2177,8 +2177,10 @@ class BlockCache {
             release_all();
             e = hipMalloc(&p, bytes);
         }
-        if (e != hipSuccess)
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation is reported here, not by the next launch check
             throw HipFailure(std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+        }
         return p;
     }
     void give(void *p, size_t bytes, hipStream_t s) {
@@ -2713,7 +2715,8 @@ struct mpa_engine {
             const uint64_t nF = static_cast<uint64_t>(h_counters[C_NF]);
             const uint64_t nA = nL + nF;
             n_ub = N + nA;
-            if (n_ub >= (uint64_t{1} << 32)) throw RunError("term count exceeds the 32-bit row index");
+            // (graph records keep the phase in bit 31 of the target row)
+            if (n_ub >= (uint64_t{1} << (graph ? 31 : 32))) throw RunError("term count exceeds the 32-bit row index");
             ensure_capacity(n_ub);
             ensure_items(nL, nA);
             double *cv = graph ? seedc.as<double>() : coef.as<double>();
@@ -3437,8 +3440,11 @@ struct mpa_engine {
         bool tight = false;
         uint64_t tight_need = 0;   // live + anticommuting rows of the gate a tight store stopped before
         uint64_t no_grow_cap = 0;  // capacity at which growing past the 2N worst case failed
-        // a batch's cap_limit stops the batch (its driver splits it); a lone store's limit is an error
-        const uint64_t limit = cap_limit ? cap_limit : store_limit;
+        // a batch's cap_limit stops the batch (its driver splits it); a lone store's limit is an error.
+        // Rows are 32-bit indices (row + 1 in the index slots, 31 bits in graph records).
+        const uint64_t row_cap = graph ? (uint64_t{1} << 31) : (uint64_t{1} << 32) - kGranule;
+        const uint64_t user_limit = cap_limit ? cap_limit : store_limit;
+        const uint64_t limit = user_limit ? std::min(user_limit, row_cap) : row_cap;
         while (gi < ng) {
             if (2 * N > cap) {
                 const uint64_t need = 2 * N + N / 4;
@@ -3451,7 +3457,11 @@ struct mpa_engine {
                     if (tight_need > cap) {
                         const uint64_t lim = limit ? std::max<uint64_t>(limit / kGranule * kGranule, cap) : UINT64_MAX;
                         if (tight_need > lim) {
-                            if (cap_limit) break;  // a batch stops before gate gi (the driver splits it)
+                            if (cap_limit && tight_need <= row_cap) break;  // a batch stops (its driver splits it)
+                            if (tight_need > row_cap)
+                                throw RunError("term count exceeds the 32-bit row index at gate " + std::to_string(gi) +
+                                               " (" + std::to_string(N) + " live terms, at most " +
+                                               std::to_string(tight_need - N) + " more)");
                             throw RunError("the term store limit of " + std::to_string(store_limit) +
                                            " rows cannot take gate " + std::to_string(gi) + " (" + std::to_string(N) +
                                            " live terms, at most " + std::to_string(tight_need - N) + " more)");
@@ -3589,7 +3599,7 @@ struct mpa_engine {
             else
                 rate = 1.0;
             gidx_rows = N;
-            if (N >= (uint64_t{1} << 31)) throw RunError("term count exceeds the 32-bit row index");
+            if (N > row_cap) throw RunError("term count exceeds the 32-bit row index");  // (the bound above prevents it)
             reset_fused_counters(next);
             gi = next;
         }

@@ -31,8 +31,10 @@ def _sim(case):
 
 
 def _capacity_rows(d):
+    """Rows of store capacity: the live terms plus the slack rows (d_terms_slack_bytes counts the
+    capacity's unused rows; the arrays themselves may sit in larger cached device blocks)."""
     words = d.rows()[0].shape[1]
-    return d.operator_memory_breakdown()["operator_terms_bytes"] // (8 * words)
+    return d.size() + d.operator_memory_breakdown()["d_terms_slack_bytes"] // (8 * words)
 
 
 @pytest.mark.parametrize("name,factor", [("hubbard6_3steps", 1.4), ("hubbard6_2steps", 1.4), ("lattice10_full_h04", 1.25)])
