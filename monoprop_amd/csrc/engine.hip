@@ -3433,10 +3433,10 @@ struct mpa_engine {
         size_t gi = 0;
         double rate = 1.0;  // per-gate growth of the live count over the last chunk
         // A gate may at most double the live count, so the store is grown to more than 2N before a gate
-        // whose worst case it could not take.  When that cannot be had (out of device memory, or past a
-        // batch's cap_limit) the store turns tight: each gate's anticommuting rows are counted first
-        // (k_count_anti) and bound what it can append, and the store grows only to what a stopped gate
-        // needs.
+        // whose worst case it could not take.  A batch past its cap_limit stops there (its driver
+        // splits it).  A lone store that cannot have that (out of device memory, or past its
+        // store_limit) turns tight: each gate's anticommuting rows are counted first (k_count_anti) and
+        // bound what it can append, and the store grows only to what a stopped gate needs.
         bool tight = false;
         uint64_t tight_need = 0;   // live + anticommuting rows of the gate a tight store stopped before
         uint64_t no_grow_cap = 0;  // capacity at which growing past the 2N worst case failed
@@ -3446,7 +3446,14 @@ struct mpa_engine {
         const uint64_t user_limit = cap_limit ? cap_limit : store_limit;
         const uint64_t limit = user_limit ? std::min(user_limit, row_cap) : row_cap;
         while (gi < ng) {
-            if (2 * N > cap) {
+            if (2 * N > cap && cap_limit) {
+                // a batch splits rather than going tight: smaller batches at the 2N rule run faster than
+                // a tight store with a count per gate (measured on the parameter sweep)
+                const uint64_t need = 2 * N + N / 4;
+                if (grown_capacity(need) > limit) break;  // stop before gate gi (the driver splits the batch)
+                ensure_capacity(need);
+                tight = false;
+            } else if (2 * N > cap) {
                 const uint64_t need = 2 * N + N / 4;
                 const bool allowed = !(limit && grown_capacity(need) > limit);
                 if (allowed && no_grow_cap != cap && try_grow(need, false, false)) {
@@ -3457,7 +3464,6 @@ struct mpa_engine {
                     if (tight_need > cap) {
                         const uint64_t lim = limit ? std::max<uint64_t>(limit / kGranule * kGranule, cap) : UINT64_MAX;
                         if (tight_need > lim) {
-                            if (cap_limit && tight_need <= row_cap) break;  // a batch stops (its driver splits it)
                             if (tight_need > row_cap)
                                 throw RunError("term count exceeds the 32-bit row index at gate " + std::to_string(gi) +
                                                " (" + std::to_string(N) + " live terms, at most " +
@@ -3466,8 +3472,13 @@ struct mpa_engine {
                                            " rows cannot take gate " + std::to_string(gi) + " (" + std::to_string(N) +
                                            " live terms, at most " + std::to_string(tight_need - N) + " more)");
                         }
-                        const uint64_t want = std::min(grown_capacity(tight_need), lim);
-                        if (!try_grow(want, true, true) && !try_grow(tight_need, true, true))
+                        // the largest of five steps from half again the capacity down to the exact need
+                        // that device memory holds (each failed try allocates nothing)
+                        const uint64_t hi = std::max(std::min(grown_capacity(tight_need), lim), tight_need);
+                        bool grown = false;
+                        for (int q = 4; q >= 0 && !grown; --q)
+                            grown = try_grow(tight_need + (hi - tight_need) / 4 * static_cast<uint64_t>(q), true, true);
+                        if (!grown)
                             throw RunError("out of device memory growing the term store to " +
                                            std::to_string(tight_need) + " rows (" + std::to_string(N) +
                                            " live terms, at most " + std::to_string(tight_need - N) +
