@@ -8,7 +8,16 @@
 //           Majorana: slot s -> column s, popcount parity -> column 2n.   detail/operator/InvertedIndex.h)
 //
 // Per gate G (reference build_layer + apply_fused_contract, layer_build/Engine.h:522-637,
-// FusedApply.h:32-83), all on one stream:
+// FusedApply.h:32-83), all on one stream.  The default (fused) pipeline is two launches per gate with no
+// host synchronisation between gates:
+//   k_sweep      anticommutation over G's columns, c <- c cos 2a in place, the emission gate, emitters
+//                listed per 1024-term tile with their pre-gate values
+//   k_resolve    every emitter's partner looked up in a keyless global row index, the rotation applied,
+//                misses appended in the reference's order by a single-pass chained scan
+// A graph build runs the same two (the sweep's graph instance leaves the coefficients unscaled, the
+// resolve records the rotations) plus k_build_cos for the cos-only rows; graph replays and the adjoint
+// run one k_layer launch per layer.  A store that cannot hold a gate's 2N worst case is bounded gate
+// by gate by k_count_anti.  The per-gate pipeline (orbital cap, mpa_set_pipeline(e, 0)) is:
 //   k_scan       XOR of G's fold columns -> anticommuting masks, split by G's pivot bit into leaders
 //                (pivot clear) and followers (pivot set) (Scan.h:125-185, 265-269)
 //   k_scan_sums  exclusive scan of per-block counts
