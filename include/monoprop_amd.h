@@ -143,7 +143,8 @@ int mpa_contract(mpa_engine *e, const double *params, int64_t n_params, int inpl
 int mpa_size(mpa_engine *e, int64_t *out);
 
 /* Graph metadata (binder.h:235-240): stored layers and the parameter count they reference; the
- * number of cos-only records and rotation records the layers hold (reference graph_size, binder.h:236). */
+ * layers' cos-only rows (recomputed from the column index at replay, not stored) and their stored
+ * rotation records (reference graph_size, binder.h:236). */
 int mpa_graph_layers(mpa_engine *e, int64_t *layers, int64_t *n_params);
 int mpa_graph_records(mpa_engine *e, int64_t *cos_only, int64_t *rotations);
 

@@ -481,7 +481,8 @@ class DeviceSimulator(_EngineHandle):
         return self._n_gates if self.graph_layers() > 0 else 0
 
     def graph_size(self) -> tuple[int, int]:
-        """``(cos-only records, rotations)`` stored by the graph."""
+        """``(cos-only rows, rotations)`` of the graph's layers (the rotations are stored, the cos-only rows
+        recomputed at replay)."""
         c, r = C.c_int64(), C.c_int64()
         _check(load_library().mpa_graph_records(self._h, C.byref(c), C.byref(r)))
         return c.value, r.value
