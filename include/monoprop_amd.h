@@ -186,9 +186,9 @@ int mpa_kernel_profile(mpa_engine *e, int kind, const char **name, double *ms, d
 int mpa_set_pipeline(mpa_engine *e, int fused);
 
 /* Launch parameters of the fused pipeline (no counterpart in the reference; set by the autotuner,
- * monoprop_amd/tune.py): the number of k_resolve blocks that take emitter chunks (default 1280) and the
- * smallest count group, 2^min_group_log2 sweep tiles (default 6; 2..10).  Results do not depend on
- * them. */
+ * monoprop_amd/tune.py): the number of k_resolve blocks that take emitter chunks (default 128) and the
+ * smallest count group, 2^min_group_log2 sweep tiles (default 2, one sweep block; 2..10).  Results do
+ * not depend on them. */
 int mpa_set_launch(mpa_engine *e, int32_t resolve_grid, int32_t min_group_log2);
 
 /* No counterpart in the reference (its store grows until host memory runs out): the largest term store

@@ -952,9 +952,10 @@ __device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, in
 
 constexpr int kWTileWords = 16;                          // sweep: one wavefront per 16 column words
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
-// Count groups: 2^group_log2 tiles (64 tiles = 65536 terms, 16 atomics per word, up to 1024 tiles for
-// the largest stores) so that at most kMaxGroups prefixes sit in LDS.
-constexpr int kMinGroupLog2 = 3, kMaxGroupLog2 = 10;
+// Count groups: 2^group_log2 tiles, at least one sweep block (4 tiles) and up to 1024 tiles for the
+// largest stores, so that at most kMaxGroups prefixes sit in LDS.  A group of one block lets the resolve
+// place an emitter from the prefixes alone.
+constexpr int kMinGroupLog2 = 2, kMaxGroupLog2 = 10;
 constexpr uint32_t kMaxGroups = 4096;
 // per-group count word: leaders | followers << 21 | anticommuting << 42 (each <= 2^20)
 constexpr uint32_t kGrpBits = 21;
@@ -965,9 +966,6 @@ constexpr int kResChunk = kBlock;                         // resolve: one emitte
 constexpr uint32_t kMaxGrid = 128;
 constexpr uint32_t kGraphGrid = 512;  // resolve blocks of a graph build (measured: 512 beats 128, 1024, 2048)
 constexpr int kBsumBatch = 16;                            // resolve: sweep-block sums read per round
-// per-tile count word: leaders | followers << 16 | anticommuting << 32 (each <= 1024)
-constexpr uint32_t kCntBits = 16;
-constexpr uint64_t kCntMask = (uint64_t{1} << kCntBits) - 1;
 
 // Lane l's x as a wave-uniform (scalar) value; l must be wave-uniform.
 __device__ __forceinline__ uint64_t lane_value(uint64_t x, int l) {
@@ -1053,21 +1051,18 @@ __global__ __launch_bounds__(kBlock) void k_count_anti(const uint64_t *__restric
 //      registers.  Terms past the lower_atol gate go to the tile's candidate list (LDS)
 //   E  candidates dense over the lanes: partner key m ^ G against the structural cutoff (or the
 //      upper_atol rescue) -> emit bit (LDS)
-//   D  the tile's region [1024 tile, ...) of t_item (row) and t_val (pre-gate value) lists the emitters,
-//      leaders ascending then followers ascending (pivot clear / set): lane b writes those of bit b of
-//      every word, with the values it kept from B, at positions from per-word popcounts.  The tile's
-//      counts, also summed per group of tiles; leader emit masks per word
+//   D  the tile's counts (returned) and leader emit masks per word; the emitters themselves are
+//      written by sweep_emit once the block knows every tile's counts
 //   BATCH  rows belong to batch members (member[row]); each term takes its member's k and |s| from the
 //          block's copy of the gate's per-member constants (s_trig: k, |s| per member, LDS)
 template <int NW, int PAULI, int BATCH, int GRAPH>
 __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &tr, const uint64_t *__restrict__ cols,
                                                uint64_t colstride, const SweepCols &sc, const int32_t *__restrict__ fold,
                                                const uint64_t *xc, const uint64_t *__restrict__ rows,
-                                               double *__restrict__ coef, double *__restrict__ t_val,
-                                               uint32_t *__restrict__ t_item, uint64_t *__restrict__ elead,
-                                               uint64_t *__restrict__ tcnt, uint64_t N, uint64_t tile,
-                                               uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx,
-                                               const uint16_t *__restrict__ member, const double *__restrict__ s_trig) {
+                                               double *__restrict__ coef, uint64_t *__restrict__ elead, uint64_t N,
+                                               uint64_t tile, uint64_t *__restrict__ sem, uint16_t *__restrict__ cidx,
+                                               const uint16_t *__restrict__ member, const double *__restrict__ s_trig,
+                                               double *v, uint64_t &p_out) {
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t w0 = tile * kWTileWords;
     const uint32_t lane = lane_id();
@@ -1105,7 +1100,7 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     if (lane < kWTileWords) sem[lane] = 0;
     // ---- B: lane owns bit `lane` of each word
     uint32_t cand = 0, resc = 0;  // candidates; those the upper_atol rescue emits regardless
-    double v[kWTileWords];        // pre-gate values, kept for D
+    // v: pre-gate values (kWTileWords), kept for sweep_emit
     {
         // BATCH: the 32-bit word holding the row's 16-bit member (a whole-word load, so the loads of
         // all 16 words stay in flight together)
@@ -1182,30 +1177,34 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     }
     const uint32_t na = wave_sum(lane < kWTileWords ? static_cast<uint32_t>(__popcll(a)) : 0u);
     if (lane < kWTileWords && w0 + lane < nwords) elead[w0 + lane] = sem[lane] & ~p;
-    if (lane == 0)
-        tcnt[tile] = static_cast<uint64_t>(nl) | (static_cast<uint64_t>(nf) << kCntBits) |
-                     (static_cast<uint64_t>(na) << (2 * kCntBits));
-    if (nl + nf) {
-        uint32_t *region = t_item + tile * kWTileRows;
-        double *vregion = t_val + tile * kWTileRows;
-        const uint64_t below = (uint64_t{1} << lane) - 1;
-        uint32_t lpre = 0, fpre = nl;  // list positions before word j: leaders, followers
-#pragma unroll
-        for (int j = 0; j < kWTileWords; ++j) {
-            const uint64_t em = sem[j], pj = lane_value(p, j);
-            const uint64_t sel = em & ~pj, sef = em & pj;
-            if ((em >> lane) & 1u) {
-                const uint32_t pos = ((pj >> lane) & 1u) ? fpre + static_cast<uint32_t>(__popcll(sef & below))
-                                                         : lpre + static_cast<uint32_t>(__popcll(sel & below));
-                region[pos] = static_cast<uint32_t>(row0 + static_cast<uint64_t>(j) * 64 + lane);
-                vregion[pos] = v[j];
-            }
-            lpre += static_cast<uint32_t>(__popcll(sel));
-            fpre += static_cast<uint32_t>(__popcll(sef));
-        }
-    }
+    p_out = p;
     return static_cast<uint64_t>(nl) | (static_cast<uint64_t>(nf) << kGrpBits) |
            (static_cast<uint64_t>(na) << (2 * kGrpBits));
+}
+
+// The tile's emitters into its sweep block's list (t_item: row, t_val: pre-gate value), which holds the
+// block's leaders ascending, then its followers ascending (pivot clear / set): lane b writes those of
+// bit b of every word, with the values it kept from B, at lpre0 / fpre0 (the block's emitters of that
+// kind in earlier tiles; fpre0 includes all the block's leaders) plus per-word popcounts.
+__device__ __forceinline__ void sweep_emit(const uint64_t *__restrict__ sem, uint64_t p, const double *v,
+                                           uint32_t *__restrict__ region, double *__restrict__ vregion,
+                                           uint64_t row0, uint32_t lpre0, uint32_t fpre0) {
+    const uint32_t lane = lane_id();
+    const uint64_t below = (uint64_t{1} << lane) - 1;
+    uint32_t lpre = lpre0, fpre = fpre0;  // list positions before word j: leaders, followers
+#pragma unroll
+    for (int j = 0; j < kWTileWords; ++j) {
+        const uint64_t em = sem[j], pj = lane_value(p, j);
+        const uint64_t sel = em & ~pj, sef = em & pj;
+        if ((em >> lane) & 1u) {
+            const uint32_t pos = ((pj >> lane) & 1u) ? fpre + static_cast<uint32_t>(__popcll(sef & below))
+                                                     : lpre + static_cast<uint32_t>(__popcll(sel & below));
+            region[pos] = static_cast<uint32_t>(row0 + static_cast<uint64_t>(j) * 64 + lane);
+            vregion[pos] = v[j];
+        }
+        lpre += static_cast<uint32_t>(__popcll(sel));
+        fpre += static_cast<uint32_t>(__popcll(sef));
+    }
 }
 
 // Column words of G's first kColArgs columns for tile `tile` (lane = slot * 16 + word); the tiles cover
@@ -1223,7 +1222,9 @@ __device__ __forceinline__ void sweep_load_cols(const uint64_t *__restrict__ col
 }
 
 // One 1024-term tile per wavefront, four per block; the block stores its tiles' summed counts
-// (bsum[block]) and adds them into their group word with one atomic.  The live count, the stop flag,
+// (bsum[block]) and adds them into their group word with one atomic, and lists its emitters in its
+// region of t_item / t_val (block b: entries [4096 b, 4096 b + its emitters)), so that the resolve
+// finds emitter f of a block from the group prefixes and the block sums alone.  The live count, the stop flag,
 // the tile's column words and (BATCH) the per-member constants are all requested before anything
 // waits on any of them.  (A persistent grid of resident blocks walking the tiles, with the next
 // tiles' columns prefetched, was measured slower: more registers, and a barrier per tile set.)
@@ -1232,8 +1233,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
                                                   uint64_t colstride, SweepCols sc, const int32_t *__restrict__ fold,
                                                   const uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                   double *__restrict__ t_val, uint32_t *__restrict__ t_item,
-                                                  uint64_t *__restrict__ elead, uint64_t *__restrict__ tcnt,
-                                                  uint64_t *__restrict__ bsum, uint64_t *__restrict__ gsum,
+                                                  uint64_t *__restrict__ elead, uint64_t *__restrict__ bsum,
+                                                  uint64_t *__restrict__ gsum,
                                                   int group_log2, uint64_t *__restrict__ counters,
                                                   const uint16_t *__restrict__ member, const Trig *__restrict__ gtrig,
                                                   uint32_t nbatch, uint64_t cap_rows, GraphArgs ga, int tight) {
@@ -1274,15 +1275,29 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
     }
     const uint64_t nwords = (N + 63) / 64;
     if (tile0 * kWTileWords >= nwords) return;  // whole block idle
-    uint64_t grp = 0;
-    if (tile * kWTileWords < nwords)
-        grp = sweep_tile<NW, PAULI, BATCH, GRAPH>(gdv, tr, cols, colstride, sc, fold, xc, rows, coef, t_val, t_item,
-                                                  elead, tcnt, N, tile, sem[wave], cidx[wave], member, s_trig);
+    uint64_t grp = 0, p = 0;
+    double v[kWTileWords];
+    const bool live = tile * kWTileWords < nwords;
+    if (live)
+        grp = sweep_tile<NW, PAULI, BATCH, GRAPH>(gdv, tr, cols, colstride, sc, fold, xc, rows, coef, elead, N, tile,
+                                                  sem[wave], cidx[wave], member, s_trig, v, p);
     if (lane_id() == 0) s_grp[wave] = grp;
     __syncthreads();
+    uint64_t g = 0;
+    uint32_t lpre = 0, fpre = 0;  // this tile's list offsets: the block's leaders / followers in earlier tiles
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint64_t gw = s_grp[w];
+        g += gw;
+        if (static_cast<uint32_t>(w) < wave) {
+            lpre += static_cast<uint32_t>(gw & kGrpMask);
+            fpre += static_cast<uint32_t>((gw >> kGrpBits) & kGrpMask);
+        }
+    }
+    if (live && (grp & ((kGrpMask << kGrpBits) | kGrpMask)))
+        sweep_emit(sem[wave], p, v, t_item + tile0 * kWTileRows, t_val + tile0 * kWTileRows, tile * kWTileRows, lpre,
+                   static_cast<uint32_t>(g & kGrpMask) + fpre);
     if (threadIdx.x == 0) {
-        uint64_t g = 0;
-        for (int w = 0; w < kWaves; ++w) g += s_grp[w];
         bsum[blockIdx.x] = g;
         if (g) atomicAdd(reinterpret_cast<unsigned long long *>(gsum + (tile0 >> group_log2)), g);
     }
@@ -1291,8 +1306,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
 // Partner resolution of every emitter (Engine.h:364-387, 465-513) and the ordered append of the
 // partners to create (leader misses then follower misses, ascending, Engine.h:428-444).
 //   positions  every block turns the sweep's per-group counts into prefixes (LDS); list position f
-//              lies in group g (binary search), in one of its sweep blocks (the blocks' count sums) and
-//              in one of that block's four tiles (the tile counts): the tile's region holds its row
+//              lies in group g (binary search) and in one of its sweep blocks (the group itself when it
+//              is one block, else the blocks' count sums): that block's list holds its row
 //   resolution one emitter per thread, 256-emitter chunks (leader chunks, then follower chunks) by work
 //              ticket.  A follower whose partner leader emits is rotated by that leader (the
 //              reference's matched-follower skip).  A hit applies the rotation against the swept
@@ -1312,8 +1327,7 @@ template <int NW, int PAULI, int BATCH, int SCHR>
 __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_t ep,
                                                     uint64_t *__restrict__ rows, double *__restrict__ coef,
                                                     uint64_t *__restrict__ cols, uint64_t colstride, int n_modes,
-                                                    const uint32_t *__restrict__ t_item,
-                                                    const uint64_t *__restrict__ tcnt, const uint64_t *__restrict__ bsum,
+                                                    const uint32_t *__restrict__ t_item, const uint64_t *__restrict__ bsum,
                                                     const uint64_t *__restrict__ gsum, uint64_t *__restrict__ gsum_next,
                                                     uint64_t n_groups_next, const double *__restrict__ t_val,
                                                     const uint64_t *__restrict__ elead, uint64_t *__restrict__ gidx,
@@ -1424,8 +1438,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
             const uint32_t cnt = std::min<uint32_t>(kResChunk, total - first);
             const uint32_t f = first + threadIdx.x;
             const bool valid = threadIdx.x < cnt;
-            // group of f: binary search over the group prefixes (LDS); then the group's sweep-block sums
-            // and the block's tile counts (two rounds of independent loads, however sparse the emitters)
+            // group of f: binary search over the group prefixes (LDS); a group of one sweep block is that
+            // block, a larger one is searched through its blocks' count sums (a round of independent loads)
             uint32_t S = 0;
             uint64_t sidx = 0;
             double vS = 0.0;
@@ -1437,40 +1451,21 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                     else hi = mid;
                 }
                 uint32_t rem = f - gp[lo];
-                uint64_t tile = 0;
-                uint32_t nl_t = 0;
-                if (group_log2 <= 4) {
-                    // groups of at most 16 tiles: their tile counts in one round of loads
-                    uint64_t tc[16];
-                    const uint64_t t0 = static_cast<uint64_t>(lo) << group_log2;
-                    const uint32_t tpg = 1u << group_log2;
-#pragma unroll
-                    for (uint32_t k = 0; k < 16; ++k) tc[k] = (k < tpg && t0 + k < ntiles) ? tcnt[t0 + k] : 0;
-                    bool found = false;
-#pragma unroll
-                    for (uint32_t k = 0; k < 16; ++k) {
-                        if (found) continue;
-                        const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
-                        const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
-                        if (rem < own) {
-                            tile = t0 + k;
-                            nl_t = nl1;
-                            found = true;
-                        } else {
-                            rem -= own;
-                        }
-                    }
-                } else {
+                uint64_t blk = lo;
+                // the block's leaders (its followers' list starts after them)
+                uint32_t nl_b = gpl[lo + 1] - gpl[lo];
+                if (group_log2 > 2) {
+                    // groups of several sweep blocks: the block sums in rounds of independent loads
                     const uint32_t blocks_per_group = 1u << (group_log2 - 2);
                     const uint64_t b0 = static_cast<uint64_t>(lo) << (group_log2 - 2);
-                    uint64_t blk = b0;
                     for (uint32_t base = 0; base < blocks_per_group; base += kBsumBatch) {
-                        uint32_t cb[kBsumBatch];
+                        uint32_t cb[kBsumBatch], cl[kBsumBatch];
 #pragma unroll
                         for (int k = 0; k < kBsumBatch; ++k) {
                             const uint64_t bb = b0 + base + k;
-                            const uint64_t v = bb < nblocks ? bsum[bb] : 0;
-                            cb[k] = static_cast<uint32_t>(leader ? (v & kGrpMask) : ((v >> kGrpBits) & kGrpMask));
+                            const uint64_t v = (base + k < blocks_per_group && bb < nblocks) ? bsum[bb] : 0;
+                            cl[k] = static_cast<uint32_t>(v & kGrpMask);
+                            cb[k] = leader ? cl[k] : static_cast<uint32_t>((v >> kGrpBits) & kGrpMask);
                         }
                         bool found = false;
 #pragma unroll
@@ -1478,6 +1473,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                             if (found) continue;
                             if (rem < cb[k]) {
                                 blk = b0 + base + k;
+                                nl_b = cl[k];
                                 found = true;
                             } else {
                                 rem -= cb[k];
@@ -1485,29 +1481,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
                         }
                         if (found) break;
                     }
-                    uint64_t tc[kWaves];
-#pragma unroll
-                    for (int k = 0; k < kWaves; ++k) {
-                        const uint64_t tt = blk * kWaves + k;
-                        tc[k] = tt < ntiles ? tcnt[tt] : 0;
-                    }
-                    tile = blk * kWaves;
-                    bool found = false;
-#pragma unroll
-                    for (int k = 0; k < kWaves; ++k) {
-                        if (found) continue;
-                        const uint32_t nl1 = static_cast<uint32_t>(tc[k] & kCntMask);
-                        const uint32_t own = leader ? nl1 : static_cast<uint32_t>((tc[k] >> kCntBits) & kCntMask);
-                        if (rem < own) {
-                            tile = blk * kWaves + k;
-                            nl_t = nl1;
-                            found = true;
-                        } else {
-                            rem -= own;
-                        }
-                    }
                 }
-                sidx = tile * kWTileRows + (leader ? 0u : nl_t) + rem;
+                sidx = blk * (kWaves * kWTileRows) + (leader ? 0u : nl_b) + rem;
                 S = t_item[sidx];
                 vS = t_val[sidx];
             }
@@ -2360,7 +2335,7 @@ struct mpa_engine {
     // Fused pipeline state: global row index; per 1024-term sweep tile a region of emitter rows (t_item)
     // and their pre-gate values (t_val); tile, sweep-block and group counts; leader emit masks per word;
     // look-back descriptors.
-    DevArray gidx{&stream}, f_elead{&stream}, t_val{&stream}, t_item{&stream}, t_cnt{&stream}, t_bsum{&stream},
+    DevArray gidx{&stream}, f_elead{&stream}, t_val{&stream}, t_item{&stream}, t_bsum{&stream},
         t_gsum{&stream}, desc_m{&stream};
     uint64_t gsum_stride = 0;  // per gate parity
     int group_log2 = kMinGroupLog2;
@@ -2527,7 +2502,7 @@ struct mpa_engine {
     // Gives back the arrays sized by the capacity other than the store itself (the fused pipeline's,
     // the row index, the per-gate pipeline's); they are sized again by the next propagate.
     void release_work_arrays() {
-        for (DevArray *a : {&gidx, &f_elead, &t_val, &t_item, &t_cnt, &t_bsum, &t_gsum, &desc_m, &AL, &lead, &foll,
+        for (DevArray *a : {&gidx, &f_elead, &t_val, &t_item, &t_bsum, &t_gsum, &desc_m, &AL, &lead, &foll,
                             &blk_a, &blk_b})
             a->release();
         fused_cap = 0;
@@ -3322,7 +3297,6 @@ struct mpa_engine {
             cap_items = tight ? cap : cap / 2;
             const uint64_t tiles = (cap_items + kWTileRows - 1) / kWTileRows;
             t_item.reserve(tiles * kWTileRows * 4);
-            t_cnt.reserve(tiles * 8);
             t_bsum.reserve(((tiles + kWaves - 1) / kWaves + 1) * 8);
             group_log2 = min_group_log2;
             while (group_log2 < kMaxGroupLog2 && ((tiles + (uint64_t{1} << group_log2) - 1) >> group_log2) + 1 > kMaxGroups)
@@ -3575,14 +3549,14 @@ struct mpa_engine {
                                    BATCH ? static_cast<size_t>(nbatch) * 16 : 0, stream, gd, ig, trc,
                                    cols.as<uint64_t>(), colstride(), sc, fold.as<int32_t>() + gd.fold_off,
                                    rows.as<uint64_t>(), cv, t_val.as<double>(), t_item.as<uint32_t>(),
-                                   f_elead.as<uint64_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur,
+                                   f_elead.as<uint64_t>(), t_bsum.as<uint64_t>(), gsum_cur,
                                    group_log2, counters.as<uint64_t>(), mem, gtrig, nbatch, cap, ga, tight ? 1 : 0);
                 HIP_TRY(hipGetLastError());
                 prof_end(K_SWEEP, pa);
                 pa = prof_begin();
                 auto *kres = (!BATCH && schr) ? k_resolve<NW, PAULI, 0, 1> : k_resolve<NW, PAULI, BATCH, 0>;
                 hipLaunchKernelGGL(kres, dim3(rgrid), dim3(kBlock), rlds, stream, gd, ig, ep, rows.as<uint64_t>(), cv, cols.as<uint64_t>(), colstride(),
-                                   n_modes, t_item.as<uint32_t>(), t_cnt.as<uint64_t>(), t_bsum.as<uint64_t>(),
+                                   n_modes, t_item.as<uint32_t>(), t_bsum.as<uint64_t>(),
                                    gsum_cur, gsum_nxt,
                                    gsum_stride, t_val.as<double>(), f_elead.as<uint64_t>(), gidx.as<uint64_t>(), gidx_slots,
                                    desc_m.as<uint64_t>(), counters.as<uint64_t>(), stats.as<int64_t>(),
@@ -4729,7 +4703,7 @@ int mpa_memory_breakdown(mpa_engine *e, int64_t *out, int32_t n) {
         f[8] = static_cast<int64_t>((e->cap - std::min(e->cap, e->N)) * e->nw * 8);  // d_terms_slack_bytes
         f[9] = static_cast<int64_t>(e->g_rot.bytes + e->g_fold.bytes + e->g_cofs.bytes + e->base.bytes);  // graph bytes
         size_t work = 0;  // propagate / evaluation scratch the engine keeps between calls
-        for (const DevArray *a : {&e->f_elead, &e->t_val, &e->t_item, &e->t_cnt, &e->t_bsum, &e->t_gsum, &e->desc_m,
+        for (const DevArray *a : {&e->f_elead, &e->t_val, &e->t_item, &e->t_bsum, &e->t_gsum, &e->desc_m,
                                   &e->AL, &e->lead, &e->foll, &e->blk_a, &e->blk_b, &e->rec, &e->lv, &e->match,
                                   &e->miss, &e->ins, &e->seedc, &e->rep, &e->snap_c, &e->snap_r, &e->lam, &e->pmark, &e->pmark1,
                                   &e->gpartial, &e->sum_blk, &e->sum_v, &e->sum_m, &e->sum_out})

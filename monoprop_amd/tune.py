@@ -30,7 +30,7 @@ from .concurrent import run_concurrently
 
 RESOLVE_GRIDS = (64, 96, 128, 192, 256, 320)
 MIN_GROUP_LOG2 = (2, 3, 4, 5)
-DEFAULT = (128, 3)  # the engine's built-in launch parameters
+DEFAULT = (128, 2)  # the engine's built-in launch parameters
 THETA_H = (0.1, 0.4, 0.8)
 
 
