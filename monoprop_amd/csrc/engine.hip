@@ -950,12 +950,16 @@ __device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, in
     }
 }
 
-constexpr int kWTileWords = 16;                          // sweep: one wavefront per 16 column words
+// sweep: one wavefront per 16 column words.  (8 words, 512 terms, keeps more waves resident — 62
+// against 80 VGPRs — and takes 7% off the sweep when propagations run one at a time, but three
+// propagations on concurrent streams ran 1.7% slower.)
+constexpr int kWTileWords = 16;
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
-// Count groups: 2^group_log2 tiles, at least one sweep block (4 tiles) and up to 1024 tiles for the
+constexpr int kTileLog2 = kWTileWords == 16 ? 10 : kWTileWords == 8 ? 9 : 8;
+// Count groups: 2^group_log2 tiles, at least one sweep block (4 tiles) and up to 2^20 rows for the
 // largest stores, so that at most kMaxGroups prefixes sit in LDS.  A group of one block lets the resolve
 // place an emitter from the prefixes alone.
-constexpr int kMinGroupLog2 = 2, kMaxGroupLog2 = 10;
+constexpr int kMinGroupLog2 = 2, kMaxGroupLog2 = 20 - kTileLog2;
 constexpr uint32_t kMaxGroups = 4096;
 // per-group count word: leaders | followers << 21 | anticommuting << 42 (each <= 2^20)
 constexpr uint32_t kGrpBits = 21;
@@ -990,14 +994,22 @@ struct SweepCols {
     int32_t piv0, piv1;
 };
 
-// Column id of slot cc (< kColArgs) for lane slot sl (wave-uniform base `pass`): a select over the
-// by-value ids, so no memory access is needed to know which column to read.
+// Sweep lanes hold (column slot, word): 64 / kWTileWords column slots per pass over the tile's words,
+// kColPasses passes for the kColArgs by-value columns.
+constexpr int kColSlots = 64 / kWTileWords;
+constexpr int kColPasses = kColArgs / kColSlots;
+
+// Column id of slot pass * kColSlots + sl (< kColArgs) for lane slot sl (wave-uniform `pass`): a select
+// over the by-value ids, so no memory access is needed to know which column to read.
 __device__ __forceinline__ int sweep_col(const SweepCols &sc, int pass, int sl) {
-    const int b = pass * 4;
+    int c = 0;
     // the ids as values (scalar registers), so the select is over values, not over their addresses
-    const int c0 = __builtin_amdgcn_readfirstlane(sc.col[b]), c1 = __builtin_amdgcn_readfirstlane(sc.col[b + 1]);
-    const int c2 = __builtin_amdgcn_readfirstlane(sc.col[b + 2]), c3 = __builtin_amdgcn_readfirstlane(sc.col[b + 3]);
-    return sl == 0 ? c0 : sl == 1 ? c1 : sl == 2 ? c2 : c3;
+#pragma unroll
+    for (int i = 0; i < kColSlots; ++i) {
+        const int ci = __builtin_amdgcn_readfirstlane(sc.col[pass * kColSlots + i]);
+        c = sl == i ? ci : c;
+    }
+    return c;
 }
 
 // A graph build on the fused kernels (build_graph without an orbital cap): the seed vector is evolved
@@ -1067,19 +1079,18 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     const uint64_t w0 = tile * kWTileWords;
     const uint32_t lane = lane_id();
     const uint64_t row0 = tile * kWTileRows;
-    // ---- A: lanes hold (column slot s = lane / 16, word j = lane % 16)
+    // ---- A: lanes hold (column slot s = lane / kWTileWords, word j = lane % kWTileWords)
     uint64_t a = 0, p = 0;
     {
         const int j = static_cast<int>(lane & (kWTileWords - 1)), sl = static_cast<int>(lane / kWTileWords);
         const uint64_t w = w0 + j;
-        constexpr int kSlots = 64 / kWTileWords;  // 4 column slots per pass
 #pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
-            const int cc = pass * kSlots + sl;
+        for (int pass = 0; pass < kColPasses; ++pass) {
+            const int cc = pass * kColSlots + sl;
             if (cc < sc.nfold) a ^= xc[pass];
             else p ^= xc[pass];
         }
-        for (int cc0 = 2 * kSlots; cc0 < sc.ncol; cc0 += kSlots) {  // generators wider than 8 columns
+        for (int cc0 = kColArgs; cc0 < sc.ncol; cc0 += kColSlots) {  // generators wider than kColArgs columns
             const int cc = cc0 + sl;
             uint64_t y = 0;
             if (w < nwords && cc < sc.ncol) {
@@ -1207,16 +1218,16 @@ __device__ __forceinline__ void sweep_emit(const uint64_t *__restrict__ sem, uin
     }
 }
 
-// Column words of G's first kColArgs columns for tile `tile` (lane = slot * 16 + word); the tiles cover
-// the capacity, so the words are in bounds for any live count.
+// Column words of G's first kColArgs columns for tile `tile` (lane = slot * kWTileWords + word); the
+// tiles cover the capacity, so the words are in bounds for any live count.
 __device__ __forceinline__ void sweep_load_cols(const uint64_t *__restrict__ cols, uint64_t colstride,
                                                 const SweepCols &sc, uint64_t tile, uint64_t *xc) {
     const uint32_t lane = lane_id();
     const int j = static_cast<int>(lane & (kWTileWords - 1)), sl = static_cast<int>(lane / kWTileWords);
     const uint64_t w = tile * kWTileWords + j;
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-        const int cc = pass * 4 + sl;
+    for (int pass = 0; pass < kColPasses; ++pass) {
+        const int cc = pass * kColSlots + sl;
         xc[pass] = cc < sc.ncol ? cols[static_cast<uint64_t>(sweep_col(sc, pass, sl)) * colstride + w] : 0;
     }
 }
@@ -1249,7 +1260,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t tile0 = static_cast<uint64_t>(blockIdx.x) * kWaves;
     const uint64_t tile = tile0 + wave;
-    uint64_t xc[2];
+    uint64_t xc[kColPasses];
     sweep_load_cols(cols, colstride, sc, tile, xc);
     if (BATCH) {
         for (uint32_t q = threadIdx.x; q < nbatch; q += kBlock) {
