@@ -126,14 +126,29 @@ struct RunError {
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 // Inclusive scan across one wavefront of 64 lanes.
+// v of lane l - shift (DPP_CTRL) for the lanes ROWS enables, 0 where the source lies outside the row or
+// the row is not enabled: a cross-lane move of the VALU, no LDS round trip.
+template <int DPP_CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_shifted(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), DPP_CTRL, ROWS, 0xf, false));
+}
+
+// Inclusive scan across one wavefront of 64 lanes by DPP: row_shr 1, 2, 4, 8 scan each row of 16 lanes,
+// then row_bcast:15 adds the last lane of rows 0 and 2 to rows 1 and 3, and row_bcast:31 the last lane of
+// row 1 to rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= static_cast<uint32_t>(d)) v += t;
-    }
+    v += dpp_shifted<0x111, 0xf>(v);
+    v += dpp_shifted<0x112, 0xf>(v);
+    v += dpp_shifted<0x114, 0xf>(v);
+    v += dpp_shifted<0x118, 0xf>(v);
+    v += dpp_shifted<0x142, 0xa>(v);
+    v += dpp_shifted<0x143, 0xc>(v);
     return v;
+}
+
+// Lane l's 32-bit value as a wave-uniform one; l must be wave-uniform.
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
 }
 
 // Exclusive scan across a block of WAVES wavefronts; returns the exclusive prefix, total in *total.
@@ -165,12 +180,8 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds) {
 // never carry into each other); total in *total.
 __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t *lds, uint64_t *total) {
     const uint32_t lane = threadIdx.x & 63u;
-    uint64_t incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t t = __shfl_up(incl, d, 64);
-        if (lane >= static_cast<uint32_t>(d)) incl += t;
-    }
+    const uint64_t incl = static_cast<uint64_t>(wave_incl_scan(static_cast<uint32_t>(v))) |
+                          (static_cast<uint64_t>(wave_incl_scan(static_cast<uint32_t>(v >> 32))) << 32);
     const uint32_t wave = threadIdx.x >> 6;
     if (lane == 63) lds[wave] = incl;
     __syncthreads();
@@ -193,7 +204,8 @@ __device__ __forceinline__ uint64_t wave_append(bool pred, uint64_t *counter) {
     unsigned long long base = 0;
     if (lane_id() == 0 && ball)
         base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), static_cast<unsigned long long>(__popcll(ball)));
-    base = __shfl(base, 0, 64);
+    base = (static_cast<uint64_t>(lane_u32(static_cast<uint32_t>(base >> 32), 0)) << 32) |
+           lane_u32(static_cast<uint32_t>(base), 0);
     const uint32_t lane = lane_id();
     const uint64_t below = lane == 0 ? 0 : (ball & ((uint64_t{1} << lane) - 1));
     return base + static_cast<uint64_t>(__popcll(below));
@@ -831,11 +843,7 @@ __device__ __forceinline__ uint64_t load_agent(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return lane_u32(wave_incl_scan(v), 63); }
 
 // Ordered prefix sums across the tiles of one launch (single-pass scan with decoupled look-back).
 // Descriptor word: epoch (30 bits) | status (2 bits) | value (32 bits); a descriptor of another epoch
@@ -892,8 +900,8 @@ __device__ uint32_t look_back(const uint64_t *desc, uint32_t t, uint32_t ep, uin
             continue;
         }
         const int first = __ffsll(static_cast<unsigned long long>(has_stop)) - 1;
-        const int f_incl = __shfl(stop_incl ? 1 : 0, first, 64);
-        const int f_i = __shfl(stop_i, first, 64);
+        const int f_incl = static_cast<int>(lane_u32(stop_incl ? 1u : 0u, first));
+        const int f_i = static_cast<int>(lane_u32(static_cast<uint32_t>(stop_i), first));
         sum += wave_sum(static_cast<int>(lane) <= first ? part : 0u);
         if (f_incl) break;
         pos -= static_cast<int64_t>(kLookSpan) * first + f_i;  // the first tile not yet published
@@ -911,7 +919,7 @@ __device__ uint32_t look_back(const uint64_t *desc, uint32_t t, uint32_t ep, uin
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        if (!__shfl(ok, 0, 64)) break;
+        if (!lane_u32(static_cast<uint32_t>(ok), 0)) break;
     }
     return sum;
 }
@@ -1146,7 +1154,7 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
     }
     const uint32_t mine = static_cast<uint32_t>(__popc(cand));
     const uint32_t incl = wave_incl_scan(mine);
-    const uint32_t ncand = __shfl(incl, 63, 64);
+    const uint32_t ncand = lane_u32(incl, 63);
     {
         uint32_t pos = incl - mine;
 #pragma unroll
@@ -1825,7 +1833,7 @@ __device__ __forceinline__ uint64_t layer_cos_chunk(const uint64_t *__restrict__
     const uint64_t mine = layer_fold_bits(cols, colstride, lc, fold, n_before, chunk * kCosChunkWords);
     const uint32_t cnt = lane < kCosChunkWords ? static_cast<uint32_t>(__popcll(mine)) : 0u;  // lanes >= 16 repeat
     const uint32_t incl = wave_incl_scan(cnt);
-    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t total = lane_u32(incl, 63);
     if (total == 0) return 0;
     {
         uint32_t pos = incl - cnt;
