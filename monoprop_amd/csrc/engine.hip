@@ -177,23 +177,32 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds) {
 }
 
 // Exclusive scan of 64-bit values across a block of kWaves wavefronts (two packed 32-bit counts that
-// never carry into each other); total in *total.
-__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t *lds, uint64_t *total) {
+// never carry into each other), total in *total, and with the same two barriers the block's sum of a
+// (in *total_a).
+__device__ __forceinline__ uint64_t block_excl_scan64_sum(uint64_t v, uint32_t a, uint64_t *lds, uint32_t *lds_a,
+                                                          uint64_t *total, uint32_t *total_a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t incl = static_cast<uint64_t>(wave_incl_scan(static_cast<uint32_t>(v))) |
                           (static_cast<uint64_t>(wave_incl_scan(static_cast<uint32_t>(v >> 32))) << 32);
+    const uint32_t wa = lane_u32(wave_incl_scan(a), 63);
     const uint32_t wave = threadIdx.x >> 6;
-    if (lane == 63) lds[wave] = incl;
+    if (lane == 63) {
+        lds[wave] = incl;
+        lds_a[wave] = wa;
+    }
     __syncthreads();
     uint64_t wave_base = 0, tot = 0;
+    uint32_t tot_a = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
         const uint64_t s = lds[w];
         if (static_cast<uint32_t>(w) < wave) wave_base += s;
         tot += s;
+        tot_a += lds_a[w];
     }
     __syncthreads();
     *total = tot;
+    *total_a = tot_a;
     return wave_base + incl - v;
 }
 
@@ -1404,8 +1413,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
     }
     if (threadIdx.x == 0) s_first = static_cast<uint32_t>(first_ticket);
     uint64_t tot;
-    uint64_t x = block_excl_scan64(mine, lds64, &tot);  // (its barriers also publish s_first)
-    const uint32_t nA = block_sum(mine_a, lds);
+    uint32_t nA;
+    uint64_t x = block_excl_scan64_sum(mine, mine_a, lds64, lds, &tot, &nA);  // (its barriers also publish s_first)
     const uint32_t nEL = static_cast<uint32_t>(tot), nEF = static_cast<uint32_t>(tot >> 32);
 #pragma unroll
     for (uint32_t i = 0; i < kMaxGroups / kBlock; ++i) {
@@ -1438,13 +1447,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve(GateDesc gd, int gi, uint32_
     uint32_t pend = 0xffffffffu, pend_miss = 0, buf = 0;
     for (bool first = true;; first = false) {
         __syncthreads();  // LDS of the previous iteration fully consumed (and the group prefixes written)
-        if (threadIdx.x == 0)
-            s_ticket = first ? ticket0
-                       : !more ? 0xffffffffu
-                               : static_cast<uint32_t>(atomicAdd(
-                                     reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
-        __syncthreads();
-        const uint32_t chunk = s_ticket;
+        uint32_t chunk = ticket0;  // the first chunk's ticket is known to every thread
+        if (!first) {
+            if (threadIdx.x == 0)
+                s_ticket = !more ? 0xffffffffu
+                                 : static_cast<uint32_t>(atomicAdd(
+                                       reinterpret_cast<unsigned long long *>(counters + C_TK_RES), 1ull));
+            __syncthreads();
+            chunk = s_ticket;
+        }
         const bool have = chunk < nchunks;
         if (!have && pend == 0xffffffffu) break;
         uint32_t n_miss = 0;
