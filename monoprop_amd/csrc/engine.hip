@@ -1327,7 +1327,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr
                    static_cast<uint32_t>(g & kGrpMask) + fpre);
     if (threadIdx.x == 0) {
         bsum[blockIdx.x] = g;
-        if (g) atomicAdd(reinterpret_cast<unsigned long long *>(gsum + (tile0 >> group_log2)), g);
+        // a group of one block has this block as its only writer: a store; larger groups add up
+        if (g) {
+            if (group_log2 == 2) gsum[blockIdx.x] = g;
+            else atomicAdd(reinterpret_cast<unsigned long long *>(gsum + (tile0 >> group_log2)), g);
+        }
     }
 }
 
