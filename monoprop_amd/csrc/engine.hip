@@ -3368,11 +3368,6 @@ struct mpa_engine {
         }
     }
 
-    // Algorithmic bytes of the fused kernels from the per-gate counters.  Sweep: fold + pivot column
-    // words, leader masks, tile counts, the anticommuting coefficients (read + write), per emitter its
-    // row, pre-gate value and tile entry.  Resolve: per emitter its tile entry,
-    // pre-gate value, row, index slot (key inline) and coefficient writes; per hit the partner
-    // coefficient (read + write); per appended row the row, coefficient, index slot and column bits.
     // Algorithmic bytes of the fused kernels per gate g, the per-unit figure of the propagate path's
     // measurement definition: the anticommutation test reads, for every live term, the limbs that can
     // meet supp(G) (lb bytes per limb unit, w_g units); each anticommuting coefficient is read and

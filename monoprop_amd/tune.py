@@ -4,9 +4,9 @@ Times the 10x10 kicked-Ising step (the three theta_h propagations, cutoff 10, lo
 once on their own engines and streams, the benchmark's schedule) for every pair of
 
 * ``resolve_grid``: k_resolve blocks that take emitter chunks, and
-* ``min_group_log2``: the smallest count group (2^g sweep tiles), which sets how many group prefixes
-  every resolve block scans against how many tile counts (groups of up to 16 tiles) or sweep-block
-  sums (larger groups) an emitter reads to find its tile,
+* ``min_group_log2``: the smallest count group (2^g sweep tiles of 1024 terms), which sets how many
+  group prefixes every resolve block scans against whether an emitter's sweep block follows from the
+  prefixes alone (one-block groups, g = 2) or from a round of sweep-block sums (larger groups),
 
 checks that every pair gives the same <O> to the bit (the results do not depend on launch
 parameters), and writes the table and the fastest pair to ``tuning.json`` next to the library, which
