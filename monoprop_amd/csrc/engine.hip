@@ -11,7 +11,7 @@
 // FusedApply.h:32-83), all on one stream.  The default (fused) pipeline is two launches per gate with no
 // host synchronisation between gates:
 //   k_sweep      anticommutation over G's columns, c <- c cos 2a in place, the emission gate, emitters
-//                listed per 1024-term tile with their pre-gate values
+//                listed per 4096-term sweep block (leaders, then followers) with their pre-gate values
 //   k_resolve    every emitter's partner looked up in a keyless global row index, the rotation applied,
 //                misses appended in the reference's order by a single-pass chained scan
 // A graph build runs the same two (the sweep's graph instance leaves the coefficients unscaled, the
@@ -34,8 +34,9 @@
 //                follower misses, Engine.h:428-444) appended at N..N+I
 //   k_cols_update  wave64 ballots transpose the new rows into the column index
 //   k_advance    N += I and the per-gate counters
-// Floating point follows the reference's operation order; the file is compiled with
-// -ffp-contract=off so nothing is fused into FMA.
+// Wave-level scans and sums use DPP row shifts and broadcasts (no LDS permutes).  Floating point follows
+// the reference's operation order; the file is compiled with -ffp-contract=off so nothing is fused into
+// FMA.
 
 #include <hip/hip_runtime.h>
 
