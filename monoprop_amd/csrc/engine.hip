@@ -968,9 +968,9 @@ __device__ __forceinline__ void gather_column_bits(uint64_t *lcol, int ncols, in
     }
 }
 
-// sweep: one wavefront per 16 column words.  (8 words, 512 terms, keeps more waves resident — 62
-// against 80 VGPRs — and takes 7% off the sweep when propagations run one at a time, but three
-// propagations on concurrent streams ran 1.7% slower.)
+// sweep: one wavefront per 16 column words.  (8 words, 512 terms, keeps more waves resident and takes
+// 4-7% off the sweep when propagations run one at a time, but three propagations on concurrent streams
+// ran 2% slower, also when only large stores used it.)
 constexpr int kWTileWords = 16;
 constexpr int kWTileRows = kWTileWords * 64;              // = 1024 terms
 constexpr int kTileLog2 = kWTileWords == 16 ? 10 : kWTileWords == 8 ? 9 : 8;
@@ -1251,11 +1251,11 @@ __device__ __forceinline__ void sweep_load_cols(const uint64_t *__restrict__ col
 }
 
 // One 1024-term tile per wavefront, four per block; the block stores its tiles' summed counts
-// (bsum[block]) and adds them into their group word with one atomic, and lists its emitters in its
-// region of t_item / t_val (block b: entries [4096 b, 4096 b + its emitters)), so that the resolve
-// finds emitter f of a block from the group prefixes and the block sums alone.  The live count, the stop flag,
-// the tile's column words and (BATCH) the per-member constants are all requested before anything
-// waits on any of them.  (A persistent grid of resident blocks walking the tiles, with the next
+// (bsum[block]) and adds them into their group word (a plain store when the group is this block), and
+// lists its emitters in its region of t_item / t_val (block b: entries [4096 b, 4096 b + its
+// emitters)), so that the resolve finds emitter f of a block from the group prefixes and the block sums
+// alone.  The live count, the stop flag, the tile's column words and (BATCH) the per-member constants
+// are all requested before anything waits on any of them.  (A persistent grid of resident blocks walking the tiles, with the next
 // tiles' columns prefetched, was measured slower: more registers, and a barrier per tile set.)
 template <int NW, int PAULI, int BATCH, int GRAPH>
 __global__ __launch_bounds__(kBlock) void k_sweep(GateDesc gdv, int gi, Trunc tr, const uint64_t *__restrict__ cols,
