@@ -16,7 +16,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libmonoprop_amd.so"
 SOURCES = [CSRC / "engine.hip"]
-HEADERS = [CSRC / "algebra.h", HERE.parent / "include" / "monoprop_amd.h"]
+HEADERS = [CSRC / h for h in ("algebra.h", "device_common.h", "kernels_pergate.h", "kernels_fused.h",
+                              "kernels_reduce_graph.h", "engine_host.h")] + [HERE.parent / "include" / "monoprop_amd.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
