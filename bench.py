@@ -18,7 +18,9 @@ after another, so that each launch's duration is its own).  Algorithmic bytes pe
 are the path's measurement definition: the sweep reads 16 B of x/z limbs per live term (per limb pair
 that meets the generator), reads and writes each anticommuting coefficient, reads the anticommuting
 terms' other limbs and writes each emitted partner record (row + value); the resolve reads those records
-back and appends each new term.  `hbm_GBps` is what the kernel really moves (PMC counters).  `gradient` reports the
+back and appends each new term.  `hbm_GBps` is what the kernel really moves (PMC counters).
+`per_propagation` lists the engines of the last timed step (theta_h, term-gate applications, device
+time of the gate loop): the step lasts at least as long as the largest.  `gradient` reports the
 other half of the metric, the full-angle gradient's time against one forward replay, on the 64-qubit
 ring brickwork (2304 angles), measured after the timed steps; `batch` times a parameter sweep (sets x
 eight observables, 10 Trotter steps) batched against one propagation per pair.  The CPU baseline is the oracle
@@ -501,14 +503,25 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         total, evs = 0, None
-        for _ in range(args.steps):
+        for i in range(args.steps):
             t, evs, sims = step_fn(args.lattice, inputs, profile=False)
             total += t
+            if i == args.steps - 1:
+                last[:] = sims  # the last step's engines, read after the timed region
             del sims
         torch.cuda.synchronize()
         return total, evs, time.perf_counter() - t0
 
+    last: list = []
     total_t, evs, elapsed = timed(step)
+    # each engine of the last timed step: its term-gate applications and device time (HIP events around
+    # its gate loop)
+    per_propagation = [{"term_gate_applications": int(sim.gate_stats()[0].sum()),
+                        "device_ms": round(sim.last_propagate_ms(), 2)} for sim in last]
+    if len(per_propagation) == len(THETA_H):  # one engine per theta_h (not the batched store)
+        for rec, th in zip(per_propagation, THETA_H):
+            rec["theta_h"] = th
+    last.clear()
 
     # per-kernel device time (HIP events around every launch) and algorithmic bytes, from one more step
     # of the same workload run with the profiler on (it is kept out of the timed steps above)
@@ -591,6 +604,8 @@ def main():
             for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
         },
         "kernel_ms_per_step": round(kern_ms, 3),
+        # the engines of the last timed step: the step lasts at least as long as the largest of them
+        "per_propagation": per_propagation,
         "propagate_merge": propagate_merge_fractions(prof),
         # the whole device over the timed step: one step's algorithmic bytes and PMC traffic of the sweep
         # and resolve launches (from the profiled step: the same gates) over the timed wall time per step,
@@ -605,6 +620,7 @@ def main():
             if other == mode:
                 continue
             t_u, evs_u, el_u = timed(fn)
+            last.clear()
             line["schedules"][other] = {"value": t_u / el_u, "ms_per_step": el_u * 1e3 / args.steps,
                                         "same_expectation_bits": [float(a).hex() == float(b).hex()
                                                                   for a, b in zip(evs, evs_u)]}
