@@ -66,7 +66,7 @@ def parse_args():
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--batch-sets", type=int, default=4, help="parameter sets of the batched sweep (0: skip)")
     ap.add_argument("--batch-no-solo", action="store_true", help="skip the one-propagation-per-pair comparison")
-    ap.add_argument("--batch-workers", type=int, default=4, help="sweep drivers running at once")
+    ap.add_argument("--batch-workers", type=int, default=8, help="sweep drivers running at once")
     ap.add_argument("--only-batch", action="store_true", help="run and print only the batched sweep")
     ap.add_argument("--only-gradient", action="store_true", help="run and print only the gradient measurement")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the known-size copy (counter scale)")
