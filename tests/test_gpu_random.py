@@ -2,7 +2,9 @@
 (store, coefficients, per-gate counters and <O> to the bit), the graph build on the fused kernels
 (store and layers, the replayed coefficient vector to the bit, <O> to the bit, the adjoint gradient to
 rounding) and, for Schrodinger cases, the evolved state.  Each seed draws the basis, width, generator
-weights and coefficients, angles, truncation (lower / upper atol, cutoff) and initial state."""
+weights and coefficients, angles, truncation (lower / upper atol, cutoff) and initial state.  The
+Heisenberg circuits are also propagated as batches of several parameter vectors, member by member
+against the oracle."""
 
 from __future__ import annotations
 
@@ -92,3 +94,32 @@ def test_random_circuit_all_paths(oracle_mod, seed):
     vd, gd = d.expectation_value_and_gradient(x)
     assert vo == vd
     assert np.allclose(gd, go, rtol=1e-9, atol=1e-13)
+
+
+@pytest.mark.parametrize("seed", [s for s in range(40) if s % 5 != 4][:30])
+def test_random_circuit_batched(oracle_mod, seed):
+    """The same circuits propagated as one batch of 2-5 members, each at its own parameter vector: every
+    member's rows, coefficients and <O> equal its own oracle propagation to the bit."""
+    from monoprop_amd._engine import BatchSimulator
+
+    c = _case(seed)
+    rng = np.random.default_rng(7000 + seed)
+    circ, n = c["circ"], c["n"]
+    k = int(rng.integers(2, 6))
+    print(f"case: random batched seed={seed} {c['basis']} n={n} members={k}", flush=True)
+    monos, gcs, pm, _ = expand_monomials(circ.gates, circ.resolved_mapping, n)
+    params = np.asarray([np.asarray(circ.parameters) * float(rng.uniform(0.5, 1.5)) + float(rng.uniform(-0.3, 0.3))
+                         for _ in range(k)], dtype=np.float64)
+    d = BatchSimulator([c["obs"]] * k, c["cutoff"], list(c["state"]), lower_atol=c["lower"], upper_atol=c["upper"],
+                       cutoff_type=c["cutoff_type"], logical_num_modes=n, basis=c["basis"])
+    d.propagate(monos, pm, gcs, params)
+    w, co, member = d.rows()
+    evs = d.expectation_values()
+    for q in range(k):
+        o = oracle_mod.Oracle(c["obs"], n, basis=c["basis"], cutoff=c["cutoff"], cutoff_type=c["cutoff_type"],
+                              lower_atol=c["lower"], upper_atol=c["upper"], initial_state=c["state"])
+        o.propagate(monos, pm, gcs, list(params[q]))
+        wo, cq = o.rows()
+        sel = member == q
+        assert np.array_equal(wo, w[sel]) and np.array_equal(cq, co[sel])
+        assert o.expectation() == evs[q]
