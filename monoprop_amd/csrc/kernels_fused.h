@@ -404,7 +404,10 @@ __device__ __forceinline__ uint64_t sweep_tile(const GateDesc &gd, const Trunc &
             const uint32_t qm = BATCH ? ((q[BATCH ? j : 0] >> (16 * (lane & 1u))) & 0xffffu) : 0u;  // r & 1 == lane & 1
             const double k = BATCH ? s_trig[2 * qm] : gd.k;
             const double abs_s = BATCH ? s_trig[2 * qm + 1] : gd.abs_s;
-            if (!GRAPH) coef[r] = v[j] * k;  // a graph build scales after its resolve (k_build_cos)
+            if (!GRAPH) {  // a graph build scales after its resolve (k_build_cos)
+                const double nv = v[j] * k;
+                if (__double_as_longlong(nv) != __double_as_longlong(v[j])) coef[r] = nv;  // unchanged bits: no store
+            }
             if (!(tr.check_lower && abs_s * fabs(v[j]) <= tr.lower)) {
                 cand |= 1u << j;
                 if (tr.check_upper && abs_s * fabs(v[j]) >= tr.upper) resc |= 1u << j;
