@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import atexit
 import ctypes as C
+import itertools
 import json
 import math
 import os
@@ -193,16 +194,21 @@ def _ptr(a: np.ndarray, ctype):
 
 
 def _csr(tuples: Sequence[Sequence[int]]):
-    offsets = np.zeros(len(tuples) + 1, dtype=np.int64)
-    flat: list[int] = []
-    for k, t in enumerate(tuples):
-        for i in t:
-            i = int(i)
-            if i < 0:
-                raise ValueError(f"Majorana/Pauli index {i} is out of range; must be non-negative.")
-            flat.append(i)
-        offsets[k + 1] = len(flat)
-    return offsets, np.asarray(flat if flat else [0], dtype=np.int32)
+    """Offsets (len + 1, int64) and concatenated indices (int32) of a list of index tuples.  Built with
+    numpy iterators: a propagate call converts its whole gate list (thousands of monomials) while
+    holding the interpreter lock, which propagations running side by side on other threads wait for."""
+    n = len(tuples)
+    offsets = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.fromiter(map(len, tuples), dtype=np.int64, count=n), out=offsets[1:])
+    total = int(offsets[-1])
+    if total == 0:
+        return offsets, np.zeros(1, dtype=np.int32)
+    flat = np.fromiter(itertools.chain.from_iterable(tuples), dtype=np.int64, count=total)
+    if flat.min() < 0:
+        raise ValueError(f"Majorana/Pauli index {int(flat[flat < 0][0])} is out of range; must be non-negative.")
+    if flat.max() > np.iinfo(np.int32).max:
+        raise ValueError(f"Majorana/Pauli index {int(flat.max())} is out of range.")
+    return offsets, flat.astype(np.int32)
 
 
 def _f64(xs) -> np.ndarray:
@@ -326,7 +332,8 @@ class DeviceSimulator(_EngineHandle):
                 f"entry per monomial ({len(majoranas)})."
             )
         offs, slots = _csr(majoranas)
-        pm = np.asarray([int(p) for p in parameter_mapping] or [0], dtype=np.int64)
+        pm = (np.fromiter(parameter_mapping, dtype=np.int64, count=len(parameter_mapping))
+              if len(parameter_mapping) else np.zeros(1, dtype=np.int64))
         gc = _f64(gen_coeffs)
         return offs, slots, pm, gc
 
